@@ -1,0 +1,138 @@
+"""Synthetic SPD test problems in CSR form (int32 indices, float64 values).
+
+The reference ships no generators (its tests read downloaded SuiteSparse
+files), so these are this package's own.  Every generator returns
+``(row_ptr, col_ind, val, m)`` with rows ordered lexicographically
+(x fastest) and columns ascending within each row.
+
+* ``poisson2d`` / ``poisson3d``: 5- and 7-point Laplacians with Dirichlet
+  boundaries by truncation (diagonal 4 / 6, off-diagonals -1).
+* ``diffusion3d``: 7-point finite-volume diffusion with per-face
+  conductances; ``cell_kappa`` gives harmonic face averaging
+  ``2 k_p k_q / (k_p + k_q)`` and Dirichlet faces use ``k_p``.
+"""
+import numpy as np
+
+
+def poisson2d(nx, ny=None):
+    ny = nx if ny is None else ny
+    return _grid_laplacian((nx, ny, 1), dim=2)
+
+
+def poisson3d(nx, ny=None, nz=None):
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    return _grid_laplacian((nx, ny, nz), dim=3)
+
+
+def _neighbours(shape, dim):
+    """Ordered (axis, step) list so that column offsets are ascending."""
+    nx, ny, nz = shape
+    out = []
+    if dim == 3:
+        out.append((2, -1))
+    out.append((1, -1))
+    out.append((0, -1))
+    out.append(None)  # diagonal
+    out.append((0, 1))
+    out.append((1, 1))
+    if dim == 3:
+        out.append((2, 1))
+    return out
+
+
+def _grid_laplacian(shape, dim, face_k=None, diag_extra=None):
+    """Assemble a 5/7-point operator.
+
+    ``face_k`` maps (axis, step) -> conductance array of grid shape (k of the
+    face between a cell and its neighbour in that direction; for a missing
+    neighbour it is the Dirichlet face conductance).  None means unit
+    conductances with a Dirichlet diagonal equal to the full stencil weight,
+    i.e. the constant-coefficient Laplacian.
+    """
+    nx, ny, nz = shape
+    m = nx * ny * nz
+    idx = np.arange(m, dtype=np.int64).reshape(nz, ny, nx)
+    k3, j3, i3 = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    coords = {0: i3, 1: j3, 2: k3}
+    sizes = {0: nx, 1: ny, 2: nz}
+    strides = {0: 1, 1: nx, 2: nx * ny}
+
+    nbrs = _neighbours(shape, dim)
+    cols = []
+    vals = []
+    present = []
+    diag = np.zeros((nz, ny, nx), dtype=np.float64)
+    for nb in nbrs:
+        if nb is None:
+            continue
+        ax, st = nb
+        c = coords[ax]
+        ok = (c + st >= 0) & (c + st < sizes[ax])
+        kf = np.ones((nz, ny, nx)) if face_k is None else face_k[(ax, st)]
+        diag += kf  # every face (interior or Dirichlet) adds to the diagonal
+        cols.append(idx + st * strides[ax])
+        vals.append(-kf)
+        present.append(ok)
+    if face_k is None:
+        diag[...] = 2.0 * dim
+    if diag_extra is not None:
+        diag += diag_extra
+
+    # interleave in ascending column order with the diagonal in the middle
+    order = []
+    half = len(cols) // 2
+    for t in range(half):
+        order.append((cols[t], vals[t], present[t]))
+    order.append((idx, diag, np.ones_like(present[0])))
+    for t in range(half, len(cols)):
+        order.append((cols[t], vals[t], present[t]))
+
+    C = np.stack([o[0].reshape(-1) for o in order], axis=1)
+    V = np.stack([o[1].reshape(-1) for o in order], axis=1)
+    P = np.stack([o[2].reshape(-1) for o in order], axis=1)
+    counts = P.sum(axis=1)
+    row_ptr = np.zeros(m + 1, dtype=np.int64)
+    np.cumsum(counts, out=row_ptr[1:])
+    col_ind = C[P].astype(np.int32)
+    val = V[P].astype(np.float64)
+    return row_ptr.astype(np.int32), col_ind, val, m
+
+
+def diffusion3d(nx, cell_kappa=None, face_kappa=None):
+    """Variable-coefficient 7-point FV diffusion on an nx^3 grid.
+
+    Exactly one of ``cell_kappa`` (grid-shaped cell conductivities; faces get
+    harmonic means, Dirichlet faces the cell value) or ``face_kappa`` (dict
+    (axis, step) -> grid-shaped face conductance) must be given.
+    """
+    shape = (nx, nx, nx)
+    if face_kappa is None:
+        k = np.asarray(cell_kappa, dtype=np.float64).reshape(nx, nx, nx)
+        face_kappa = {}
+        for ax_np, ax in ((2, 0), (1, 1), (0, 2)):
+            for st in (-1, 1):
+                nb = np.roll(k, -st, axis=ax_np)
+                h = 2.0 * k * nb / (k + nb)
+                # boundary faces (rolled-around neighbour) use the cell value
+                sl = [slice(None)] * 3
+                sl[ax_np] = -1 if st == 1 else 0
+                h[tuple(sl)] = k[tuple(sl)]
+                face_kappa[(ax, st)] = h
+    return _grid_laplacian(shape, 3, face_k=face_kappa)
+
+
+def rhs_from_solution(row_ptr, col_ind, val, x):
+    """b = A x in float64 (row-sequential sums, like the reference SpMV)."""
+    m = len(row_ptr) - 1
+    rows = np.repeat(np.arange(m), np.diff(row_ptr))
+    prod = val * x[col_ind]
+    return np.bincount(rows, weights=prod, minlength=m)
+
+
+def uniform(n, seed, lo=-1.0, hi=1.0):
+    return np.random.default_rng(seed).uniform(lo, hi, n)
+
+
+def normal(n, seed):
+    return np.random.default_rng(seed).standard_normal(n)

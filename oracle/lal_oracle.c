@@ -1,0 +1,900 @@
+/*
+ * CPU oracle for the SA-AMG-PCG hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of LinAlgLib's host algorithms, used by tests/ and by
+ * bench.py's cpu_baseline leg as the checker.  The product path (the HIP
+ * library under linalglib_amd/) never links, loads or calls this file.
+ *
+ * Reference citations are relative to the LinAlgLib source tree.  Build flags
+ * (oracle/Makefile): -O3 -fopenmp -ffp-contract=off, no -march, so that every
+ * product and sum is rounded exactly as the reference's host build rounds it.
+ * Only explicit std::fma in the reference becomes fma() here.
+ */
+#include "lal_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+void or_set_threads(int n)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+void or_csr_free(or_csr* A)
+{
+    if(!A)
+        return;
+    free(A->row_ptr);
+    free(A->col_ind);
+    free(A->val);
+    A->row_ptr = NULL;
+    A->col_ind = NULL;
+    A->val     = NULL;
+    A->m = A->n = A->nnz = 0;
+}
+
+static void csr_alloc(or_csr* A, int m, int n, int nnz)
+{
+    A->m       = m;
+    A->n       = n;
+    A->nnz     = nnz;
+    A->row_ptr = (int*)calloc((size_t)m + 1, sizeof(int));
+    A->col_ind = (int*)malloc(sizeof(int) * (size_t)(nnz > 0 ? nnz : 1));
+    A->val     = (double*)malloc(sizeof(double) * (size_t)(nnz > 0 ? nnz : 1));
+}
+
+/* ------------------------------------------------------------------------- */
+/* BLAS-1 and SpMV                                                           */
+/* ------------------------------------------------------------------------- */
+
+/* library/src/backend/host/host_matrix_vector.cpp:59-84 (host_csrmv_impl):
+ * s = fma(val, x[col], s) in CSR order from s = 0.0, then
+ * y = beta == 0 ? alpha*s : fma(alpha, s, beta*y). */
+void or_csrmv(int m, int n, int nnz, double alpha, const int* rp, const int* ci,
+              const double* v, const double* x, double beta, double* y)
+{
+    (void)n;
+    (void)nnz;
+#pragma omp parallel for schedule(dynamic, 1024)
+    for(int i = 0; i < m; i++)
+    {
+        double s = 0.0;
+        for(int j = rp[i]; j < rp[i + 1]; j++)
+            s = fma(v[j], x[ci[j]], s);
+        y[i] = beta == 0.0 ? alpha * s : fma(alpha, s, beta * y[i]);
+    }
+}
+
+/* host_matrix_vector.cpp:35-57 (host_compute_residual_impl): s += v*x (two
+ * roundings), r = b - s. */
+void or_residual(int m, const int* rp, const int* ci, const double* v, const double* x,
+                 const double* b, double* r)
+{
+#pragma omp parallel for schedule(dynamic, 1024)
+    for(int i = 0; i < m; i++)
+    {
+        double s = 0.0;
+        for(int j = rp[i]; j < rp[i + 1]; j++)
+            s += v[j] * x[ci[j]];
+        r[i] = b[i] - s;
+    }
+}
+
+/* host_axpy.cpp:84-97 (host_dot_product_impl): OpenMP reduction(+); with one
+ * thread this is the sequential left fold. */
+double or_dot(int n, const double* x, const double* y)
+{
+    double s = 0.0;
+#pragma omp parallel for reduction(+ : s)
+    for(int i = 0; i < n; i++)
+        s += x[i] * y[i];
+    return s;
+}
+
+/* host_math.cpp:35-50 (host_norm_inf_impl): max |x_i| from 0. */
+double or_norm_inf(int n, const double* x)
+{
+    double s = 0.0;
+#pragma omp parallel for reduction(max : s)
+    for(int i = 0; i < n; i++)
+    {
+        double a = fabs(x[i]);
+        s        = (a < s) ? s : a;
+    }
+    return s;
+}
+
+/* host_axpy.cpp:34-45 (host_axpy_impl): alpha == 1 skips the multiply. */
+void or_axpy(int n, double alpha, const double* x, double* y)
+{
+#pragma omp parallel for schedule(dynamic, 1024)
+    for(int i = 0; i < n; i++)
+        y[i] = alpha == 1.0 ? x[i] + y[i] : alpha * x[i] + y[i];
+}
+
+/* host_axpy.cpp:47-69 (host_axpby_impl). */
+void or_axpby(int n, double alpha, const double* x, double beta, double* y)
+{
+#pragma omp parallel for schedule(dynamic, 1024)
+    for(int i = 0; i < n; i++)
+    {
+        if(alpha == 0.0)
+            y[i] = beta * y[i];
+        else if(beta == 0.0)
+            y[i] = alpha * x[i];
+        else
+            y[i] = alpha * x[i] + beta * y[i];
+    }
+}
+
+/* host_extract.cpp:34-52 (host_diagonal_impl): first diagonal entry of the
+ * row; d_i is left untouched when the row has none. */
+void or_diagonal(int m, const int* rp, const int* ci, const double* v, double* d)
+{
+    for(int i = 0; i < m; i++)
+        for(int j = rp[i]; j < rp[i + 1]; j++)
+            if(ci[j] == i)
+            {
+                d[i] = v[j];
+                break;
+            }
+}
+
+/* host_primitives.cpp:76-91 (host_exclusive_scan_impl). */
+void or_exclusive_scan_i64(int n, int64_t* x)
+{
+    int64_t s = 0;
+    for(int i = 0; i < n; i++)
+    {
+        int64_t t = x[i];
+        x[i]      = s;
+        s += t;
+    }
+}
+
+void or_exclusive_scan_i32(int n, int32_t* x)
+{
+    int32_t s = 0;
+    for(int i = 0; i < n; i++)
+    {
+        int32_t t = x[i];
+        x[i]      = s;
+        s += t;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* SA-AMG setup                                                              */
+/* ------------------------------------------------------------------------- */
+
+/* host_amg_strength.cpp:33-70: diag from extract_diagonal into a zeroed
+ * vector; conn[j] = (c != i) && |a_ij| >= eps*sqrt(|d_i|*|d_c|). */
+void or_strength(int m, const int* rp, const int* ci, const double* v, double eps, int* conn)
+{
+    double* d = (double*)calloc((size_t)m + 1, sizeof(double));
+    or_diagonal(m, rp, ci, v, d);
+    for(int i = 0; i < m; i++)
+        for(int j = rp[i]; j < rp[i + 1]; j++)
+        {
+            int c   = ci[j];
+            conn[j] = (c != i) && (fabs(v[j]) >= eps * sqrt(fabs(d[i]) * fabs(d[c])));
+        }
+    free(d);
+}
+
+/* host_amg_aggregation.cpp:32-38. */
+unsigned int or_hash1(unsigned int x)
+{
+    x = ((x >> 16) ^ x) * 0x45d9f3bu;
+    x = ((x >> 16) ^ x) * 0x45d9f3bu;
+    x = (x >> 16) ^ x;
+    return x / 2;
+}
+
+/* host_amg_aggregation.cpp:40-68. */
+void or_pmis_init(int m, const int* rp, const int* conn, int* state, int* hash)
+{
+    for(int i = 0; i < m; i++)
+    {
+        int s = -2;
+        for(int j = rp[i]; j < rp[i + 1]; j++)
+            if(conn[j] == 1)
+            {
+                s = 0;
+                break;
+            }
+        state[i] = s;
+        hash[i]  = (int)or_hash1((unsigned int)i);
+    }
+}
+
+/* host_amg_aggregation.cpp:77-93: candidate wins on strictly greater state,
+ * or equal state and strictly greater hash. */
+static void lex_max(int* ms, int* mh, int* mr, int s, int h, int r)
+{
+    if(s > *ms || (s == *ms && h > *mh))
+    {
+        *ms = s;
+        *mh = h;
+        *mr = r;
+    }
+}
+
+/* host_amg_aggregation.cpp:95-180: one PMIS round, reading `state` and
+ * writing max_state / aggregates. */
+void or_pmis_round(int m, const int* rp, const int* ci, const int* conn, const int* state,
+                   const int* hash, int64_t* aggregates, int* max_state, int* complete)
+{
+    for(int i = 0; i < m; i++)
+    {
+        int ms = state[i], mh = hash[i], mr = i;
+        for(int j = rp[i]; j < rp[i + 1]; j++)
+            if(conn[j] == 1)
+            {
+                int c = ci[j];
+                lex_max(&ms, &mh, &mr, state[c], hash[c], c);
+            }
+        int r2 = mr;
+        for(int j = rp[r2]; j < rp[r2 + 1]; j++)
+            if(conn[j] == 1)
+            {
+                int c = ci[j];
+                lex_max(&ms, &mh, &mr, state[c], hash[c], c);
+            }
+        if(state[i] == 0)
+        {
+            if(mr == i)
+            {
+                max_state[i]  = 1;
+                aggregates[i] = 1;
+            }
+            else if(ms == 1)
+            {
+                max_state[i]  = -1;
+                aggregates[i] = 0;
+            }
+            else
+                *complete = 0;
+        }
+    }
+}
+
+/* host_amg_aggregation.cpp:182-221. */
+void or_pmis_join(int m, const int* rp, const int* ci, const int* conn, const int* state,
+                  int64_t* aggregates, int* max_state)
+{
+    for(int i = 0; i < m; i++)
+    {
+        if(state[i] == -1)
+        {
+            for(int j = rp[i]; j < rp[i + 1]; j++)
+                if(conn[j] == 1)
+                {
+                    int c = ci[j];
+                    if(state[c] == 1)
+                    {
+                        aggregates[i] = aggregates[c];
+                        max_state[i]  = 1;
+                        break;
+                    }
+                }
+        }
+        else if(state[i] == -2)
+            aggregates[i] = -2;
+    }
+}
+
+/* amg_aggregation.cpp:106-215 (compute_aggregates_using_pmis). `aggregates`
+ * must arrive zeroed, as saamg.cpp:242-245 leaves it. */
+void or_aggregates_pmis(int m, const int* rp, const int* ci, const int* conn,
+                        int64_t* aggregates, int64_t* root_nodes)
+{
+    int* hash      = (int*)calloc((size_t)m + 1, sizeof(int));
+    int* state     = (int*)calloc((size_t)m + 1, sizeof(int));
+    int* max_state = (int*)calloc((size_t)m + 1, sizeof(int));
+
+    or_pmis_init(m, rp, conn, max_state, hash);
+    for(int iter = 0; iter < 20; iter++)
+    {
+        memcpy(state, max_state, sizeof(int) * (size_t)m);
+        int complete = 1;
+        or_pmis_round(m, rp, ci, conn, state, hash, aggregates, max_state, &complete);
+        if(complete)
+            break;
+    }
+    if(root_nodes)
+        for(int i = 0; i < m; i++)
+            root_nodes[i] = (aggregates[i] == 1) ? 1 : -1;
+
+    or_exclusive_scan_i64(m, aggregates);
+
+    for(int k = 0; k < 2; k++)
+    {
+        memcpy(state, max_state, sizeof(int) * (size_t)m);
+        or_pmis_join(m, rp, ci, conn, state, aggregates, max_state);
+    }
+    free(hash);
+    free(state);
+    free(max_state);
+}
+
+typedef struct
+{
+    int64_t agg;
+    double  val;
+} agg_val;
+
+/* saamg.cpp:51-207 (construct_prolongation_using_smoothed_aggregation).
+ * Pattern: distinct aggregates >= 0 among the diagonal and the strong
+ * neighbours; columns ascending (std::map order); per column the values are
+ * folded in CSR order, the first one assigned.  d~ = a_ii - sum(weak a_ij) in
+ * CSR order; value = 1 - relax on the diagonal, else (-relax * (1/d~)) * a_ij.
+ * Column count = max aggregate + 1 (saamg.cpp:68-76). */
+void or_prolongator(int m, const int* rp, const int* ci, const double* v, const int* conn,
+                    const int64_t* aggregates, double relax, or_csr* P)
+{
+    int64_t nc = -1;
+    for(int i = 0; i < m; i++)
+        if(nc < aggregates[i])
+            nc = aggregates[i];
+    nc++;
+
+    int maxlen = 0;
+    for(int i = 0; i < m; i++)
+        if(rp[i + 1] - rp[i] > maxlen)
+            maxlen = rp[i + 1] - rp[i];
+    agg_val* buf = (agg_val*)malloc(sizeof(agg_val) * (size_t)(maxlen > 0 ? maxlen : 1));
+
+    /* pass 1: row counts */
+    int* cnt = (int*)calloc((size_t)m + 1, sizeof(int));
+    for(int i = 0; i < m; i++)
+    {
+        int k = 0;
+        for(int j = rp[i]; j < rp[i + 1]; j++)
+        {
+            int c = ci[j];
+            if(c == i || conn[j] == 1)
+            {
+                int64_t a = aggregates[c];
+                if(a >= 0)
+                {
+                    int seen = 0;
+                    for(int t = 0; t < k; t++)
+                        if(buf[t].agg == a)
+                        {
+                            seen = 1;
+                            break;
+                        }
+                    if(!seen)
+                        buf[k++].agg = a;
+                }
+            }
+        }
+        cnt[i + 1] = k;
+    }
+    for(int i = 0; i < m; i++)
+        cnt[i + 1] += cnt[i];
+
+    csr_alloc(P, m, (int)nc, cnt[m]);
+    memcpy(P->row_ptr, cnt, sizeof(int) * ((size_t)m + 1));
+    free(cnt);
+
+    /* pass 2: values */
+    for(int i = 0; i < m; i++)
+    {
+        double diagonal = 0.0;
+        for(int j = rp[i]; j < rp[i + 1]; j++)
+        {
+            int c = ci[j];
+            if(c == i)
+                diagonal += v[j];
+            else if(conn[j] == 0)
+                diagonal -= v[j];
+        }
+        double invd = 1.0 / diagonal;
+
+        int k = 0;
+        for(int j = rp[i]; j < rp[i + 1]; j++)
+        {
+            int c = ci[j];
+            if(c == i || conn[j] == 1)
+            {
+                int64_t a = aggregates[c];
+                if(a >= 0)
+                {
+                    double value = (c == i) ? 1.0 - relax : -relax * invd * v[j];
+                    /* stable insertion: keeps CSR order among equal keys */
+                    int t = k;
+                    while(t > 0 && buf[t - 1].agg > a)
+                    {
+                        buf[t] = buf[t - 1];
+                        t--;
+                    }
+                    buf[t].agg = a;
+                    buf[t].val = value;
+                    k++;
+                }
+            }
+        }
+        int out = P->row_ptr[i];
+        for(int t = 0; t < k;)
+        {
+            int64_t a   = buf[t].agg;
+            double  acc = buf[t].val;
+            t++;
+            while(t < k && buf[t].agg == a)
+            {
+                acc += buf[t].val;
+                t++;
+            }
+            P->col_ind[out] = (int)a;
+            P->val[out]     = acc;
+            out++;
+        }
+    }
+    free(buf);
+}
+
+/* host_csr2csc.cpp:30-75: the slot search fills each column's slots in
+ * ascending row order, i.e. a stable counting-sort transpose. */
+void or_transpose(const or_csr* A, or_csr* T)
+{
+    csr_alloc(T, A->n, A->m, A->nnz);
+    for(int j = 0; j < A->nnz; j++)
+        T->row_ptr[A->col_ind[j] + 1]++;
+    for(int i = 0; i < A->n; i++)
+        T->row_ptr[i + 1] += T->row_ptr[i];
+    int* fill = (int*)malloc(sizeof(int) * ((size_t)A->n + 1));
+    memcpy(fill, T->row_ptr, sizeof(int) * ((size_t)A->n + 1));
+    for(int i = 0; i < A->m; i++)
+        for(int j = A->row_ptr[i]; j < A->row_ptr[i + 1]; j++)
+        {
+            int c             = A->col_ind[j];
+            int k             = fill[c]++;
+            T->col_ind[k]     = i;
+            T->val[k]         = A->val[j];
+        }
+    free(fill);
+}
+
+typedef struct
+{
+    int    col;
+    double val;
+} col_val;
+
+static int cmp_col_val(const void* a, const void* b)
+{
+    int ca = ((const col_val*)a)->col, cb = ((const col_val*)b)->col;
+    return (ca > cb) - (ca < cb);
+}
+
+/* host_csrgemm.cpp:36-89 (symbolic: marker array, distinct columns per row)
+ * and :91-183 (numeric: first touch stores val_A*val_B, later touches add
+ * val_A*val_B, A-row order then B-row order; then each row is sorted by
+ * column).  alpha = 1, beta = 0 as csr_matrix::multiply_by_matrix passes
+ * (csr_matrix.cpp:332-353). */
+void or_csrgemm(const or_csr* A, const or_csr* B, or_csr* C)
+{
+    int  m   = A->m;
+    int  n   = B->n;
+    int* mk  = (int*)malloc(sizeof(int) * ((size_t)n + 1));
+    int* rpc = (int*)calloc((size_t)m + 1, sizeof(int));
+    for(int c = 0; c < n; c++)
+        mk[c] = -1;
+    for(int i = 0; i < m; i++)
+        for(int j = A->row_ptr[i]; j < A->row_ptr[i + 1]; j++)
+        {
+            int ca = A->col_ind[j];
+            for(int p = B->row_ptr[ca]; p < B->row_ptr[ca + 1]; p++)
+            {
+                int cb = B->col_ind[p];
+                if(mk[cb] != i)
+                {
+                    mk[cb] = i;
+                    rpc[i + 1]++;
+                }
+            }
+        }
+    for(int i = 0; i < m; i++)
+        rpc[i + 1] += rpc[i];
+
+    csr_alloc(C, m, n, rpc[m]);
+    memcpy(C->row_ptr, rpc, sizeof(int) * ((size_t)m + 1));
+    free(rpc);
+
+    for(int c = 0; c < n; c++)
+        mk[c] = -1;
+    int      maxrow = 0;
+    for(int i = 0; i < m; i++)
+        if(C->row_ptr[i + 1] - C->row_ptr[i] > maxrow)
+            maxrow = C->row_ptr[i + 1] - C->row_ptr[i];
+    col_val* tmp = (col_val*)malloc(sizeof(col_val) * (size_t)(maxrow > 0 ? maxrow : 1));
+    for(int i = 0; i < m; i++)
+    {
+        int begin = C->row_ptr[i];
+        int end   = begin;
+        for(int j = A->row_ptr[i]; j < A->row_ptr[i + 1]; j++)
+        {
+            int    ca = A->col_ind[j];
+            double va = 1.0 * A->val[j];
+            for(int p = B->row_ptr[ca]; p < B->row_ptr[ca + 1]; p++)
+            {
+                int    cb = B->col_ind[p];
+                double vb = B->val[p];
+                if(mk[cb] < begin)
+                {
+                    mk[cb]        = end;
+                    C->col_ind[end] = cb;
+                    C->val[end]     = va * vb;
+                    end++;
+                }
+                else
+                    C->val[mk[cb]] += va * vb;
+            }
+        }
+        int len = end - begin;
+        for(int t = 0; t < len; t++)
+        {
+            tmp[t].col = C->col_ind[begin + t];
+            tmp[t].val = C->val[begin + t];
+        }
+        /* columns are distinct within a row, so any sort gives one order */
+        qsort(tmp, (size_t)len, sizeof(col_val), cmp_col_val);
+        for(int t = 0; t < len; t++)
+        {
+            C->col_ind[begin + t] = tmp[t].col;
+            C->val[begin + t]     = tmp[t].val;
+        }
+    }
+    free(tmp);
+    free(mk);
+}
+
+/* ------------------------------------------------------------------------- */
+/* hierarchy                                                                 */
+/* ------------------------------------------------------------------------- */
+
+static void csr_copy(or_csr* dst, int m, int n, int nnz, const int* rp, const int* ci,
+                     const double* v)
+{
+    csr_alloc(dst, m, n, nnz);
+    memcpy(dst->row_ptr, rp, sizeof(int) * ((size_t)m + 1));
+    memcpy(dst->col_ind, ci, sizeof(int) * (size_t)nnz);
+    memcpy(dst->val, v, sizeof(double) * (size_t)nnz);
+}
+
+/* saamg.cpp:210-282 (saamg_setup): eps = 0.08 halved per level, relax = 2/3,
+ * stop when P has no columns or at min(max_level, 20) levels.  eps0 and relax
+ * are parameters here so the same loop can run other values. */
+or_hierarchy* or_saamg_setup(int m, int n, int nnz, const int* rp, const int* ci,
+                             const double* v, int max_level, double eps0, double relax)
+{
+    or_hierarchy* h = (or_hierarchy*)calloc(1, sizeof(or_hierarchy));
+    if(max_level > OR_MAX_LEVELS)
+        max_level = OR_MAX_LEVELS;
+    csr_copy(&h->A[0], m, n, nnz, rp, ci, v);
+    double eps   = eps0;
+    int    level = 0;
+    while(level < max_level)
+    {
+        or_csr*  Af   = &h->A[level];
+        int*     conn = (int*)calloc((size_t)Af->nnz + 1, sizeof(int));
+        int64_t* agg  = (int64_t*)calloc((size_t)Af->m + 1, sizeof(int64_t));
+        or_strength(Af->m, Af->row_ptr, Af->col_ind, Af->val, eps, conn);
+        or_aggregates_pmis(Af->m, Af->row_ptr, Af->col_ind, conn, agg, NULL);
+        or_prolongator(Af->m, Af->row_ptr, Af->col_ind, Af->val, conn, agg, relax,
+                       &h->P[level]);
+        free(conn);
+        free(agg);
+        if(h->P[level].n == 0)
+        {
+            or_csr_free(&h->P[level]);
+            break;
+        }
+        or_transpose(&h->P[level], &h->R[level]);
+        or_csr AP;
+        or_csrgemm(Af, &h->P[level], &AP);
+        or_csrgemm(&h->R[level], &AP, &h->A[level + 1]);
+        or_csr_free(&AP);
+        level++;
+        eps *= 0.5;
+    }
+    h->total_levels = level;
+    return h;
+}
+
+void or_hierarchy_free(or_hierarchy* h)
+{
+    if(!h)
+        return;
+    for(int l = 0; l <= OR_MAX_LEVELS; l++)
+        or_csr_free(&h->A[l]);
+    for(int l = 0; l < OR_MAX_LEVELS; l++)
+    {
+        or_csr_free(&h->P[l]);
+        or_csr_free(&h->R[l]);
+    }
+    free(h);
+}
+
+int or_hierarchy_levels(const or_hierarchy* h)
+{
+    return h->total_levels;
+}
+
+static const or_csr* pick(const or_hierarchy* h, int level, int kind)
+{
+    if(level < 0)
+        return NULL;
+    if(kind == 0)
+        return level <= h->total_levels ? &h->A[level] : NULL;
+    if(level >= h->total_levels)
+        return NULL;
+    return kind == 1 ? &h->P[level] : &h->R[level];
+}
+
+int or_hierarchy_dims(const or_hierarchy* h, int level, int kind, int* m, int* n, int* nnz)
+{
+    const or_csr* A = pick(h, level, kind);
+    if(!A)
+        return -1;
+    *m   = A->m;
+    *n   = A->n;
+    *nnz = A->nnz;
+    return 0;
+}
+
+void or_hierarchy_copy(const or_hierarchy* h, int level, int kind, int* rp, int* ci, double* v)
+{
+    const or_csr* A = pick(h, level, kind);
+    if(!A)
+        return;
+    memcpy(rp, A->row_ptr, sizeof(int) * ((size_t)A->m + 1));
+    memcpy(ci, A->col_ind, sizeof(int) * (size_t)A->nnz);
+    memcpy(v, A->val, sizeof(double) * (size_t)A->nnz);
+}
+
+/* ------------------------------------------------------------------------- */
+/* solve                                                                     */
+/* ------------------------------------------------------------------------- */
+
+/* iterative_solvers/classic/jacobi.cpp:43-75: undamped Jacobi,
+ * sigma = sigma + a_jk*xold_k over k != j in CSR order; x_j = (b_j - sigma)/a_jj. */
+void or_jacobi_iteration(int m, const int* rp, const int* ci, const double* v, double* x,
+                         const double* xold, const double* b)
+{
+    for(int j = 0; j < m; j++)
+    {
+        double sigma = 0.0, ajj = 0.0;
+        for(int k = rp[j]; k < rp[j + 1]; k++)
+        {
+            if(ci[k] != j)
+                sigma = sigma + v[k] * xold[ci[k]];
+            else
+                ajj = v[k];
+        }
+        x[j] = (b[j] - sigma) / ajj;
+    }
+}
+
+/* Dense Cholesky of the coarsest operator (no reference counterpart: the
+ * reference runs 1000 Gauss-Seidel sweeps there). */
+static void coarse_cholesky_solve(const or_csr* A, double* x, const double* b)
+{
+    int     N = A->m;
+    double* L = (double*)calloc((size_t)N * N, sizeof(double));
+    for(int i = 0; i < N; i++)
+        for(int j = A->row_ptr[i]; j < A->row_ptr[i + 1]; j++)
+            L[(size_t)i * N + A->col_ind[j]] = A->val[j];
+    for(int j = 0; j < N; j++)
+    {
+        double s = L[(size_t)j * N + j];
+        for(int k = 0; k < j; k++)
+            s -= L[(size_t)j * N + k] * L[(size_t)j * N + k];
+        double d              = sqrt(s);
+        L[(size_t)j * N + j] = d;
+        for(int i = j + 1; i < N; i++)
+        {
+            double t = L[(size_t)i * N + j];
+            for(int k = 0; k < j; k++)
+                t -= L[(size_t)i * N + k] * L[(size_t)j * N + k];
+            L[(size_t)i * N + j] = t / d;
+        }
+    }
+    for(int i = 0; i < N; i++)
+    {
+        double t = b[i];
+        for(int k = 0; k < i; k++)
+            t -= L[(size_t)i * N + k] * x[k];
+        x[i] = t / L[(size_t)i * N + i];
+    }
+    for(int i = N - 1; i >= 0; i--)
+    {
+        double t = x[i];
+        for(int k = i + 1; k < N; k++)
+            t -= L[(size_t)k * N + i] * x[k];
+        x[i] = t / L[(size_t)i * N + i];
+    }
+    free(L);
+}
+
+/* amg.cpp:144-267 (vcycle) with apply_smoother(jacobi) (amg.cpp:110-126). */
+void or_vcycle(const or_hierarchy* h, double* x, const double* b, int n1, int n2, int level,
+               int coarse_solver)
+{
+    const or_csr* A = &h->A[level];
+    int           N = A->m;
+    if(level < h->total_levels)
+    {
+        const or_csr* R  = &h->R[level];
+        const or_csr* P  = &h->P[level];
+        int           Nc = h->A[level + 1].m;
+        double*       xold = (double*)malloc(sizeof(double) * ((size_t)N + 1));
+        for(int i = 0; i < n1; i++)
+        {
+            memcpy(xold, x, sizeof(double) * (size_t)N);
+            or_jacobi_iteration(N, A->row_ptr, A->col_ind, A->val, x, xold, b);
+        }
+        double* r    = (double*)calloc((size_t)N + 1, sizeof(double));
+        double* wres = (double*)calloc((size_t)Nc + 1, sizeof(double));
+        double* ec   = (double*)calloc((size_t)Nc + 1, sizeof(double));
+        or_residual(N, A->row_ptr, A->col_ind, A->val, x, b, r);
+        or_csrmv(R->m, R->n, R->nnz, 1.0, R->row_ptr, R->col_ind, R->val, r, 0.0, wres);
+        or_vcycle(h, ec, wres, n1, n2, level + 1, coarse_solver);
+        or_csrmv(P->m, P->n, P->nnz, 1.0, P->row_ptr, P->col_ind, P->val, ec, 1.0, x);
+        for(int i = 0; i < n2; i++)
+        {
+            memcpy(xold, x, sizeof(double) * (size_t)N);
+            or_jacobi_iteration(N, A->row_ptr, A->col_ind, A->val, x, xold, b);
+        }
+        free(xold);
+        free(r);
+        free(wres);
+        free(ec);
+    }
+    else if(N == 1)
+    {
+        x[0] = b[0] / A->val[0];
+    }
+    else if(coarse_solver == 1)
+    {
+        coarse_cholesky_solve(A, x, b);
+    }
+    else
+    {
+        for(int it = 0; it < 1000; it++)
+            for(int j = 0; j < N; j++)
+            {
+                double sigma = 0.0, ajj = 0.0;
+                for(int k = A->row_ptr[j]; k < A->row_ptr[j + 1]; k++)
+                {
+                    int col = A->col_ind[k];
+                    if(col != j)
+                        sigma = sigma + A->val[k] * x[col];
+                    else
+                        ajj = A->val[k];
+                }
+                x[j] = (b[j] - sigma) / ajj;
+            }
+    }
+}
+
+/* The reference's CPU composition of an AMG preconditioner: x.zeros() and
+ * amg_solve(h, x, rhs, n1, n2, vcycle, jacobi, {max_cycle = 0}), which runs
+ * exactly one V-cycle (amg.cpp:516-571, iter_control.cpp:88-91); amg_solve's
+ * residual norms do not touch x. */
+void or_amg_precond(const or_hierarchy* h, const double* r, double* z, int n1, int n2,
+                    int coarse_solver)
+{
+    memset(z, 0, sizeof(double) * (size_t)h->A[0].m);
+    or_vcycle(h, z, r, n1, n2, 0, coarse_solver);
+}
+
+/* iter_control.cpp:38-78. */
+static int converges(double res, double init, double rel_tol, double abs_tol)
+{
+    if(isnan(res) || isnan(init))
+        return 0;
+    if(res <= abs_tol)
+        return 1;
+    if(res / init <= rel_tol)
+        return 1;
+    return 0;
+}
+
+/* krylov/cg.cpp:136-223 (solve_precond), restart_iter = -1. */
+int or_pcg_saamg(const or_hierarchy* h, double* x, const double* b, double rel_tol,
+                 double abs_tol, int max_iter, int n1, int n2, int coarse_solver,
+                 double* res_norm)
+{
+    const or_csr* A   = &h->A[0];
+    int           n   = A->m;
+    double*       res = (double*)calloc((size_t)n + 1, sizeof(double));
+    double*       z   = (double*)calloc((size_t)n + 1, sizeof(double));
+    double*       p   = (double*)calloc((size_t)n + 1, sizeof(double));
+
+    or_residual(n, A->row_ptr, A->col_ind, A->val, x, b, res);
+    double init = or_norm_inf(n, res);
+    or_amg_precond(h, res, z, n1, n2, coarse_solver);
+    memcpy(p, z, sizeof(double) * (size_t)n);
+    double gamma = or_dot(n, z, res);
+    double rn    = init;
+
+    int iter = 0;
+    while(!(iter > max_iter))
+    {
+        or_csrmv(n, n, A->nnz, 1.0, A->row_ptr, A->col_ind, A->val, p, 0.0, z);
+        double alpha = gamma / or_dot(n, z, p);
+        or_axpy(n, alpha, p, x);
+        or_axpy(n, -1.0 * alpha, z, res);
+        rn = or_norm_inf(n, res);
+        if(converges(rn, init, rel_tol, abs_tol))
+            break;
+        or_amg_precond(h, res, z, n1, n2, coarse_solver);
+        double old_gamma = gamma;
+        gamma            = or_dot(n, z, res);
+        double beta      = gamma / old_gamma;
+        or_axpby(n, 1.0, z, beta, p);
+        iter++;
+    }
+    if(res_norm)
+        *res_norm = rn;
+    free(res);
+    free(z);
+    free(p);
+    return iter;
+}
+
+/* krylov/cg.cpp:60-134 (solve_nonprecond).  restart_iter = -1 makes
+ * `iter % restart_iter == 0` true for every iter > 0 (cg.cpp:46,89), so the
+ * search direction restarts from the recomputed residual each iteration. */
+int or_cg(int m, const int* rp, const int* ci, const double* v, double* x, const double* b,
+          double rel_tol, double abs_tol, int max_iter, double* res_norm)
+{
+    int     n   = m;
+    double* res = (double*)calloc((size_t)n + 1, sizeof(double));
+    double* z   = (double*)calloc((size_t)n + 1, sizeof(double));
+    double* p   = (double*)calloc((size_t)n + 1, sizeof(double));
+    or_residual(n, rp, ci, v, x, b, res);
+    double init = or_norm_inf(n, res);
+    memcpy(p, res, sizeof(double) * (size_t)n);
+    double gamma = or_dot(n, res, res);
+    double rn    = init;
+    int    iter  = 0;
+    while(!(iter > max_iter))
+    {
+        if(iter > 0)
+        {
+            or_residual(n, rp, ci, v, x, b, res);
+            memcpy(p, res, sizeof(double) * (size_t)n);
+            gamma = or_dot(n, res, res);
+        }
+        or_csrmv(n, n, rp[n], 1.0, rp, ci, v, p, 0.0, z);
+        double alpha = gamma / or_dot(n, z, p);
+        or_axpy(n, alpha, p, x);
+        or_axpy(n, -1.0 * alpha, z, res);
+        rn = or_norm_inf(n, res);
+        if(converges(rn, init, rel_tol, abs_tol))
+            break;
+        double old_gamma = gamma;
+        gamma            = or_dot(n, res, res);
+        double beta      = gamma / old_gamma;
+        or_axpby(n, 1.0, res, beta, p);
+        iter++;
+    }
+    if(res_norm)
+        *res_norm = rn;
+    free(res);
+    free(z);
+    free(p);
+    return iter;
+}

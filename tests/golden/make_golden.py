@@ -1,0 +1,155 @@
+"""Write the golden fixtures in tests/golden/ from LinAlgLib's own host library.
+
+Runs only in a container that holds the reference sources: it needs
+oracle/_ref/libref.so, which `make -C oracle ref` compiles from
+/root/reference/library/src.  The fixtures are data only (inputs and the
+reference's outputs); the tests read them without the reference present.
+
+    python tests/golden/make_golden.py
+"""
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from linalglib_amd import problems as pb  # noqa: E402
+from oracle.oracle import CSR, Reference  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def grid(name):
+    if name == "2d8":
+        return pb.poisson2d(8)
+    if name == "3d8":
+        return pb.poisson3d(8)
+    if name == "2d128":
+        return pb.poisson2d(128)
+    if name == "3d16":
+        return pb.poisson3d(16)
+    if name == "diff12":
+        return diffusion12()
+    raise KeyError(name)
+
+
+def diffusion12():
+    """12^3 diffusion with log-normal cell conductivities (sigma = 1.5), so
+    that strength marks a mix of strong and weak couplings."""
+    k = np.exp(1.5 * pb.normal(12 ** 3, seed=7))
+    return pb.diffusion3d(12, cell_kappa=k)
+
+
+def hierarchy_case(ref, name, full):
+    rp, ci, v, m = grid(name)
+    A = CSR(rp, ci, v, m, m)
+    h = ref.saamg_setup(A)
+    out = {"levels": np.int32(h.levels)}
+    for lvl in range(h.levels + 1):
+        for kind in "APR":
+            M = h.get(lvl, kind)
+            if M is None:
+                continue
+            key = f"{kind}{lvl}"
+            out[key + "_dims"] = np.array([M.m, M.n, M.nnz], dtype=np.int64)
+            if full:
+                out[key + "_rp"] = M.row_ptr
+                out[key + "_ci"] = M.col_ind
+                out[key + "_v"] = M.val
+            else:
+                out[key + "_sha"] = np.array(
+                    [sha(M.row_ptr), sha(M.col_ind), sha(M.val)], dtype="U64")
+    xs = pb.uniform(m, seed=1)
+    b = pb.rhs_from_solution(rp, ci, v, xs)
+    out["b"] = b
+    z = h.precond(b, 1, 1)
+    out["precond_z"] = z
+    for tol, tag in ((1e-8, "8"), (1e-10, "10")):
+        x, it = h.pcg(b, rel_tol=tol, abs_tol=1e-300, max_iter=1000, n1=1, n2=1)
+        out[f"pcg{tag}_iters"] = np.int32(it)
+        out[f"pcg{tag}_x"] = x
+    np.savez_compressed(os.path.join(OUT, f"golden_{name}.npz"), **out)
+    print(name, "levels", h.levels, "iters", int(out["pcg8_iters"]), int(out["pcg10_iters"]))
+
+
+def random_csr(rng, m, n, per_row, vals="uniform"):
+    rows = []
+    for i in range(m):
+        k = int(rng.integers(0, per_row + 1))
+        rows.append(np.sort(rng.choice(n, size=min(k, n), replace=False)))
+    rp = np.zeros(m + 1, dtype=np.int32)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    ci = np.concatenate(rows).astype(np.int32) if rp[-1] else np.zeros(0, np.int32)
+    if vals == "uniform":
+        v = rng.uniform(-1, 1, rp[-1])
+    else:  # small integers: products cancel to exact zeros that must be kept
+        v = rng.integers(-2, 3, rp[-1]).astype(np.float64)
+    return CSR(rp, ci, v, m, n)
+
+
+def ops_case(ref):
+    rng = np.random.default_rng(1234567)
+    out = {}
+    # SpMV with alpha/beta, including empty rows
+    A = random_csr(rng, 300, 200, 12)
+    x = rng.uniform(-1, 1, 200)
+    y0 = rng.uniform(-1, 1, 300)
+    out.update(mv_rp=A.row_ptr, mv_ci=A.col_ind, mv_v=A.val, mv_x=x, mv_y0=y0)
+    for tag, a, bb in (("a1b0", 1.0, 0.0), ("a1b1", 1.0, 1.0), ("am2b05", -2.0, 0.5)):
+        out["mv_" + tag] = ref.csrmv(A, x, a, bb, y0)
+    # residual and dot
+    S = random_csr(rng, 250, 250, 9)
+    xs = rng.uniform(-1, 1, 250)
+    bs = rng.uniform(-1, 1, 250)
+    out.update(res_rp=S.row_ptr, res_ci=S.col_ind, res_v=S.val, res_x=xs, res_b=bs)
+    out["res_r"] = ref.residual(S, xs, bs)
+    dx = rng.uniform(-1, 1, 5000)
+    dy = rng.uniform(-1, 1, 5000)
+    ref.set_threads(1)
+    out.update(dot_x=dx, dot_y=dy, dot_1t=np.float64(ref.dot(dx, dy)))
+    # SpGEMM (values with cancellations) and transpose
+    G1 = random_csr(rng, 120, 90, 8, vals="int")
+    G2 = random_csr(rng, 90, 110, 8, vals="int")
+    G = ref.csrgemm(G1, G2)
+    out.update(g1_rp=G1.row_ptr, g1_ci=G1.col_ind, g1_v=G1.val,
+               g2_rp=G2.row_ptr, g2_ci=G2.col_ind, g2_v=G2.val,
+               g_rp=G.row_ptr, g_ci=G.col_ind, g_v=G.val)
+    H1 = random_csr(rng, 80, 70, 10)
+    H2 = random_csr(rng, 70, 60, 10)
+    H = ref.csrgemm(H1, H2)
+    out.update(h1_rp=H1.row_ptr, h1_ci=H1.col_ind, h1_v=H1.val,
+               h2_rp=H2.row_ptr, h2_ci=H2.col_ind, h2_v=H2.val,
+               h_rp=H.row_ptr, h_ci=H.col_ind, h_v=H.val)
+    T = ref.transpose(H1)
+    out.update(t_rp=T.row_ptr, t_ci=T.col_ind, t_v=T.val)
+    # strength and PMIS aggregates on a heterogeneous operator
+    rp, ci, v, m = diffusion12()
+    D = CSR(rp, ci, v, m, m)
+    for eps, tag in ((0.08, "08"), (0.25, "25")):
+        conn = ref.strength(D, eps)
+        out["str_" + tag] = conn
+        out["agg_" + tag] = ref.aggregates(D, conn)
+    np.savez_compressed(os.path.join(OUT, "golden_ops.npz"), **out)
+    print("ops written")
+
+
+def main():
+    ref = Reference()
+    ref.set_threads(1)
+    hierarchy_case(ref, "2d8", full=True)
+    hierarchy_case(ref, "3d8", full=True)
+    hierarchy_case(ref, "diff12", full=True)
+    hierarchy_case(ref, "2d128", full=False)
+    hierarchy_case(ref, "3d16", full=False)
+    ops_case(ref)
+
+
+if __name__ == "__main__":
+    main()
