@@ -1,0 +1,373 @@
+// SA-AMG setup kernels: strength of connection, PMIS aggregation and the
+// smoothed prolongator.
+//
+// Every kernel gives one row to one lane and walks that row in CSR order, so
+// each decision and each floating-point value is formed by the same
+// operations in the same order as the host path:
+//   strength     backend/host/host_amg_strength.cpp:33-70
+//   PMIS         backend/host/host_amg_aggregation.cpp:32-221 driven as
+//                iterative_solvers/amg/amg_aggregation.cpp:106-215
+//   prolongator  iterative_solvers/amg/saamg.cpp:51-207
+// The library is built with -ffp-contract=off, and HIP's double sqrt and
+// division are correctly rounded, so the results are bit-identical.
+#include "lal_internal.h"
+
+#include <algorithm>
+#include <climits>
+
+namespace
+{
+    __global__ void __launch_bounds__(256)
+        strength_kernel(int m, const int* __restrict__ rp, const int* __restrict__ ci,
+                        const double* __restrict__ val, const double* __restrict__ d, double eps,
+                        int* __restrict__ conn)
+    {
+        int i = blockIdx.x * 256 + threadIdx.x;
+        if(i >= m)
+            return;
+        const double di = fabs(d[i]);
+        for(int j = rp[i]; j < rp[i + 1]; j++)
+        {
+            int    c = ci[j];
+            double v = val[j];
+            conn[j]  = (c != i) && (fabs(v) >= eps * sqrt(di * fabs(d[c])));
+        }
+    }
+
+    __device__ __forceinline__ int hash1(unsigned x)
+    {
+        x = ((x >> 16) ^ x) * 0x45d9f3bu;
+        x = ((x >> 16) ^ x) * 0x45d9f3bu;
+        x = (x >> 16) ^ x;
+        return (int)(x / 2);
+    }
+
+    __global__ void __launch_bounds__(256)
+        pmis_init_kernel(int m, const int* __restrict__ rp, const int* __restrict__ conn,
+                         int* __restrict__ state, int* __restrict__ hash)
+    {
+        int i = blockIdx.x * 256 + threadIdx.x;
+        if(i >= m)
+            return;
+        int s = -2;
+        for(int j = rp[i]; j < rp[i + 1]; j++)
+            if(conn[j] == 1)
+            {
+                s = 0;
+                break;
+            }
+        state[i] = s;
+        hash[i]  = hash1((unsigned)i);
+    }
+
+    // Candidate (s, h, r) replaces the running maximum when its state is
+    // strictly greater, or equal with a strictly greater hash.
+    __device__ __forceinline__ void lex_max(int& ms, int& mh, int& mr, int s, int h, int r)
+    {
+        if(s > ms || (s == ms && h > mh))
+        {
+            ms = s;
+            mh = h;
+            mr = r;
+        }
+    }
+
+    __global__ void __launch_bounds__(256)
+        pmis_round_kernel(int m, const int* __restrict__ rp, const int* __restrict__ ci,
+                          const int* __restrict__ conn, const int* __restrict__ state,
+                          const int* __restrict__ hash, int64_t* __restrict__ agg,
+                          int* __restrict__ max_state, int* __restrict__ incomplete)
+    {
+        int i = blockIdx.x * 256 + threadIdx.x;
+        if(i >= m)
+            return;
+        const int si = state[i];
+        if(si != 0)
+            return; // only undecided rows change anything
+        int ms = si, mh = hash[i], mr = i;
+        for(int j = rp[i]; j < rp[i + 1]; j++)
+            if(conn[j] == 1)
+            {
+                int c = ci[j];
+                lex_max(ms, mh, mr, state[c], hash[c], c);
+            }
+        const int r2 = mr;
+        for(int j = rp[r2]; j < rp[r2 + 1]; j++)
+            if(conn[j] == 1)
+            {
+                int c = ci[j];
+                lex_max(ms, mh, mr, state[c], hash[c], c);
+            }
+        if(mr == i)
+        {
+            max_state[i] = 1;
+            agg[i]       = 1;
+        }
+        else if(ms == 1)
+        {
+            max_state[i] = -1;
+            agg[i]       = 0;
+        }
+        else
+            *incomplete = 1;
+    }
+
+    __global__ void __launch_bounds__(256)
+        pmis_join_kernel(int m, const int* __restrict__ rp, const int* __restrict__ ci,
+                         const int* __restrict__ conn, const int* __restrict__ state,
+                         int64_t* __restrict__ agg, int* __restrict__ max_state)
+    {
+        int i = blockIdx.x * 256 + threadIdx.x;
+        if(i >= m)
+            return;
+        const int si = state[i];
+        if(si == -1)
+        {
+            for(int j = rp[i]; j < rp[i + 1]; j++)
+                if(conn[j] == 1)
+                {
+                    int c = ci[j];
+                    if(state[c] == 1)
+                    {
+                        // c has state 1 in the snapshot, so its aggregate is
+                        // not written during this pass
+                        agg[i]       = agg[c];
+                        max_state[i] = 1;
+                        break;
+                    }
+                }
+        }
+        else if(si == -2)
+            agg[i] = -2;
+    }
+
+    __device__ __forceinline__ bool p_candidate(int i, int c, int cn)
+    {
+        return c == i || cn == 1;
+    }
+
+    // Distinct aggregates >= 0 among the diagonal and the strong neighbours.
+    __global__ void __launch_bounds__(256)
+        prolong_count_kernel(int m, const int* __restrict__ rp, const int* __restrict__ ci,
+                             const int* __restrict__ conn, const int64_t* __restrict__ agg,
+                             int* __restrict__ count)
+    {
+        int i = blockIdx.x * 256 + threadIdx.x;
+        if(i > m)
+            return;
+        if(i == m)
+        {
+            count[m] = 0;
+            return;
+        }
+        int       k  = 0;
+        const int rs = rp[i], re = rp[i + 1];
+        for(int j = rs; j < re; j++)
+        {
+            int c = ci[j];
+            if(!p_candidate(i, c, conn[j]))
+                continue;
+            int64_t a = agg[c];
+            if(a < 0)
+                continue;
+            bool seen = false;
+            for(int t = rs; t < j && !seen; t++)
+            {
+                int ct = ci[t];
+                seen   = p_candidate(i, ct, conn[t]) && agg[ct] == a;
+            }
+            k += !seen;
+        }
+        count[i] = k;
+    }
+
+    // Columns ascending; each column's value is the CSR-order fold of its
+    // contributions starting from the first one.
+    __global__ void __launch_bounds__(256)
+        prolong_fill_kernel(int m, const int* __restrict__ rp, const int* __restrict__ ci,
+                            const double* __restrict__ val, const int* __restrict__ conn,
+                            const int64_t* __restrict__ agg, double relax,
+                            const int* __restrict__ prp, int* __restrict__ pci,
+                            double* __restrict__ pv)
+    {
+        int i = blockIdx.x * 256 + threadIdx.x;
+        if(i >= m)
+            return;
+        const int rs = rp[i], re = rp[i + 1];
+        double    diagonal = 0.0;
+        for(int j = rs; j < re; j++)
+        {
+            int c = ci[j];
+            if(c == i)
+                diagonal += val[j];
+            else if(conn[j] == 0)
+                diagonal -= val[j];
+        }
+        const double invd = 1.0 / diagonal;
+        int          out  = prp[i];
+        const int    end  = prp[i + 1];
+        int64_t      last = -1;
+        for(; out < end; out++)
+        {
+            int64_t best = LLONG_MAX;
+            for(int j = rs; j < re; j++)
+            {
+                int c = ci[j];
+                if(!p_candidate(i, c, conn[j]))
+                    continue;
+                int64_t a = agg[c];
+                if(a > last && a < best)
+                    best = a;
+            }
+            bool   first = true;
+            double acc   = 0.0;
+            for(int j = rs; j < re; j++)
+            {
+                int c = ci[j];
+                if(!p_candidate(i, c, conn[j]) || agg[c] != best)
+                    continue;
+                double value = (c == i) ? 1.0 - relax : -relax * invd * val[j];
+                if(first)
+                {
+                    acc   = value;
+                    first = false;
+                }
+                else
+                    acc += value;
+            }
+            pci[out] = (int)best;
+            pv[out]  = acc;
+            last     = best;
+        }
+    }
+}
+
+namespace lal
+{
+    int diagonal_f64(lal_context* c, int m, const int* rp, const int* ci, const double* v,
+                     double* d);
+
+    int sa_strength(lal_context* c, int m, int nnz, const int* rp, const int* ci,
+                    const double* v, double eps, int* conn)
+    {
+        if(m == 0)
+            return LAL_OK;
+        double* d = nullptr;
+        LAL_HIP(hipMallocAsync((void**)&d, sizeof(double) * m, c->stream));
+        LAL_HIP(hipMemsetAsync(d, 0, sizeof(double) * m, c->stream));
+        LAL_TRY(diagonal_f64(c, m, rp, ci, v, d));
+        strength_kernel<<<grid_for(m, 256), 256, 0, c->stream>>>(m, rp, ci, v, d, eps, conn);
+        LAL_LAUNCHED();
+        LAL_HIP(hipFreeAsync(d, c->stream));
+        (void)nnz;
+        return LAL_OK;
+    }
+
+    int pmis_aggregate(lal_context* c, int m, int nnz, const int* rp, const int* ci,
+                       const int* conn, int64_t* agg)
+    {
+        if(m == 0)
+            return LAL_OK;
+        int *state = nullptr, *max_state = nullptr, *hash = nullptr;
+        LAL_HIP(hipMallocAsync((void**)&state, sizeof(int) * m, c->stream));
+        LAL_HIP(hipMallocAsync((void**)&max_state, sizeof(int) * m, c->stream));
+        LAL_HIP(hipMallocAsync((void**)&hash, sizeof(int) * m, c->stream));
+        LAL_HIP(hipMemsetAsync(agg, 0, sizeof(int64_t) * m, c->stream));
+        const int g = grid_for(m, 256);
+        pmis_init_kernel<<<g, 256, 0, c->stream>>>(m, rp, conn, max_state, hash);
+        LAL_LAUNCHED();
+        for(int iter = 0; iter < 20; iter++)
+        {
+            LAL_HIP(hipMemcpyAsync(state, max_state, sizeof(int) * m, hipMemcpyDeviceToDevice,
+                                   c->stream));
+            LAL_HIP(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
+            pmis_round_kernel<<<g, 256, 0, c->stream>>>(m, rp, ci, conn, state, hash, agg,
+                                                        max_state, c->d_flag);
+            LAL_LAUNCHED();
+            LAL_HIP(hipMemcpyAsync(c->h_flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost,
+                                   c->stream));
+            LAL_HIP(hipStreamSynchronize(c->stream));
+            if(c->h_flag[0] == 0)
+                break;
+        }
+        LAL_TRY(scan_i64_dev(c, m, agg));
+        for(int k = 0; k < 2; k++)
+        {
+            LAL_HIP(hipMemcpyAsync(state, max_state, sizeof(int) * m, hipMemcpyDeviceToDevice,
+                                   c->stream));
+            pmis_join_kernel<<<g, 256, 0, c->stream>>>(m, rp, ci, conn, state, agg, max_state);
+            LAL_LAUNCHED();
+        }
+        LAL_HIP(hipFreeAsync(state, c->stream));
+        LAL_HIP(hipFreeAsync(max_state, c->stream));
+        LAL_HIP(hipFreeAsync(hash, c->stream));
+        (void)nnz;
+        return LAL_OK;
+    }
+
+    int sa_prolongator_nnz(lal_context* c, int m, const int* rp, const int* ci,
+                           const int* conn, const int64_t* agg, int* prp, int* nnz_P,
+                           int* n_c)
+    {
+        prolong_count_kernel<<<grid_for(m + 1, 256), 256, 0, c->stream>>>(m, rp, ci, conn, agg,
+                                                                          prp);
+        LAL_LAUNCHED();
+        LAL_TRY(scan_i32_dev(c, m + 1, prp));
+        int64_t mx = LLONG_MIN;
+        LAL_TRY(max_i64_dev(c, m, agg, &mx));
+        int64_t nc = std::max<int64_t>(mx, -1) + 1;
+        if(nc > INT_MAX)
+            return LAL_ERR_UNSUPPORTED;
+        LAL_HIP(hipMemcpyAsync(nnz_P, prp + m, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        LAL_HIP(hipStreamSynchronize(c->stream));
+        *n_c = (int)nc;
+        return LAL_OK;
+    }
+
+    int sa_prolongator_fill(lal_context* c, int m, const int* rp, const int* ci, const double* v,
+                            const int* conn, const int64_t* agg, double relax, const int* prp,
+                            int* pci, double* pv)
+    {
+        if(m == 0)
+            return LAL_OK;
+        prolong_fill_kernel<<<grid_for(m, 256), 256, 0, c->stream>>>(m, rp, ci, v, conn, agg,
+                                                                     relax, prp, pci, pv);
+        LAL_LAUNCHED();
+        return LAL_OK;
+    }
+}
+
+extern "C" int lal_sa_strength_f64(lal_handle c, int m, int nnz, const int* rp, const int* ci,
+                                   const double* v, double eps, int* conn)
+{
+    LAL_REQUIRE(c != nullptr && m >= 0 && nnz >= 0);
+    LAL_REQUIRE(m == 0 || (rp && (nnz == 0 || (ci && v && conn))));
+    return lal::sa_strength(c, m, nnz, rp, ci, v, eps, conn);
+}
+
+extern "C" int lal_pmis_aggregate(lal_handle c, int m, int nnz, const int* rp, const int* ci,
+                                  const int* conn, int64_t* agg)
+{
+    LAL_REQUIRE(c != nullptr && m >= 0 && nnz >= 0);
+    LAL_REQUIRE(m == 0 || (rp && agg && (nnz == 0 || (ci && conn))));
+    return lal::pmis_aggregate(c, m, nnz, rp, ci, conn, agg);
+}
+
+extern "C" int lal_sa_prolongator_nnz(lal_handle c, int m, int nnz, const int* rp,
+                                      const int* ci, const int* conn, const int64_t* agg,
+                                      int* prp, int* nnz_P, int* n_c)
+{
+    LAL_REQUIRE(c != nullptr && m >= 0 && nnz >= 0 && prp && nnz_P && n_c);
+    LAL_REQUIRE(m == 0 || (rp && agg && (nnz == 0 || (ci && conn))));
+    return lal::sa_prolongator_nnz(c, m, rp, ci, conn, agg, prp, nnz_P, n_c);
+}
+
+extern "C" int lal_sa_prolongator_fill_f64(lal_handle c, int m, int nnz, const int* rp,
+                                           const int* ci, const double* v, const int* conn,
+                                           const int64_t* agg, double relax, const int* prp,
+                                           int* pci, double* pv)
+{
+    LAL_REQUIRE(c != nullptr && m >= 0 && nnz >= 0);
+    LAL_REQUIRE(m == 0 || (rp && agg && prp && (nnz == 0 || (ci && v && conn))));
+    return lal::sa_prolongator_fill(c, m, rp, ci, v, conn, agg, relax, prp, pci, pv);
+}

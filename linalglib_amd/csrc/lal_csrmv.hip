@@ -1,0 +1,307 @@
+// CSR SpMV, residual and diagonal extraction for gfx950.
+//
+// y = alpha*A*x + beta*y follows backend/host/host_matrix_vector.cpp:59-84
+// (row sum s = fma(val, x[col], s), then beta == 0 ? alpha*s :
+// fma(alpha, s, beta*y)); r = b - A*x follows :35-57 (s += val*x, two
+// roundings).  The row-block layout and LDS staging are described in
+// lal_spmv.h.
+#include "lal_spmv.h"
+
+#include <vector>
+
+namespace
+{
+    using lal::SPMV_THREADS;
+
+    // MODE 0: y = alpha*A*x + beta*y ;  MODE 1: y = b - A*x
+    template <typename T, typename ACC, int MODE>
+    __global__ void __launch_bounds__(SPMV_THREADS)
+        csrmv_kernel(const int* __restrict__ blocks, int nb, int m, const int* __restrict__ rp,
+                     const int* __restrict__ ci, const T* __restrict__ val,
+                     const T* __restrict__ x, ACC alpha, ACC beta, const T* __restrict__ b,
+                     T* __restrict__ y, int exact)
+    {
+        constexpr int CAP = lal::spmv_cap<T>::value;
+        __shared__ T  s_v[CAP];
+        __shared__ T  s_x[CAP];
+        __shared__ int s_rp[SPMV_THREADS + 1];
+
+        const int tid  = threadIdx.x;
+        const int tile = lal::xcd_tile(blockIdx.x, nb);
+        int       r0, r1;
+        lal::block_rows(blocks, m, tile, &r0, &r1);
+        const int nrows = r1 - r0;
+        for(int i = tid; i <= nrows; i += SPMV_THREADS)
+            s_rp[i] = rp[r0 + i];
+        __syncthreads();
+
+        const int  base   = s_rp[0];
+        const int  end    = s_rp[nrows];
+        const int  g      = lal::spmv_group(nrows, exact);
+        const int  lrow   = tid / g;
+        const int  lig    = tid & (g - 1);
+        const bool active = lrow < nrows;
+        const int  rs     = active ? s_rp[lrow] : 0;
+        const int  re     = active ? s_rp[lrow + 1] : 0;
+
+        ACC acc = 0;
+        for(int cs = base; cs < end; cs += CAP)
+        {
+            const int len = min(CAP, end - cs);
+#pragma unroll 4
+            for(int k = tid; k < len; k += SPMV_THREADS)
+            {
+                int c  = ci[cs + k];
+                s_v[k] = val[cs + k];
+                s_x[k] = x[c];
+            }
+            __syncthreads();
+            if(active)
+            {
+                const int a = max(rs, cs) - cs;
+                const int e = min(re, cs + len) - cs;
+                for(int j = a + lig; j < e; j += g)
+                {
+                    if(MODE == 0)
+                        acc = fma((ACC)s_v[j], (ACC)s_x[j], acc);
+                    else
+                        acc = acc + (ACC)s_v[j] * (ACC)s_x[j];
+                }
+            }
+            __syncthreads();
+        }
+        if(g > 1)
+            acc = lal::group_sum(acc, g);
+        if(active && lig == 0)
+        {
+            const int row = r0 + lrow;
+            if(MODE == 0)
+                y[row] = (T)(beta == (ACC)0 ? alpha * acc : fma(alpha, acc, beta * (ACC)y[row]));
+            else
+                y[row] = (T)((ACC)b[row] - acc);
+        }
+    }
+
+    // Flags 1 in *bad if a column index lies outside [0, n): an invalid matrix
+    // is refused before any gather can run off the end of x.
+    __global__ void __launch_bounds__(256)
+        check_cols_kernel(int nnz, int n, const int* __restrict__ ci, int* __restrict__ bad)
+    {
+        int k = blockIdx.x * 256 + threadIdx.x;
+        int stride = gridDim.x * 256;
+        int b = 0;
+        for(; k < nnz; k += stride)
+            b |= (unsigned)ci[k] >= (unsigned)n;
+        if(__any(b) && (threadIdx.x & 63) == 0)
+            atomicOr(bad, 1);
+    }
+
+    __global__ void __launch_bounds__(256)
+        diagonal_kernel(int m, const int* __restrict__ rp, const int* __restrict__ ci,
+                        const double* __restrict__ val, double* __restrict__ d)
+    {
+        int i = blockIdx.x * 256 + threadIdx.x;
+        if(i >= m)
+            return;
+        for(int j = rp[i]; j < rp[i + 1]; j++)
+            if(ci[j] == i)
+            {
+                d[i] = val[j];
+                break;
+            }
+    }
+
+    template <typename T, typename ACC, int MODE>
+    int csrmv_launch(lal_context* c, const lal_csrmv_info* d, int m, int n, int nnz, ACC alpha,
+                     const int* rp, const int* ci, const T* v, const T* x, ACC beta,
+                     const T* b, T* y)
+    {
+        LAL_REQUIRE(c != nullptr && m >= 0 && n >= 0 && nnz >= 0);
+        if(m == 0)
+            return LAL_OK;
+        LAL_REQUIRE(rp != nullptr && y != nullptr);
+        LAL_REQUIRE(nnz == 0 || (ci != nullptr && v != nullptr && x != nullptr));
+        if(d)
+            LAL_REQUIRE(d->m == m && d->nnz == nnz);
+        int nb = lal::spmv_grid(d, m);
+        csrmv_kernel<T, ACC, MODE><<<nb, SPMV_THREADS, 0, c->stream>>>(
+            d ? d->d_blocks : nullptr, nb, m, rp, ci, v, x, alpha, beta, b, y,
+            d ? d->exact : 0);
+        LAL_LAUNCHED();
+        return LAL_OK;
+    }
+}
+
+namespace lal
+{
+    // Greedy row packing: a block closes at 256 rows or when the next row
+    // would push it past CAP nonzeros; a row longer than CAP is a block of
+    // its own.
+    int csrmv_analysis_host(const int* h_rp, int m, std::vector<int>& blocks)
+    {
+        const int cap = spmv_cap<double>::value;
+        blocks.clear();
+        blocks.push_back(0);
+        int r = 0;
+        while(r < m)
+        {
+            int start = r;
+            int nz0   = h_rp[r];
+            while(r < m && r - start < SPMV_THREADS && h_rp[r + 1] - nz0 <= cap)
+                r++;
+            if(r == start)
+                r++;
+            blocks.push_back(r);
+        }
+        return LAL_OK;
+    }
+
+    int check_columns(lal_context* c, int nnz, int n, const int* ci)
+    {
+        if(nnz == 0)
+            return LAL_OK;
+        if(ci == nullptr)
+            return LAL_ERR_INVALID;
+        LAL_HIP(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
+        check_cols_kernel<<<grid_for(nnz, 256, 4096), 256, 0, c->stream>>>(nnz, n, ci,
+                                                                            c->d_flag);
+        LAL_LAUNCHED();
+        LAL_HIP(hipMemcpyAsync(c->h_flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost,
+                               c->stream));
+        LAL_HIP(hipStreamSynchronize(c->stream));
+        if(c->h_flag[0])
+        {
+            set_last_error("column index outside [0, n)");
+            return LAL_ERR_INVALID;
+        }
+        return LAL_OK;
+    }
+
+    int csrmv_analysis_dev(lal_context* c, int m, int n, int nnz, const int* rp,
+                           const int* ci, int exact, lal_csrmv_info** out)
+    {
+        LAL_TRY(check_columns(c, nnz, n, ci));
+        std::vector<int> h(m + 1);
+        LAL_HIP(hipMemcpyAsync(h.data(), rp, sizeof(int) * (m + 1), hipMemcpyDeviceToHost,
+                               c->stream));
+        LAL_HIP(hipStreamSynchronize(c->stream));
+        if(h[0] != 0 || h[m] != nnz)
+        {
+            set_last_error("row_ptr does not start at 0 or end at nnz");
+            return LAL_ERR_INVALID;
+        }
+        for(int i = 0; i < m; i++)
+            if(h[i + 1] < h[i])
+            {
+                set_last_error("row_ptr is not monotone");
+                return LAL_ERR_INVALID;
+            }
+        std::vector<int> blocks;
+        csrmv_analysis_host(h.data(), m, blocks);
+        auto* d    = new lal_csrmv_info;
+        d->m       = m;
+        d->nnz     = nnz;
+        d->exact   = exact;
+        d->nblocks = (int)blocks.size() - 1;
+        if(hipMalloc(&d->d_blocks, sizeof(int) * blocks.size()) != hipSuccess)
+        {
+            delete d;
+            return LAL_ERR_OOM;
+        }
+        LAL_HIP(hipMemcpyAsync(d->d_blocks, blocks.data(), sizeof(int) * blocks.size(),
+                               hipMemcpyHostToDevice, c->stream));
+        LAL_HIP(hipStreamSynchronize(c->stream));
+        *out = d;
+        (void)n;
+        return LAL_OK;
+    }
+
+    int csrmv_f64(lal_context* c, const lal_csrmv_info* d, int m, int n, int nnz, double alpha,
+                  const int* rp, const int* ci, const double* v, const double* x, double beta,
+                  double* y)
+    {
+        return csrmv_launch<double, double, 0>(c, d, m, n, nnz, alpha, rp, ci, v, x, beta,
+                                               nullptr, y);
+    }
+
+    int csrmv_f32(lal_context* c, const lal_csrmv_info* d, int m, int n, int nnz, float alpha,
+                  const int* rp, const int* ci, const float* v, const float* x, float beta,
+                  float* y)
+    {
+        return csrmv_launch<float, float, 0>(c, d, m, n, nnz, alpha, rp, ci, v, x, beta,
+                                             nullptr, y);
+    }
+
+    int residual_f64(lal_context* c, const lal_csrmv_info* d, int m, int n, int nnz,
+                     const int* rp, const int* ci, const double* v, const double* x,
+                     const double* b, double* r)
+    {
+        LAL_REQUIRE(m == 0 || b != nullptr);
+        return csrmv_launch<double, double, 1>(c, d, m, n, nnz, 0.0, rp, ci, v, x, 0.0, b, r);
+    }
+
+    int residual_f32(lal_context* c, const lal_csrmv_info* d, int m, int n, int nnz,
+                     const int* rp, const int* ci, const float* v, const float* x,
+                     const float* b, float* r)
+    {
+        LAL_REQUIRE(m == 0 || b != nullptr);
+        return csrmv_launch<float, float, 1>(c, d, m, n, nnz, 0.0f, rp, ci, v, x, 0.0f, b, r);
+    }
+
+    int diagonal_f64(lal_context* c, int m, const int* rp, const int* ci, const double* v,
+                     double* d)
+    {
+        if(m == 0)
+            return LAL_OK;
+        diagonal_kernel<<<grid_for(m, 256), 256, 0, c->stream>>>(m, rp, ci, v, d);
+        LAL_LAUNCHED();
+        return LAL_OK;
+    }
+}
+
+extern "C" int lal_csrmv_analysis(lal_handle c, int m, int n, int nnz, const int* row_ptr,
+                                  const int* col_ind, int exact, lal_csrmv_descr* descr)
+{
+    LAL_REQUIRE(c != nullptr && descr != nullptr && m >= 0 && n >= 0 && nnz >= 0);
+    LAL_REQUIRE(row_ptr != nullptr && (nnz == 0 || col_ind != nullptr));
+    *descr = nullptr;
+    return lal::csrmv_analysis_dev(c, m, n, nnz, row_ptr, col_ind, exact, descr);
+}
+
+extern "C" int lal_csrmv_descr_destroy(lal_csrmv_descr d)
+{
+    if(!d)
+        return LAL_OK;
+    (void)hipFree(d->d_blocks);
+    delete d;
+    return LAL_OK;
+}
+
+extern "C" int lal_csrmv_f64(lal_handle c, lal_csrmv_descr d, int m, int n, int nnz,
+                             double alpha, const int* rp, const int* ci, const double* v,
+                             const double* x, double beta, double* y)
+{
+    return lal::csrmv_f64(c, d, m, n, nnz, alpha, rp, ci, v, x, beta, y);
+}
+
+extern "C" int lal_csrmv_f32(lal_handle c, lal_csrmv_descr d, int m, int n, int nnz,
+                             float alpha, const int* rp, const int* ci, const float* v,
+                             const float* x, float beta, float* y)
+{
+    return lal::csrmv_f32(c, d, m, n, nnz, alpha, rp, ci, v, x, beta, y);
+}
+
+extern "C" int lal_residual_f64(lal_handle c, lal_csrmv_descr d, int m, int n, int nnz,
+                                const int* rp, const int* ci, const double* v, const double* x,
+                                const double* b, double* r)
+{
+    return lal::residual_f64(c, d, m, n, nnz, rp, ci, v, x, b, r);
+}
+
+extern "C" int lal_diagonal_f64(lal_handle c, int m, const int* rp, const int* ci,
+                                const double* v, double* d)
+{
+    LAL_REQUIRE(c != nullptr && m >= 0);
+    LAL_REQUIRE(m == 0 || (rp != nullptr && d != nullptr));
+    return lal::diagonal_f64(c, m, rp, ci, v, d);
+}

@@ -1,0 +1,129 @@
+// Internal helpers shared by the gfx950 kernels and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+
+#include "../../include/lal.h"
+
+// Deterministic two-stage reductions use a fixed grid of this many blocks.
+#define LAL_RED_BLOCKS 1024
+#define LAL_RED_THREADS 256
+// Device result slots in the context (dot/norm outputs, solver scalars).
+#define LAL_NUM_SLOTS 64
+
+struct lal_context
+{
+    hipStream_t stream     = nullptr;
+    bool        own_stream = false;
+    int         device     = 0;
+    int         num_cu     = 0;
+    // reduction workspace: partials, a ticket counter per reduction slot
+    double*   d_part   = nullptr; // LAL_RED_BLOCKS * 2
+    unsigned* d_ticket = nullptr; // LAL_NUM_SLOTS
+    double*   d_slots  = nullptr; // LAL_NUM_SLOTS device scalars
+    double*   h_slots  = nullptr; // pinned mirror
+    int*      d_flag   = nullptr;
+    int*      h_flag   = nullptr;
+};
+
+namespace lal
+{
+    int  hip_status(hipError_t e, const char* what, const char* file, int line);
+    void set_last_error(const char* msg);
+}
+
+#define LAL_HIP(call)                                                                  \
+    do                                                                                 \
+    {                                                                                  \
+        hipError_t lal_e_ = (call);                                                    \
+        if(lal_e_ != hipSuccess)                                                       \
+            return lal::hip_status(lal_e_, #call, __FILE__, __LINE__);                 \
+    } while(0)
+
+#define LAL_LAUNCHED()                                                                 \
+    do                                                                                 \
+    {                                                                                  \
+        hipError_t lal_e_ = hipGetLastError();                                         \
+        if(lal_e_ != hipSuccess)                                                       \
+            return lal::hip_status(lal_e_, "kernel launch", __FILE__, __LINE__);       \
+    } while(0)
+
+#define LAL_TRY(expr)                                                                  \
+    do                                                                                 \
+    {                                                                                  \
+        int lal_s_ = (expr);                                                           \
+        if(lal_s_ != LAL_OK)                                                           \
+            return lal_s_;                                                             \
+    } while(0)
+
+#define LAL_REQUIRE(cond)                                                              \
+    do                                                                                 \
+    {                                                                                  \
+        if(!(cond))                                                                    \
+        {                                                                              \
+            lal::set_last_error("invalid argument: " #cond);                           \
+            return LAL_ERR_INVALID;                                                    \
+        }                                                                              \
+    } while(0)
+
+namespace lal
+{
+    inline int grid_for(long long n, int threads, int cap = 1 << 30)
+    {
+        long long g = (n + threads - 1) / threads;
+        if(g < 1)
+            g = 1;
+        if(g > cap)
+            g = cap;
+        return (int)g;
+    }
+
+    // Blocks are dispatched round-robin over the 8 XCDs of an MI355X (block b
+    // runs on XCD b % 8).  This maps the blocks of one XCD onto one
+    // contiguous range of logical tiles so that neighbouring rows (and the x
+    // entries they share) stay in that XCD's L2.
+    __device__ __forceinline__ int xcd_tile(int bid, int nb)
+    {
+        const int nx  = 8;
+        int       per = nb / nx;
+        int       rem = nb % nx;
+        int       x   = bid % nx;
+        int       k   = bid / nx;
+        return x * per + (x < rem ? x : rem) + k;
+    }
+
+    template <typename T>
+    __device__ __forceinline__ T wave_sum(T v)
+    {
+#pragma unroll
+        for(int o = 32; o > 0; o >>= 1)
+            v += __shfl_xor(v, o, 64);
+        return v;
+    }
+
+    template <typename T>
+    __device__ __forceinline__ T wave_max(T v)
+    {
+#pragma unroll
+        for(int o = 32; o > 0; o >>= 1)
+        {
+            T w = __shfl_xor(v, o, 64);
+            v   = (w < v) ? v : w;
+        }
+        return v;
+    }
+
+    // Internal launchers shared across translation units (device-resident
+    // results; nothing is copied to the host).
+    int dot_f64_dev(lal_context* c, int n, const double* x, const double* y, double* d_out,
+                    int slot);
+    int dot_f32_dev(lal_context* c, int n, const float* x, const float* y, double* d_out,
+                    int slot);
+    int norm_inf_f64_dev(lal_context* c, int n, const double* x, double* d_out, int slot);
+    int scan_i32_dev(lal_context* c, int n, int32_t* x);
+    int scan_i64_dev(lal_context* c, int n, int64_t* x);
+    int max_i64_dev(lal_context* c, int n, const int64_t* x, int64_t* h_out);
+}

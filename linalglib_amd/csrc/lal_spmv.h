@@ -1,0 +1,72 @@
+// Row-block CSR kernels shared by SpMV, residual and the smoothers.
+//
+// A workgroup of 256 lanes owns a contiguous block of rows [r0, r1) chosen by
+// the analysis (at most 256 rows and, unless a single row is longer, at most
+// CAP nonzeros).  The block's nonzero range is staged through LDS in chunks of
+// CAP entries with coalesced loads: the matrix value and the gathered x value
+// of every nonzero land side by side, so the row sums read LDS only.  With
+// G = 1 each row is summed by one lane in CSR order, which reproduces the
+// host loop bit for bit; sparse blocks (few, long rows) split each row over a
+// group of G lanes and combine the partial sums with a fixed butterfly.
+#pragma once
+
+#include "lal_internal.h"
+
+struct lal_csrmv_info
+{
+    int  m        = 0;
+    int  nnz      = 0;
+    int  nblocks  = 0;
+    int  exact    = 0;
+    int* d_blocks = nullptr; // nblocks + 1 row offsets
+};
+
+namespace lal
+{
+    constexpr int SPMV_THREADS = 256;
+
+    template <typename T>
+    struct spmv_cap
+    {
+        static constexpr int value = 2048;
+    };
+
+    // Lanes per row for a block of `rows` rows.
+    __device__ __forceinline__ int spmv_group(int rows, int exact)
+    {
+        if(exact || rows >= 128)
+            return 1;
+        int g = 1;
+        while(g < 64 && 2 * g * rows <= SPMV_THREADS)
+            g <<= 1;
+        return g;
+    }
+
+    template <typename T>
+    __device__ __forceinline__ T group_sum(T v, int g)
+    {
+        for(int o = g >> 1; o > 0; o >>= 1)
+            v += __shfl_xor(v, o, 64);
+        return v;
+    }
+
+    __device__ __forceinline__ void block_rows(const int* __restrict__ blocks, int m, int tile,
+                                               int* r0, int* r1)
+    {
+        if(blocks)
+        {
+            *r0 = blocks[tile];
+            *r1 = blocks[tile + 1];
+        }
+        else
+        {
+            *r0 = tile * SPMV_THREADS;
+            *r1 = min(m, *r0 + SPMV_THREADS);
+        }
+    }
+
+    inline int spmv_grid(const lal_csrmv_info* d, int m)
+    {
+        return d ? d->nblocks : (m + SPMV_THREADS - 1) / SPMV_THREADS;
+    }
+}

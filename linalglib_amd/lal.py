@@ -1,0 +1,202 @@
+"""ctypes binding of the C ABI in include/lal.h, with torch tensors as the
+device buffers.
+
+torch is plumbing here: it allocates HBM and provides the stream.  Every
+handle is bound to torch's current stream, so library kernels are ordered
+with torch's own copies.  If liblal.so is missing, or the device is not
+gfx950, calls raise LalError -- there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+from . import build as _build
+
+_lib = None
+
+LAL_OK = 0
+STATUS = {
+    0: "ok", 1: "invalid argument", 2: "out of device memory", 3: "HIP runtime error",
+    4: "not converged", 5: "NaN in residual", 6: "no usable device", 7: "unsupported input",
+}
+
+
+class LalError(RuntimeError):
+    def __init__(self, status, where, detail=""):
+        self.status = status
+        super().__init__(f"{where}: {STATUS.get(status, status)}"
+                         + (f" ({detail})" if detail else ""))
+
+
+vp = C.c_void_p
+i32 = C.c_int
+f64 = C.c_double
+f32 = C.c_float
+sz = C.c_size_t
+
+
+class SaamgOptions(C.Structure):
+    _fields_ = [("max_levels", i32), ("eps", f64), ("relax", f64), ("coarse_solver", i32),
+                ("exact", i32), ("build_f32", i32)]
+
+
+class SolveOptions(C.Structure):
+    _fields_ = [("rel_tol", f64), ("abs_tol", f64), ("max_iter", i32), ("n1", i32),
+                ("n2", i32), ("inner_rel_tol", f64), ("inner_max_iter", i32),
+                ("max_outer", i32)]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("iterations", i32), ("outer_iterations", i32), ("converged", i32),
+                ("initial_res_norm", f64), ("final_res_norm", f64)]
+
+
+# name -> argtypes (all return int status unless listed in _RESTYPES)
+_SIGS = {
+    "lal_create": [C.POINTER(vp)],
+    "lal_destroy": [vp],
+    "lal_set_stream": [vp, vp],
+    "lal_get_stream": [vp, C.POINTER(vp)],
+    "lal_sync": [vp],
+    "lal_alloc": [C.POINTER(vp), sz],
+    "lal_free": [vp],
+    "lal_memcpy_h2d": [vp, vp, vp, sz],
+    "lal_memcpy_d2h": [vp, vp, vp, sz],
+    "lal_memcpy_d2d": [vp, vp, vp, sz],
+    "lal_fill_f64": [vp, vp, sz, f64],
+    "lal_fill_f32": [vp, vp, sz, f32],
+    "lal_fill_i32": [vp, vp, sz, C.c_int32],
+    "lal_fill_i64": [vp, vp, sz, C.c_int64],
+    "lal_convert_f64_f32": [vp, vp, vp, sz],
+    "lal_convert_f32_f64": [vp, vp, vp, sz],
+    "lal_axpy_f64": [vp, i32, f64, vp, vp],
+    "lal_axpby_f64": [vp, i32, f64, vp, f64, vp],
+    "lal_axpbypgz_f64": [vp, i32, f64, vp, f64, vp, f64, vp],
+    "lal_axpy_f32": [vp, i32, f32, vp, vp],
+    "lal_axpby_f32": [vp, i32, f32, vp, f32, vp],
+    "lal_dot_f64": [vp, i32, vp, vp, C.POINTER(f64)],
+    "lal_dot_f32": [vp, i32, vp, vp, C.POINTER(f64)],
+    "lal_norm_inf_f64": [vp, i32, vp, C.POINTER(f64)],
+    "lal_norm_inf_f32": [vp, i32, vp, C.POINTER(f64)],
+    "lal_exclusive_scan_i32": [vp, i32, vp],
+    "lal_exclusive_scan_i64": [vp, i32, vp],
+    "lal_csrmv_analysis": [vp, i32, i32, i32, vp, vp, i32, C.POINTER(vp)],
+    "lal_csrmv_descr_destroy": [vp],
+    "lal_csrmv_f64": [vp, vp, i32, i32, i32, f64, vp, vp, vp, vp, f64, vp],
+    "lal_csrmv_f32": [vp, vp, i32, i32, i32, f32, vp, vp, vp, vp, f32, vp],
+    "lal_residual_f64": [vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, vp],
+    "lal_diagonal_f64": [vp, i32, vp, vp, vp, vp],
+    "lal_transpose_f64": [vp, i32, i32, i32, vp, vp, vp, vp, vp, vp],
+    "lal_csrgemm_nnz": [vp, i32, i32, i32, i32, vp, vp, i32, vp, vp, vp, C.POINTER(i32)],
+    "lal_csrgemm_solve_f64": [vp, i32, i32, i32, i32, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp],
+    "lal_sa_strength_f64": [vp, i32, i32, vp, vp, vp, f64, vp],
+    "lal_pmis_aggregate": [vp, i32, i32, vp, vp, vp, vp],
+    "lal_sa_prolongator_nnz": [vp, i32, i32, vp, vp, vp, vp, vp, C.POINTER(i32),
+                               C.POINTER(i32)],
+    "lal_sa_prolongator_fill_f64": [vp, i32, i32, vp, vp, vp, vp, vp, f64, vp, vp, vp],
+    "lal_saamg_default_options": [C.POINTER(SaamgOptions)],
+    "lal_saamg_setup": [vp, i32, i32, i32, vp, vp, vp, C.POINTER(SaamgOptions), C.POINTER(vp)],
+    "lal_hierarchy_destroy": [vp],
+    "lal_hierarchy_levels": [vp, C.POINTER(i32)],
+    "lal_hierarchy_level": [vp, i32, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
+                            C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)],
+    "lal_amg_precond_f64": [vp, vp, vp, vp, i32, i32],
+    "lal_solve_default_options": [C.POINTER(SolveOptions)],
+    "lal_pcg_solve_f64": [vp, vp, vp, vp, C.POINTER(SolveOptions), C.POINTER(SolveInfo)],
+    "lal_ir_solve": [vp, vp, vp, vp, C.POINTER(SolveOptions), C.POINTER(SolveInfo)],
+    "lal_sddmm_f64": [vp, i32, i32, vp, vp, f64, vp, vp, vp],
+    "lal_status_string": [i32],
+    "lal_last_error": [],
+}
+_RESTYPES = {"lal_status_string": C.c_char_p, "lal_last_error": C.c_char_p,
+             "lal_saamg_default_options": None, "lal_solve_default_options": None}
+
+
+def lib_path():
+    return _build.LIB
+
+
+def load():
+    """Loads liblal.so; raises LalError when it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = lib_path()
+    if not os.path.exists(path):
+        raise LalError(6, "load", f"{path} not built: run python -m linalglib_amd.build")
+    L = C.CDLL(path)
+    for name, args in _SIGS.items():
+        fn = getattr(L, name, None)
+        if fn is None:
+            continue
+        fn.argtypes = args
+        fn.restype = _RESTYPES.get(name, C.c_int)
+    _lib = L
+    return L
+
+
+def check(status, where):
+    if status != LAL_OK:
+        detail = ""
+        try:
+            detail = load().lal_last_error().decode()
+        except Exception:
+            pass
+        raise LalError(status, where, detail)
+
+
+def call(name, *args):
+    L = load()
+    check(getattr(L, name)(*args), name)
+
+
+def ptr(t):
+    """Device pointer of a torch tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return C.c_void_p(t.data_ptr())
+
+
+class Handle:
+    """A lal context bound to torch's current stream on the current device."""
+
+    def __init__(self):
+        import torch
+
+        if not torch.cuda.is_available():
+            raise LalError(6, "Handle", "no GPU visible to torch")
+        L = load()
+        h = vp()
+        check(L.lal_create(C.byref(h)), "lal_create")
+        self.h = h
+        self.L = L
+        check(L.lal_set_stream(h, C.c_void_p(torch.cuda.current_stream().cuda_stream)),
+              "lal_set_stream")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.lal_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        check(self.L.lal_sync(self.h), "lal_sync")
+
+
+class CsrDescr:
+    """SpMV analysis descriptor (row blocks) for one matrix pattern."""
+
+    def __init__(self, h, m, n, nnz, row_ptr, col_ind, exact=False):
+        d = vp()
+        check(load().lal_csrmv_analysis(h.h, m, n, nnz, ptr(row_ptr), ptr(col_ind),
+                                        int(exact), C.byref(d)), "lal_csrmv_analysis")
+        self.d = d
+
+    def __del__(self):
+        if getattr(self, "d", None):
+            load().lal_csrmv_descr_destroy(self.d)
+            self.d = None
