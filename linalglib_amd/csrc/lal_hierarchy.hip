@@ -1,0 +1,746 @@
+// SA-AMG hierarchy on the device, the V-cycle preconditioner, SA-AMG-PCG
+// and mixed-precision iterative refinement.
+//
+//   setup      iterative_solvers/amg/saamg.cpp:210-282 (level loop: strength,
+//              PMIS, smoothed P, R = P^T, A_c = R (A P); eps halves per level;
+//              stop when P has no columns or at min(max_levels, 20) levels)
+//   V-cycle    iterative_solvers/amg/amg.cpp:144-267 with apply_smoother(jacobi)
+//   PCG        iterative_solvers/krylov/cg.cpp:136-223, convergence test of
+//              iterative_solvers/iter_control.cpp:38-91
+//
+// Everything stays on the device.  Solver scalars (gamma, alpha, beta) live in
+// device memory and are formed by the last block of each reduction, so a PCG
+// iteration hands one 8-byte residual norm to the host for the convergence
+// test and nothing else.
+#include "lal_vcycle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace lal
+{
+    int sa_strength(lal_context* c, int m, int nnz, const int* rp, const int* ci,
+                    const double* v, double eps, int* conn);
+    int pmis_aggregate(lal_context* c, int m, int nnz, const int* rp, const int* ci,
+                       const int* conn, int64_t* agg);
+    int sa_prolongator_nnz(lal_context* c, int m, const int* rp, const int* ci, const int* conn,
+                           const int64_t* agg, int* prp, int* nnz_P, int* n_c);
+    int sa_prolongator_fill(lal_context* c, int m, const int* rp, const int* ci, const double* v,
+                            const int* conn, const int64_t* agg, double relax, const int* prp,
+                            int* pci, double* pv);
+    int transpose_f64(lal_context* c, int m, int n, int nnz, const int* rp, const int* ci,
+                      const double* v, int* trp, int* tci, double* tv);
+    int csrgemm_nnz(lal_context* c, int m, int k, int n, const int* arp, const int* aci,
+                    const int* brp, const int* bci, int* crp, int* nnz_c);
+    int csrgemm_solve_f64(lal_context* c, int m, int k, int n, const int* arp, const int* aci,
+                          const double* av, const int* brp, const int* bci, const double* bv,
+                          const int* crp, int* cci, double* cv);
+}
+
+namespace
+{
+    struct csr_dev
+    {
+        int     m = 0, n = 0, nnz = 0;
+        int*    rp = nullptr;
+        int*    ci = nullptr;
+        double* v  = nullptr;
+        float*  v32 = nullptr;
+        lal_csrmv_info* d = nullptr;
+
+        void free_all()
+        {
+            (void)hipFree(rp);
+            (void)hipFree(ci);
+            (void)hipFree(v);
+            (void)hipFree(v32);
+            lal_csrmv_descr_destroy(d);
+            rp = ci = nullptr;
+            v       = nullptr;
+            v32     = nullptr;
+            d       = nullptr;
+        }
+    };
+
+    template <typename T>
+    struct work
+    {
+        T* x    = nullptr;
+        T* t    = nullptr;
+        T* b    = nullptr;
+        T* r    = nullptr;
+        T* diag = nullptr;
+    };
+
+    struct level_t
+    {
+        csr_dev      A, P, R;
+        work<double> w64;
+        work<float>  w32;
+    };
+
+    int alloc_bytes(void** p, size_t bytes)
+    {
+        *p = nullptr;
+        if(bytes == 0)
+            return LAL_OK;
+        LAL_HIP(hipMalloc(p, bytes));
+        return LAL_OK;
+    }
+
+    template <typename T>
+    int alloc_n(T** p, size_t n)
+    {
+        return alloc_bytes((void**)p, sizeof(T) * std::max<size_t>(n, 1));
+    }
+
+}
+
+struct lal_hierarchy_t
+{
+    lal_context*      c = nullptr;
+    lal_saamg_options opts{};
+    int               levels = 0;
+    level_t           L[21];
+    // coarsest level
+    int     coarse_kind = 0; // 0 scalar, 1 Gauss-Seidel, 2 dense inverse
+    double* coarse_inv  = nullptr;
+    // PCG workspaces and device scalars
+    double* res = nullptr;
+    double* z   = nullptr;
+    double* p   = nullptr;
+    double* q   = nullptr;
+    float*  res32 = nullptr;
+    float*  z32   = nullptr;
+    float*  p32   = nullptr;
+    float*  q32   = nullptr;
+    float*  d32   = nullptr;
+    double* s     = nullptr; // LAL_SOLVER_SLOTS scalars
+    double* h_s   = nullptr; // pinned mirror
+    bool    has32 = false;
+
+    ~lal_hierarchy_t()
+    {
+        for(int l = 0; l <= 20; l++)
+        {
+            L[l].A.free_all();
+            L[l].P.free_all();
+            L[l].R.free_all();
+            for(double* ptr : {L[l].w64.x, L[l].w64.t, L[l].w64.b, L[l].w64.r, L[l].w64.diag})
+                (void)hipFree(ptr);
+            for(float* ptr : {L[l].w32.x, L[l].w32.t, L[l].w32.b, L[l].w32.r, L[l].w32.diag})
+                (void)hipFree(ptr);
+        }
+        for(void* ptr : {(void*)coarse_inv, (void*)res, (void*)z, (void*)p, (void*)q,
+                         (void*)res32, (void*)z32, (void*)p32, (void*)q32, (void*)d32, (void*)s})
+            (void)hipFree(ptr);
+        if(h_s)
+            (void)hipHostFree(h_s);
+    }
+
+    template <typename T>
+    work<T>& w(int l);
+};
+
+template <>
+work<double>& lal_hierarchy_t::w<double>(int l)
+{
+    return L[l].w64;
+}
+template <>
+work<float>& lal_hierarchy_t::w<float>(int l)
+{
+    return L[l].w32;
+}
+
+namespace
+{
+    template <typename T>
+    const T* vals(const csr_dev& A);
+    template <>
+    const double* vals<double>(const csr_dev& A)
+    {
+        return A.v;
+    }
+    template <>
+    const float* vals<float>(const csr_dev& A)
+    {
+        return A.v32;
+    }
+
+    // Dense inverse of the coarsest operator through its Cholesky factor
+    // (FP64, on the host; the operator is small).  Returns false if a pivot
+    // is not positive.
+    bool dense_inverse(int n, const std::vector<int>& rp, const std::vector<int>& ci,
+                       const std::vector<double>& v, std::vector<double>& inv)
+    {
+        std::vector<double> Lf((size_t)n * n, 0.0);
+        for(int i = 0; i < n; i++)
+            for(int j = rp[i]; j < rp[i + 1]; j++)
+                Lf[(size_t)i * n + ci[j]] = v[j];
+        for(int j = 0; j < n; j++)
+        {
+            double s = Lf[(size_t)j * n + j];
+            for(int k = 0; k < j; k++)
+                s -= Lf[(size_t)j * n + k] * Lf[(size_t)j * n + k];
+            if(!(s > 0.0))
+                return false;
+            double d             = std::sqrt(s);
+            Lf[(size_t)j * n + j] = d;
+            for(int i = j + 1; i < n; i++)
+            {
+                double t = Lf[(size_t)i * n + j];
+                for(int k = 0; k < j; k++)
+                    t -= Lf[(size_t)i * n + k] * Lf[(size_t)j * n + k];
+                Lf[(size_t)i * n + j] = t / d;
+            }
+        }
+        inv.assign((size_t)n * n, 0.0);
+        std::vector<double> col(n);
+        for(int e = 0; e < n; e++)
+        {
+            for(int i = 0; i < n; i++)
+            {
+                double t = (i == e) ? 1.0 : 0.0;
+                for(int k = 0; k < i; k++)
+                    t -= Lf[(size_t)i * n + k] * col[k];
+                col[i] = t / Lf[(size_t)i * n + i];
+            }
+            for(int i = n - 1; i >= 0; i--)
+            {
+                double t = col[i];
+                for(int k = i + 1; k < n; k++)
+                    t -= Lf[(size_t)k * n + i] * col[k];
+                col[i] = t / Lf[(size_t)i * n + i];
+            }
+            for(int i = 0; i < n; i++)
+                inv[(size_t)i * n + e] = col[i];
+        }
+        return true;
+    }
+
+    int to_f32(lal_context* c, const double* x, float** y, size_t n)
+    {
+        LAL_TRY(alloc_n(y, n));
+        if(n)
+            LAL_TRY(lal_convert_f64_f32(c, x, *y, n));
+        return LAL_OK;
+    }
+
+    int finish_setup(lal_hierarchy_t* h)
+    {
+        lal_context* c     = h->c;
+        const int    exact = h->opts.exact;
+        for(int l = 0; l <= h->levels; l++)
+        {
+            level_t& L = h->L[l];
+            LAL_TRY(lal::csrmv_analysis_dev(c, L.A.m, L.A.n, L.A.nnz, L.A.rp, L.A.ci, exact,
+                                            &L.A.d));
+            LAL_TRY(alloc_n(&L.w64.x, L.A.m));
+            LAL_TRY(alloc_n(&L.w64.t, L.A.m));
+            LAL_TRY(alloc_n(&L.w64.b, L.A.m));
+            LAL_TRY(alloc_n(&L.w64.r, L.A.m));
+            LAL_TRY(alloc_n(&L.w64.diag, L.A.m));
+            LAL_TRY(lal::last_diag<double>(c, L.A.m, L.A.rp, L.A.ci, L.A.v, L.w64.diag));
+            if(l < h->levels)
+            {
+                LAL_TRY(lal::csrmv_analysis_dev(c, L.P.m, L.P.n, L.P.nnz, L.P.rp, L.P.ci, exact,
+                                                &L.P.d));
+                LAL_TRY(lal::csrmv_analysis_dev(c, L.R.m, L.R.n, L.R.nnz, L.R.rp, L.R.ci, exact,
+                                                &L.R.d));
+            }
+            if(h->opts.build_f32)
+            {
+                LAL_TRY(to_f32(c, L.A.v, &L.A.v32, L.A.nnz));
+                if(l < h->levels)
+                {
+                    LAL_TRY(to_f32(c, L.P.v, &L.P.v32, L.P.nnz));
+                    LAL_TRY(to_f32(c, L.R.v, &L.R.v32, L.R.nnz));
+                }
+                LAL_TRY(alloc_n(&L.w32.x, L.A.m));
+                LAL_TRY(alloc_n(&L.w32.t, L.A.m));
+                LAL_TRY(alloc_n(&L.w32.b, L.A.m));
+                LAL_TRY(alloc_n(&L.w32.r, L.A.m));
+                LAL_TRY(alloc_n(&L.w32.diag, L.A.m));
+                LAL_TRY(lal::last_diag<float>(c, L.A.m, L.A.rp, L.A.ci, L.A.v32, L.w32.diag));
+            }
+        }
+        // coarsest level
+        csr_dev& Ac = h->L[h->levels].A;
+        if(Ac.m == 1)
+        {
+            if(Ac.nnz < 1)
+            {
+                lal::set_last_error("coarsest 1x1 operator has no entry");
+                return LAL_ERR_UNSUPPORTED;
+            }
+            h->coarse_kind = 0;
+        }
+        else if(h->opts.coarse_solver == 1 && Ac.m <= 4096)
+        {
+            std::vector<int>    rp(Ac.m + 1), ci(Ac.nnz);
+            std::vector<double> v(Ac.nnz), inv;
+            LAL_HIP(hipMemcpyAsync(rp.data(), Ac.rp, sizeof(int) * (Ac.m + 1),
+                                   hipMemcpyDeviceToHost, c->stream));
+            if(Ac.nnz)
+            {
+                LAL_HIP(hipMemcpyAsync(ci.data(), Ac.ci, sizeof(int) * Ac.nnz,
+                                       hipMemcpyDeviceToHost, c->stream));
+                LAL_HIP(hipMemcpyAsync(v.data(), Ac.v, sizeof(double) * Ac.nnz,
+                                       hipMemcpyDeviceToHost, c->stream));
+            }
+            LAL_HIP(hipStreamSynchronize(c->stream));
+            if(dense_inverse(Ac.m, rp, ci, v, inv))
+            {
+                LAL_TRY(alloc_n(&h->coarse_inv, inv.size()));
+                LAL_HIP(hipMemcpyAsync(h->coarse_inv, inv.data(), sizeof(double) * inv.size(),
+                                       hipMemcpyHostToDevice, c->stream));
+                LAL_HIP(hipStreamSynchronize(c->stream));
+                h->coarse_kind = 2;
+            }
+            else
+                h->coarse_kind = 1;
+        }
+        else
+            h->coarse_kind = 1;
+
+        const int n = h->L[0].A.m;
+        LAL_TRY(alloc_n(&h->res, n));
+        LAL_TRY(alloc_n(&h->z, n));
+        LAL_TRY(alloc_n(&h->p, n));
+        LAL_TRY(alloc_n(&h->q, n));
+        LAL_TRY(alloc_n(&h->s, lal::LAL_SOLVER_SLOTS));
+        LAL_HIP(hipHostMalloc((void**)&h->h_s, sizeof(double) * lal::LAL_SOLVER_SLOTS,
+                              hipHostMallocDefault));
+        if(h->opts.build_f32)
+        {
+            LAL_TRY(alloc_n(&h->res32, n));
+            LAL_TRY(alloc_n(&h->z32, n));
+            LAL_TRY(alloc_n(&h->p32, n));
+            LAL_TRY(alloc_n(&h->q32, n));
+            LAL_TRY(alloc_n(&h->d32, n));
+            h->has32 = true;
+        }
+        LAL_HIP(hipStreamSynchronize(c->stream));
+        return LAL_OK;
+    }
+
+    int setup(lal_context* c, int m, int n, int nnz, const int* rp, const int* ci,
+              const double* v, const lal_saamg_options& o, lal_hierarchy_t* h)
+    {
+        h->c    = c;
+        h->opts = o;
+        csr_dev& A0 = h->L[0].A;
+        A0.m = m;
+        A0.n = n;
+        A0.nnz = nnz;
+        LAL_TRY(alloc_n(&A0.rp, m + 1));
+        LAL_TRY(alloc_n(&A0.ci, nnz));
+        LAL_TRY(alloc_n(&A0.v, nnz));
+        LAL_HIP(hipMemcpyAsync(A0.rp, rp, sizeof(int) * (m + 1), hipMemcpyDeviceToDevice,
+                               c->stream));
+        if(nnz)
+        {
+            LAL_HIP(hipMemcpyAsync(A0.ci, ci, sizeof(int) * nnz, hipMemcpyDeviceToDevice,
+                                   c->stream));
+            LAL_HIP(hipMemcpyAsync(A0.v, v, sizeof(double) * nnz, hipMemcpyDeviceToDevice,
+                                   c->stream));
+        }
+        int    max_level = std::min(o.max_levels, 20);
+        double eps       = o.eps;
+        int    level     = 0;
+        while(level < max_level)
+        {
+            csr_dev& Af = h->L[level].A;
+            csr_dev& P  = h->L[level].P;
+            csr_dev& R  = h->L[level].R;
+            csr_dev& Ac = h->L[level + 1].A;
+            int*     conn = nullptr;
+            int64_t* agg  = nullptr;
+            LAL_TRY(alloc_n(&conn, Af.nnz));
+            LAL_TRY(alloc_n(&agg, Af.m));
+            LAL_TRY(lal::sa_strength(c, Af.m, Af.nnz, Af.rp, Af.ci, Af.v, eps, conn));
+            LAL_TRY(lal::pmis_aggregate(c, Af.m, Af.nnz, Af.rp, Af.ci, conn, agg));
+            LAL_TRY(alloc_n(&P.rp, Af.m + 1));
+            int nnzP = 0, nc = 0;
+            LAL_TRY(lal::sa_prolongator_nnz(c, Af.m, Af.rp, Af.ci, conn, agg, P.rp, &nnzP, &nc));
+            if(nc == 0)
+            {
+                (void)hipFree(conn);
+                (void)hipFree(agg);
+                P.free_all();
+                break;
+            }
+            P.m   = Af.m;
+            P.n   = nc;
+            P.nnz = nnzP;
+            LAL_TRY(alloc_n(&P.ci, nnzP));
+            LAL_TRY(alloc_n(&P.v, nnzP));
+            LAL_TRY(lal::sa_prolongator_fill(c, Af.m, Af.rp, Af.ci, Af.v, conn, agg, o.relax,
+                                             P.rp, P.ci, P.v));
+            (void)hipStreamSynchronize(c->stream);
+            (void)hipFree(conn);
+            (void)hipFree(agg);
+            // R = P^T
+            R.m   = nc;
+            R.n   = Af.m;
+            R.nnz = nnzP;
+            LAL_TRY(alloc_n(&R.rp, nc + 1));
+            LAL_TRY(alloc_n(&R.ci, nnzP));
+            LAL_TRY(alloc_n(&R.v, nnzP));
+            LAL_TRY(lal::transpose_f64(c, P.m, P.n, P.nnz, P.rp, P.ci, P.v, R.rp, R.ci, R.v));
+            // AP = A P
+            csr_dev AP;
+            AP.m = Af.m;
+            AP.n = nc;
+            LAL_TRY(alloc_n(&AP.rp, Af.m + 1));
+            LAL_TRY(lal::csrgemm_nnz(c, Af.m, Af.n, nc, Af.rp, Af.ci, P.rp, P.ci, AP.rp,
+                                     &AP.nnz));
+            LAL_TRY(alloc_n(&AP.ci, AP.nnz));
+            LAL_TRY(alloc_n(&AP.v, AP.nnz));
+            LAL_TRY(lal::csrgemm_solve_f64(c, Af.m, Af.n, nc, Af.rp, Af.ci, Af.v, P.rp, P.ci,
+                                           P.v, AP.rp, AP.ci, AP.v));
+            // A_c = R (A P)
+            Ac.m = nc;
+            Ac.n = nc;
+            LAL_TRY(alloc_n(&Ac.rp, nc + 1));
+            LAL_TRY(lal::csrgemm_nnz(c, nc, Af.m, nc, R.rp, R.ci, AP.rp, AP.ci, Ac.rp, &Ac.nnz));
+            LAL_TRY(alloc_n(&Ac.ci, Ac.nnz));
+            LAL_TRY(alloc_n(&Ac.v, Ac.nnz));
+            LAL_TRY(lal::csrgemm_solve_f64(c, nc, Af.m, nc, R.rp, R.ci, R.v, AP.rp, AP.ci, AP.v,
+                                           Ac.rp, Ac.ci, Ac.v));
+            (void)hipStreamSynchronize(c->stream);
+            AP.free_all();
+            level++;
+            eps *= 0.5;
+        }
+        h->levels = level;
+        return finish_setup(h);
+    }
+
+    // One V-cycle at level l for rhs b with the initial guess zero.  The
+    // result lands in dst when given, else in a level buffer; *out gets it.
+    template <typename T>
+    int vcycle(lal_hierarchy_t* h, int l, const T* b, T* dst, int n1, int n2, T** out)
+    {
+        lal_context* c = h->c;
+        level_t&     L = h->L[l];
+        work<T>&     W = h->w<T>(l);
+        const csr_dev& A = L.A;
+        const T*       Av = vals<T>(A);
+        if(l == h->levels)
+        {
+            T* x = dst ? dst : W.x;
+            if(h->coarse_kind == 0)
+                LAL_TRY(lal::coarse_scalar<T>(c, Av, b, x));
+            else if(h->coarse_kind == 2)
+                LAL_TRY(lal::coarse_dense<T>(c, A.m, h->coarse_inv, b, x));
+            else
+                LAL_TRY(lal::coarse_gs<T>(c, A.m, A.rp, A.ci, Av, b, x));
+            *out = x;
+            return LAL_OK;
+        }
+        T* cur   = W.x;
+        T* other = W.t;
+        if(n1 == 0)
+        {
+            // x = 0 and r = b - A*0, which is b bit for bit
+            LAL_HIP(hipMemsetAsync(cur, 0, sizeof(T) * A.m, c->stream));
+            LAL_HIP(hipMemcpyAsync(W.r, b, sizeof(T) * A.m, hipMemcpyDeviceToDevice, c->stream));
+        }
+        else
+        {
+            LAL_TRY(lal::jacobi0_residual<T>(c, A.d, A.m, A.rp, A.ci, Av, W.diag, b, cur, W.r));
+            if(n1 > 1)
+            {
+                for(int k = 1; k < n1; k++)
+                {
+                    LAL_TRY(lal::jacobi_sweep<T>(c, A.d, A.m, A.rp, A.ci, Av, cur, b, other));
+                    std::swap(cur, other);
+                }
+                if(sizeof(T) == 8)
+                    LAL_TRY(lal::residual_f64(c, A.d, A.m, A.n, A.nnz, A.rp, A.ci,
+                                              (const double*)Av, (const double*)cur,
+                                              (const double*)b, (double*)W.r));
+                else
+                    LAL_TRY(lal::residual_f32(c, A.d, A.m, A.n, A.nnz, A.rp, A.ci,
+                                              (const float*)Av, (const float*)cur,
+                                              (const float*)b, (float*)W.r));
+            }
+        }
+        // restriction: b_{l+1} = R r
+        const csr_dev& R  = L.R;
+        work<T>&       Wc = h->w<T>(l + 1);
+        LAL_TRY(lal::csrmv(c, R.d, R.m, R.n, R.nnz, 1.0, R.rp, R.ci, vals<T>(R), W.r, 0.0, Wc.b));
+        T* ec = nullptr;
+        LAL_TRY(vcycle<T>(h, l + 1, Wc.b, nullptr, n1, n2, &ec));
+        // prolongation and correction: x += P ec
+        const csr_dev& P = L.P;
+        LAL_TRY(lal::csrmv(c, P.d, P.m, P.n, P.nnz, 1.0, P.rp, P.ci, vals<T>(P), ec, 1.0, cur));
+        for(int k = 0; k < n2; k++)
+        {
+            T* o = (k == n2 - 1 && dst) ? dst : other;
+            LAL_TRY(lal::jacobi_sweep<T>(c, A.d, A.m, A.rp, A.ci, Av, cur, b, o));
+            other = cur;
+            cur   = o;
+        }
+        if(dst && cur != dst)
+        {
+            LAL_HIP(hipMemcpyAsync(dst, cur, sizeof(T) * A.m, hipMemcpyDeviceToDevice,
+                                   c->stream));
+            cur = dst;
+        }
+        *out = cur;
+        return LAL_OK;
+    }
+
+    template <typename T>
+    int precond(lal_hierarchy_t* h, const T* r, T* z, int n1, int n2)
+    {
+        T* out = nullptr;
+        return vcycle<T>(h, 0, r, z, n1, n2, &out);
+    }
+
+    bool converges(double res, double init, double rel_tol, double abs_tol)
+    {
+        if(std::isnan(res) || std::isnan(init))
+            return false;
+        if(res <= abs_tol)
+            return true;
+        if(res / init <= rel_tol)
+            return true;
+        return false;
+    }
+
+    int read_slot(lal_hierarchy_t* h, int slot, double* v)
+    {
+        LAL_HIP(hipMemcpyAsync(h->h_s + slot, h->s + slot, sizeof(double),
+                               hipMemcpyDeviceToHost, h->c->stream));
+        LAL_HIP(hipStreamSynchronize(h->c->stream));
+        *v = h->h_s[slot];
+        return LAL_OK;
+    }
+
+    // cg_solver::solve_precond on the device.  T = double uses the FP64
+    // levels and workspaces, T = float the FP32 ones.
+    template <typename T>
+    int pcg(lal_hierarchy_t* h, const T* b, T* x, T* res, T* z, T* p, T* q, double rel_tol,
+            double abs_tol, int max_iter, int n1, int n2, lal_solve_info* info)
+    {
+        lal_context*   c  = h->c;
+        const csr_dev& A  = h->L[0].A;
+        const int      n  = A.m;
+        const T*       Av = vals<T>(A);
+        double*        s  = h->s;
+        if(sizeof(T) == 8)
+            LAL_TRY(lal::residual_f64(c, A.d, n, A.n, A.nnz, A.rp, A.ci, (const double*)Av,
+                                      (const double*)x, (const double*)b, (double*)res));
+        else
+            LAL_TRY(lal::residual_f32(c, A.d, n, A.n, A.nnz, A.rp, A.ci, (const float*)Av,
+                                      (const float*)x, (const float*)b, (float*)res));
+        double init = 0.0;
+        LAL_TRY(lal::norm_inf_slot<T>(c, n, res, s, lal::S_RNORM0));
+        LAL_TRY(read_slot(h, lal::S_RNORM0, &init));
+        LAL_TRY(precond<T>(h, res, z, n1, n2));
+        LAL_HIP(hipMemcpyAsync(p, z, sizeof(T) * n, hipMemcpyDeviceToDevice, c->stream));
+        LAL_TRY(lal::dot_store<T>(c, n, z, res, s, lal::S_GAMMA));
+        double rn   = init;
+        int    iter = 0;
+        bool   conv = false;
+        while(!(iter > max_iter))
+        {
+            LAL_TRY(lal::csrmv(c, A.d, n, A.n, A.nnz, 1.0, A.rp, A.ci, Av, p, 0.0, q));
+            LAL_TRY(lal::dot_alpha<T>(c, n, q, p, s));
+            LAL_TRY(lal::update_xr<T>(c, n, p, q, x, res, s));
+            LAL_TRY(read_slot(h, lal::S_RNORM, &rn));
+            if(converges(rn, init, rel_tol, abs_tol))
+            {
+                conv = true;
+                break;
+            }
+            LAL_TRY(precond<T>(h, res, z, n1, n2));
+            LAL_TRY(lal::dot_beta<T>(c, n, z, res, s));
+            LAL_TRY(lal::update_p<T>(c, n, z, p, s));
+            iter++;
+        }
+        if(info)
+        {
+            info->iterations       = iter;
+            info->converged        = conv;
+            info->initial_res_norm = init;
+            info->final_res_norm   = rn;
+        }
+        return LAL_OK;
+    }
+}
+
+extern "C" void lal_saamg_default_options(lal_saamg_options* o)
+{
+    if(!o)
+        return;
+    o->max_levels    = 20;
+    o->eps           = 0.08;
+    o->relax         = 2.0 / 3.0;
+    o->coarse_solver = 1;
+    o->exact         = 0;
+    o->build_f32     = 0;
+}
+
+extern "C" void lal_solve_default_options(lal_solve_options* o)
+{
+    if(!o)
+        return;
+    o->rel_tol        = 1e-6;
+    o->abs_tol        = 1e-6;
+    o->max_iter       = 1000;
+    o->n1             = 1;
+    o->n2             = 1;
+    o->inner_rel_tol  = 1e-4;
+    o->inner_max_iter = 200;
+    o->max_outer      = 30;
+}
+
+extern "C" int lal_saamg_setup(lal_handle c, int m, int n, int nnz, const int* rp,
+                               const int* ci, const double* v, const lal_saamg_options* opts,
+                               lal_hierarchy* out)
+{
+    LAL_REQUIRE(c != nullptr && out != nullptr && m >= 1 && n == m && nnz >= 0);
+    LAL_REQUIRE(rp != nullptr && (nnz == 0 || (ci && v)));
+    *out = nullptr;
+    lal_saamg_options o;
+    lal_saamg_default_options(&o);
+    if(opts)
+        o = *opts;
+    LAL_REQUIRE(o.max_levels >= 0 && o.eps >= 0.0);
+    // validate the input pattern before any setup kernel gathers through it
+    {
+        lal_csrmv_info* d = nullptr;
+        LAL_TRY(lal::csrmv_analysis_dev(c, m, n, nnz, rp, ci, 0, &d));
+        lal_csrmv_descr_destroy(d);
+    }
+    auto* h = new lal_hierarchy_t;
+    int   s = setup(c, m, n, nnz, rp, ci, v, o, h);
+    if(s != LAL_OK)
+    {
+        (void)hipStreamSynchronize(c->stream);
+        delete h;
+        return s;
+    }
+    *out = h;
+    return LAL_OK;
+}
+
+extern "C" int lal_hierarchy_destroy(lal_hierarchy h)
+{
+    if(h)
+    {
+        (void)hipStreamSynchronize(h->c->stream);
+        delete h;
+    }
+    return LAL_OK;
+}
+
+extern "C" int lal_hierarchy_levels(lal_hierarchy h, int* total)
+{
+    LAL_REQUIRE(h != nullptr && total != nullptr);
+    *total = h->levels;
+    return LAL_OK;
+}
+
+extern "C" int lal_hierarchy_level(lal_hierarchy h, int level, int kind, int* m, int* n,
+                                   int* nnz, const int** rp, const int** ci, const double** v)
+{
+    LAL_REQUIRE(h != nullptr && m && n && nnz && rp && ci && v);
+    LAL_REQUIRE(level >= 0 && kind >= 0 && kind <= 2);
+    LAL_REQUIRE(kind == 0 ? level <= h->levels : level < h->levels);
+    const csr_dev& A = kind == 0 ? h->L[level].A : (kind == 1 ? h->L[level].P : h->L[level].R);
+    *m   = A.m;
+    *n   = A.n;
+    *nnz = A.nnz;
+    *rp  = A.rp;
+    *ci  = A.ci;
+    *v   = A.v;
+    return LAL_OK;
+}
+
+extern "C" int lal_amg_precond_f64(lal_handle c, lal_hierarchy h, const double* r, double* z,
+                                   int n1, int n2)
+{
+    LAL_REQUIRE(c != nullptr && h != nullptr && r != nullptr && z != nullptr);
+    LAL_REQUIRE(n1 >= 0 && n2 >= 0 && c == h->c);
+    return precond<double>(h, r, z, n1, n2);
+}
+
+extern "C" int lal_pcg_solve_f64(lal_handle c, lal_hierarchy h, const double* b, double* x,
+                                 const lal_solve_options* opts, lal_solve_info* info)
+{
+    LAL_REQUIRE(c != nullptr && h != nullptr && b != nullptr && x != nullptr && c == h->c);
+    lal_solve_options o;
+    lal_solve_default_options(&o);
+    if(opts)
+        o = *opts;
+    LAL_REQUIRE(o.max_iter >= 0 && o.n1 >= 0 && o.n2 >= 0);
+    return pcg<double>(h, b, x, h->res, h->z, h->p, h->q, o.rel_tol, o.abs_tol, o.max_iter,
+                       o.n1, o.n2, info);
+}
+
+extern "C" int lal_ir_solve(lal_handle c, lal_hierarchy h, const double* b, double* x,
+                            const lal_solve_options* opts, lal_solve_info* info)
+{
+    LAL_REQUIRE(c != nullptr && h != nullptr && b != nullptr && x != nullptr && c == h->c);
+    if(!h->has32)
+    {
+        lal::set_last_error("hierarchy was built without build_f32");
+        return LAL_ERR_INVALID;
+    }
+    lal_solve_options o;
+    lal_solve_default_options(&o);
+    if(opts)
+        o = *opts;
+    const csr_dev& A = h->L[0].A;
+    const int      n = A.m;
+    double*        r = h->q; // FP64 outer residual lives in the FP64 q buffer
+    LAL_TRY(lal::residual_f64(c, A.d, n, A.n, A.nnz, A.rp, A.ci, A.v, x, b, r));
+    LAL_TRY(lal::norm_inf_slot<double>(c, n, r, h->s, lal::S_RNORM0));
+    double init = 0.0;
+    LAL_TRY(read_slot(h, lal::S_RNORM0, &init));
+    double rn    = init;
+    int    outer = 0, inner_total = 0;
+    bool   conv  = false;
+    while(true)
+    {
+        if(converges(rn, init, o.rel_tol, o.abs_tol))
+        {
+            conv = true;
+            break;
+        }
+        if(outer >= o.max_outer)
+            break;
+        // inner FP32 solve of A d = r/|r|; level 0's FP32 rhs buffer is not
+        // used by the V-cycle, so it holds the inner right-hand side
+        float* rhs32 = h->L[0].w32.b;
+        LAL_TRY(lal::scale_cast_f64_f32(c, n, r, 1.0 / rn, rhs32));
+        LAL_HIP(hipMemsetAsync(h->d32, 0, sizeof(float) * n, c->stream));
+        lal_solve_info ii{};
+        LAL_TRY(pcg<float>(h, rhs32, h->d32, h->res32, h->z32, h->p32, h->q32,
+                           o.inner_rel_tol, 1e-30, o.inner_max_iter, o.n1, o.n2, &ii));
+        inner_total += ii.iterations + 1;
+        // x += |r| d in FP64, then the true FP64 residual
+        LAL_TRY(lal::axpy_f32_f64(c, n, rn, h->d32, x));
+        LAL_TRY(lal::residual_f64(c, A.d, n, A.n, A.nnz, A.rp, A.ci, A.v, x, b, r));
+        LAL_TRY(lal::norm_inf_slot<double>(c, n, r, h->s, lal::S_RNORM));
+        LAL_TRY(read_slot(h, lal::S_RNORM, &rn));
+        outer++;
+    }
+    if(info)
+    {
+        info->iterations       = inner_total;
+        info->outer_iterations = outer;
+        info->converged        = conv;
+        info->initial_res_norm = init;
+        info->final_res_norm   = rn;
+    }
+    return LAL_OK;
+}

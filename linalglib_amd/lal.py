@@ -187,6 +187,95 @@ class Handle:
         check(self.L.lal_sync(self.h), "lal_sync")
 
 
+def d2h(handle, dptr, count, dtype):
+    """Copies `count` elements at a raw device pointer into a numpy array."""
+    import numpy as np
+
+    out = np.empty(max(int(count), 0), dtype=dtype)
+    if count:
+        check(load().lal_memcpy_d2h(handle.h, C.c_void_p(out.ctypes.data), dptr,
+                                    out.nbytes), "lal_memcpy_d2h")
+    return out
+
+
+def saamg_options(**kw):
+    o = SaamgOptions()
+    load().lal_saamg_default_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def solve_options(**kw):
+    o = SolveOptions()
+    load().lal_solve_default_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Hierarchy:
+    """SA-AMG hierarchy built on the device from device CSR arrays (torch
+    tensors int32/int32/float64)."""
+
+    KIND = {"A": 0, "P": 1, "R": 2}
+
+    def __init__(self, handle, m, nnz, row_ptr, col_ind, val, **opts):
+        self.handle = handle
+        self.opts = saamg_options(**opts)
+        hp = vp()
+        check(load().lal_saamg_setup(handle.h, m, m, nnz, ptr(row_ptr), ptr(col_ind), ptr(val),
+                                     C.byref(self.opts), C.byref(hp)), "lal_saamg_setup")
+        self.hp = hp
+        self.m = m
+
+    def close(self):
+        if getattr(self, "hp", None):
+            load().lal_hierarchy_destroy(self.hp)
+            self.hp = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def levels(self):
+        n = i32()
+        check(load().lal_hierarchy_levels(self.hp, C.byref(n)), "lal_hierarchy_levels")
+        return n.value
+
+    def level(self, level, kind):
+        """Host copy (row_ptr, col_ind, val, m, n) of A/P/R at a level."""
+        import numpy as np
+
+        m, n, nnz = i32(), i32(), i32()
+        rp, ci, v = vp(), vp(), vp()
+        check(load().lal_hierarchy_level(self.hp, level, self.KIND[kind], C.byref(m), C.byref(n),
+                                         C.byref(nnz), C.byref(rp), C.byref(ci), C.byref(v)),
+              "lal_hierarchy_level")
+        return (d2h(self.handle, rp, m.value + 1, np.int32),
+                d2h(self.handle, ci, nnz.value, np.int32),
+                d2h(self.handle, v, nnz.value, np.float64), m.value, n.value)
+
+    def precond(self, r, z, n1=1, n2=1):
+        call("lal_amg_precond_f64", self.handle.h, self.hp, ptr(r), ptr(z), n1, n2)
+
+    def pcg(self, b, x, **opts):
+        o = solve_options(**opts)
+        info = SolveInfo()
+        call("lal_pcg_solve_f64", self.handle.h, self.hp, ptr(b), ptr(x), C.byref(o),
+             C.byref(info))
+        return info
+
+    def ir(self, b, x, **opts):
+        o = solve_options(**opts)
+        info = SolveInfo()
+        call("lal_ir_solve", self.handle.h, self.hp, ptr(b), ptr(x), C.byref(o), C.byref(info))
+        return info
+
+
 class CsrDescr:
     """SpMV analysis descriptor (row blocks) for one matrix pattern."""
 

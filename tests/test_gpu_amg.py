@@ -1,0 +1,280 @@
+"""GPU parity of the SA-AMG setup, the V-cycle and the PCG / mixed-precision
+solves against the CPU oracle and the reference's golden fixtures, through the
+C ABI."""
+import ctypes as C
+import hashlib
+
+import numpy as np
+import pytest
+import torch
+
+from linalglib_amd import lal, problems as pb
+from oracle.oracle import CSR
+from tests.golden.make_golden import grid, random_csr
+from tests.gpu_util import DevCSR, bits_equal, dev, empty, host, log
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def h():
+    return lal.Handle()
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def csr_equal(got, ref):
+    rp, ci, v, m, n = got
+    return (m == ref.m and n == ref.n and np.array_equal(rp, ref.row_ptr)
+            and np.array_equal(ci, ref.col_ind) and bits_equal(v, ref.val))
+
+
+# ---------------------------------------------------------------- setup pieces
+
+
+def test_strength_aggregates_prolongator(h, oracle):
+    cases = [("diff12", grid("diff12")), ("poisson3d10", pb.poisson3d(10)),
+             ("poisson2d33", pb.poisson2d(33))]
+    for name, (rp, ci, v, m) in cases:
+        A = CSR(rp, ci, v, m, m)
+        d = DevCSR(A)
+        for eps in (0.08, 0.25, 0.01):
+            log(f"setup pieces {name} eps={eps}")
+            conn = empty(A.nnz, torch.int32)
+            lal.call("lal_sa_strength_f64", h.h, m, A.nnz, lal.ptr(d.row_ptr),
+                     lal.ptr(d.col_ind), lal.ptr(d.val), eps, lal.ptr(conn))
+            c_ref = oracle.strength(A, eps)
+            assert np.array_equal(host(conn)[: A.nnz], c_ref)
+            agg = empty(m, torch.int64)
+            lal.call("lal_pmis_aggregate", h.h, m, A.nnz, lal.ptr(d.row_ptr), lal.ptr(d.col_ind),
+                     lal.ptr(conn), lal.ptr(agg))
+            a_ref = oracle.aggregates(A, c_ref)
+            assert np.array_equal(host(agg), a_ref)
+            prp = empty(m + 1, torch.int32)
+            nnz_p, nc = C.c_int(), C.c_int()
+            lal.call("lal_sa_prolongator_nnz", h.h, m, A.nnz, lal.ptr(d.row_ptr),
+                     lal.ptr(d.col_ind), lal.ptr(conn), lal.ptr(agg), lal.ptr(prp),
+                     C.byref(nnz_p), C.byref(nc))
+            P_ref = oracle.prolongator(A, c_ref, a_ref)
+            assert (nnz_p.value, nc.value) == (P_ref.nnz, P_ref.n)
+            pci = empty(nnz_p.value, torch.int32)
+            pv = empty(nnz_p.value, torch.float64)
+            lal.call("lal_sa_prolongator_fill_f64", h.h, m, A.nnz, lal.ptr(d.row_ptr),
+                     lal.ptr(d.col_ind), lal.ptr(d.val), lal.ptr(conn), lal.ptr(agg), 2.0 / 3.0,
+                     lal.ptr(prp), lal.ptr(pci), lal.ptr(pv))
+            got = (host(prp), host(pci)[: P_ref.nnz], host(pv)[: P_ref.nnz], m, nc.value)
+            assert csr_equal(got, P_ref)
+
+
+def _gpu_transpose(h, A):
+
+
+    d = DevCSR(A)
+    trp = empty(A.n + 1, torch.int32)
+    tci = empty(A.nnz, torch.int32)
+    tv = empty(A.nnz, torch.float64)
+    lal.call("lal_transpose_f64", h.h, A.m, A.n, A.nnz, lal.ptr(d.row_ptr), lal.ptr(d.col_ind),
+             lal.ptr(d.val), lal.ptr(trp), lal.ptr(tci), lal.ptr(tv))
+    return host(trp), host(tci)[: A.nnz], host(tv)[: A.nnz], A.n, A.m
+
+
+def _gpu_spgemm(h, A, B):
+
+
+    da, db = DevCSR(A), DevCSR(B)
+    crp = empty(A.m + 1, torch.int32)
+    nnz = C.c_int()
+    lal.call("lal_csrgemm_nnz", h.h, A.m, A.n, B.n, A.nnz, lal.ptr(da.row_ptr),
+             lal.ptr(da.col_ind), B.nnz, lal.ptr(db.row_ptr), lal.ptr(db.col_ind), lal.ptr(crp),
+             C.byref(nnz))
+    cci = empty(nnz.value, torch.int32)
+    cv = empty(nnz.value, torch.float64)
+    lal.call("lal_csrgemm_solve_f64", h.h, A.m, A.n, B.n, A.nnz, lal.ptr(da.row_ptr),
+             lal.ptr(da.col_ind), lal.ptr(da.val), B.nnz, lal.ptr(db.row_ptr),
+             lal.ptr(db.col_ind), lal.ptr(db.val), lal.ptr(crp), lal.ptr(cci), lal.ptr(cv))
+    return host(crp), host(cci)[: nnz.value], host(cv)[: nnz.value], A.m, B.n
+
+
+def test_transpose_and_spgemm_golden(h, oracle, golden):
+    g = golden("ops")
+
+    def mk(p, m, n):
+        return CSR(g[p + "_rp"], g[p + "_ci"], g[p + "_v"], m, n)
+
+    log("spgemm golden")
+    for a, b, c, (m, k, n) in (("g1", "g2", "g", (120, 90, 110)), ("h1", "h2", "h", (80, 70, 60))):
+        got = _gpu_spgemm(h, mk(a, m, k), mk(b, k, n))
+        assert csr_equal(got, mk(c, m, n)), c
+    got = _gpu_transpose(h, mk("h1", 80, 70))
+    assert csr_equal(got, mk("t", 70, 80))
+
+
+def test_spgemm_bins_and_global_tables(h, oracle):
+    """Rows spanning every hash-table size, including rows beyond the LDS
+    tables (products > 2048) and empty rows."""
+    rng = np.random.default_rng(21)
+    m, k, n = 300, 400, 6000
+    lens = rng.integers(0, 6, m)
+    lens[:40] = rng.integers(40, 120, 40)  # many products
+    rows = [np.sort(rng.choice(k, size=x, replace=False)) for x in lens]
+    rp = np.zeros(m + 1, np.int32)
+    rp[1:] = np.cumsum(lens)
+    A = CSR(rp, np.concatenate(rows).astype(np.int32), rng.uniform(-1, 1, rp[-1]), m, k)
+    B = random_csr(rng, k, n, 90)
+    log("spgemm bins")
+    got = _gpu_spgemm(h, A, B)
+    assert csr_equal(got, oracle.csrgemm(A, B))
+    # a narrow B (few columns) caps the tables by the column count
+    Bn = random_csr(rng, k, 50, 30)
+    assert csr_equal(_gpu_spgemm(h, A, Bn), oracle.csrgemm(A, Bn))
+    T = random_csr(rng, 500, 3000, 40)
+    assert csr_equal(_gpu_transpose(h, T), oracle.transpose(T))
+
+
+# ------------------------------------------------------------------ hierarchy
+
+
+def _hier(h, A, **kw):
+    d = DevCSR(A)
+    H = lal.Hierarchy(h, A.m, A.nnz, d.row_ptr, d.col_ind, d.val, **kw)
+    H._keep = d
+    return H
+
+
+@pytest.mark.parametrize("name,full", [("2d8", True), ("3d8", True), ("diff12", True),
+                                       ("2d128", False), ("3d16", False)])
+def test_hierarchy_matches_reference_golden(h, golden, name, full):
+    g = golden(name)
+    rp, ci, v, m = grid(name)
+    log(f"hierarchy {name}")
+    H = _hier(h, CSR(rp, ci, v, m, m))
+    assert H.levels == int(g["levels"])
+    for lvl in range(H.levels + 1):
+        for kind in "APR":
+            key = f"{kind}{lvl}"
+            if kind != "A" and lvl == H.levels:
+                continue
+            grp, gci, gv, gm, gn = H.level(lvl, kind)
+            assert list(g[key + "_dims"]) == [gm, gn, len(gci)], key
+            if full:
+                assert np.array_equal(grp, g[key + "_rp"]), key
+                assert np.array_equal(gci, g[key + "_ci"]), key
+                assert bits_equal(gv, g[key + "_v"]), key
+            else:
+                assert list(g[key + "_sha"]) == [sha(grp), sha(gci), sha(gv)], key
+
+
+def test_hierarchy_matches_oracle_3d48(h, oracle):
+    rp, ci, v, m = pb.poisson3d(48)
+    A = CSR(rp, ci, v, m, m)
+    log("hierarchy 3d48")
+    H = _hier(h, A)
+    ho = oracle.saamg_setup(A)
+    assert H.levels == ho.levels
+    for lvl in range(H.levels + 1):
+        for kind in "APR":
+            if kind != "A" and lvl == H.levels:
+                continue
+            assert csr_equal(H.level(lvl, kind), ho.get(lvl, kind)), (lvl, kind)
+
+
+@pytest.mark.parametrize("name", ["2d8", "3d8", "diff12", "2d128", "3d16"])
+def test_vcycle_bitwise_reference_golden(h, golden, name):
+    """One V-cycle (1+1 Jacobi) with sequential row sums and the reference's
+    coarse solve reproduces the reference's preconditioner output bit for bit."""
+    g = golden(name)
+    rp, ci, v, m = grid(name)
+    log(f"vcycle {name}")
+    H = _hier(h, CSR(rp, ci, v, m, m), exact=1, coarse_solver=0)
+    b = dev(g["b"])
+    z = empty(m, torch.float64)
+    H.precond(b, z, 1, 1)
+    assert bits_equal(host(z), g["precond_z"])
+
+
+@pytest.mark.parametrize("n1,n2", [(0, 1), (2, 0), (2, 3)])
+def test_vcycle_sweep_counts_bitwise(h, oracle, n1, n2):
+    rp, ci, v, m = pb.poisson3d(14)
+    A = CSR(rp, ci, v, m, m)
+    ho = oracle.saamg_setup(A)
+    r = pb.uniform(m, 4)
+    log(f"vcycle n1={n1} n2={n2}")
+    H = _hier(h, A, exact=1, coarse_solver=0)
+    R = dev(r)
+    z = empty(m, torch.float64)
+    H.precond(R, z, n1, n2)
+    assert bits_equal(host(z), ho.precond(r, n1, n2, 0))
+
+
+@pytest.mark.parametrize("name", ["2d128", "3d16", "diff12"])
+def test_pcg_matches_reference(h, golden, name):
+    """FP64 SA-AMG-PCG: the GPU reductions sum in another order than the
+    host, so the trajectory differs by rounding only: same iteration count
+    (+-1), solution within 1e-10 relative of the reference's."""
+    g = golden(name)
+    rp, ci, v, m = grid(name)
+    log(f"pcg {name}")
+    H = _hier(h, CSR(rp, ci, v, m, m))
+    b = dev(g["b"])
+    for tag, tol in (("8", 1e-8), ("10", 1e-10)):
+        x = empty(m, torch.float64)
+        info = H.pcg(b, x, rel_tol=tol, abs_tol=1e-300, max_iter=1000)
+        assert info.converged
+        assert abs(info.iterations - int(g[f"pcg{tag}_iters"])) <= 1
+        xr = g[f"pcg{tag}_x"]
+        xg = host(x)
+        assert np.linalg.norm(xg - xr) <= 1e-10 * np.linalg.norm(xr) * (1e2 if tag == "8" else 1)
+        A = CSR(rp, ci, v, m, m)
+        res = g["b"] - A.to_scipy() @ xg
+        assert np.abs(res).max() <= 1.01 * tol * np.abs(g["b"]).max()
+
+
+def test_pcg_3d64_against_oracle(h, oracle):
+    rp, ci, v, m = pb.poisson3d(64)
+    A = CSR(rp, ci, v, m, m)
+    xs = pb.uniform(m, seed=1)
+    b = pb.rhs_from_solution(rp, ci, v, xs)
+    log("pcg 3d64")
+    ho = oracle.saamg_setup(A)
+    xo, ito, _ = ho.pcg(b, rel_tol=1e-10, coarse_solver=1)
+    H = _hier(h, A)
+    B = dev(b)
+    x = empty(m, torch.float64)
+    info = H.pcg(B, x, rel_tol=1e-10, abs_tol=1e-300, max_iter=500)
+    xg = host(x)
+    assert info.converged and abs(info.iterations - ito) <= 1
+    assert np.linalg.norm(xg - xo) <= 1e-10 * np.linalg.norm(xo) * 10
+    assert np.linalg.norm(xg - xs) <= 1e-8 * np.linalg.norm(xs)
+
+
+@pytest.mark.parametrize("name", ["3d16", "diff12", "2d128"])
+def test_mixed_precision_ir(h, oracle, name):
+    rp, ci, v, m = grid(name)
+    A = CSR(rp, ci, v, m, m)
+    b = pb.rhs_from_solution(rp, ci, v, pb.uniform(m, seed=1))
+    log(f"ir {name}")
+    H = _hier(h, A, build_f32=1)
+    B = dev(b)
+    x = empty(m, torch.float64)
+    info = H.ir(B, x, rel_tol=1e-10, abs_tol=1e-300, inner_rel_tol=1e-4)
+    assert info.converged and info.outer_iterations >= 2
+    xg = host(x)
+    r = oracle.residual(A, xg, b)
+    assert np.abs(r).max() <= 1e-10 * np.abs(b).max()
+    xo, _, _ = oracle.saamg_setup(A).pcg(b, rel_tol=1e-12, coarse_solver=1)
+    assert np.linalg.norm(xg - xo) <= 1e-8 * np.linalg.norm(xo)
+
+
+def test_sddmm(h):
+    A = random_csr(np.random.default_rng(8), 200, 200, 9)
+    d = DevCSR(A)
+    u = np.random.default_rng(1).uniform(-1, 1, 200)
+    w = np.random.default_rng(2).uniform(-1, 1, 200)
+    U, W = dev(u), dev(w)
+    out = empty(A.nnz, torch.float64)
+    lal.call("lal_sddmm_f64", h.h, A.m, A.nnz, lal.ptr(d.row_ptr), lal.ptr(d.col_ind), -1.0,
+             lal.ptr(U), lal.ptr(W), lal.ptr(out))
+    rows = np.repeat(np.arange(200), np.diff(A.row_ptr))
+    assert bits_equal(host(out)[: A.nnz], (-1.0 * u[rows]) * w[A.col_ind])
