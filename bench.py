@@ -1,0 +1,286 @@
+#!/usr/bin/env python3
+"""SA-AMG-PCG on one MI355X: solve seconds to rel. residual 1e-10, mixed
+FP32/FP64 against plain FP64, and standalone CSR SpMV GB/s.
+
+Workload: 3D 7-point Poisson on a 128^3 grid (n = 2,097,152, nnz = 14,581,760),
+FP64 CSR with int32 indices, x* ~ U[-1,1] (seed 1), b = A x*, x0 = 0.  SA-AMG
+hierarchy (strength eps = 0.08 halved per level, smoothed prolongator with
+relax = 2/3), one V-cycle with 1 pre + 1 post undamped Jacobi sweep per
+preconditioner application, PCG to ||r||_inf / ||r0||_inf <= 1e-10.
+
+A step is one complete solve from x = 0 with the hierarchy resident in HBM
+(setup is timed once and reported separately).  `value` is the mixed
+precision solve time (FP32 inner SA-AMG-PCG to 1e-4, FP64 refinement to
+1e-10); the plain FP64 solve time is reported beside it.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1]
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nx", type=int, default=128)
+    ap.add_argument("--spmv-calls", type=int, default=200)
+    ap.add_argument("--spmv-only", action="store_true",
+                    help="only the standalone SpMV loop (for counter runs)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def spmv_bytes(m, n, nnz, vbytes, xbytes, beta=False):
+    """Algorithmic bytes of y = A x: values + int32 columns per nonzero,
+    int32 row pointers, x read once, y written (and read when beta != 0)."""
+    return nnz * (vbytes + 4) + 4 * (m + 1) + xbytes * n + xbytes * m * (2 if beta else 1)
+
+
+def solve_bytes(levels, iters, vbytes):
+    """Algorithmic bytes of a SA-AMG-PCG solve: per iteration one SpMV with A,
+    one V-cycle, two dot products, one inf-norm and three axpy-class updates
+    (vectors of vbytes each); levels = [(m, nnz_A, nnz_P)] finest first, the
+    last entry being the coarsest operator."""
+    v = vbytes
+    m0, nnz0, _ = levels[0]
+    prec = 0
+    for (m, nnz, nnz_p), (mc, _, _) in zip(levels[:-1], levels[1:]):
+        jac0 = 3 * v * m                                   # x = b / a_ii
+        res = nnz * (v + 4) + 4 * (m + 1) + v * m + 2 * v * m
+        restr = spmv_bytes(mc, m, nnz_p, v, v)
+        prol = spmv_bytes(m, mc, nnz_p, v, v, beta=True)
+        post = nnz * (v + 4) + 4 * (m + 1) + 3 * v * m
+        prec += jac0 + res + restr + prol + post
+    per_iter = spmv_bytes(m0, m0, nnz0, v, v) + prec + 2 * 2 * v * m0 + v * m0 + 3 * 3 * v * m0
+    return iters * per_iter
+
+
+def cpu_baseline(rp, ci, v, m, b, threads):
+    """LinAlgLib's own host path (oracle/_ref, compiled from its sources) on
+    the same 128^3 problem when it was built, else the C oracle; SA setup and
+    one SA-AMG-PCG solve to 1e-10, timed on this machine's host cores."""
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, here)
+    from oracle.oracle import CSR, Oracle, Reference
+
+    A = CSR(rp, ci, v, m, m)
+    if Reference.available():
+        lib, kind = Reference(), "reference"
+    else:
+        lib, kind = Oracle(), "port"
+    lib.set_threads(threads)
+    t0 = time.perf_counter()
+    h = lib.saamg_setup(A)
+    t1 = time.perf_counter()
+    out = h.pcg(b, rel_tol=1e-10, abs_tol=1e-300, max_iter=1000)
+    t2 = time.perf_counter()
+    iters = out[1]
+    return {
+        "value": round(t2 - t1, 4),
+        "unit": "s per SA-AMG-PCG solve to 1e-10 (FP64)",
+        "cores": threads,
+        "kind": kind,
+        "sample": (f"one full solve of the same 3D Poisson {round(m ** (1 / 3))}^3 problem "
+                   f"(setup {t1 - t0:.2f} s excluded, {iters} PCG iterations, "
+                   f"{h.levels} levels), OMP_NUM_THREADS={threads}"),
+        "setup_s": round(t1 - t0, 3),
+        "iterations": iters,
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the SA-AMG-PCG path here is built and "
+              "measured on a single MI355X (no multi-GPU code); use --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from linalglib_amd import lal, problems
+
+    nx = args.nx
+    rp, ci, v, m = problems.poisson3d(nx)
+    nnz = int(rp[-1])
+    xs = problems.uniform(m, seed=1)
+    b = problems.rhs_from_solution(rp, ci, v, xs)
+
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    h = lal.Handle()
+    RP = torch.from_numpy(rp).to(dev)
+    CI = torch.from_numpy(ci).to(dev)
+    V = torch.from_numpy(v).to(dev)
+    B = torch.from_numpy(b).to(dev)
+    torch.cuda.synchronize()
+
+    # ---------------------------------------------------------------- SpMV
+    def spmv_loop(dtype, calls):
+        vb = 8 if dtype == torch.float64 else 4
+        fn = "lal_csrmv_f64" if vb == 8 else "lal_csrmv_f32"
+        desc = lal.CsrDescr(h, m, m, nnz, RP, CI)
+        copies = []
+        for k in range(3):  # 3 rotating copies: > 256 MB Infinity Cache
+            copies.append((RP.clone(), CI.clone(), V.to(dtype).clone(),
+                           torch.rand(m, dtype=dtype, device=dev) * 2 - 1,
+                           torch.zeros(m, dtype=dtype, device=dev)))
+        L = lal.load()
+        f = getattr(L, fn)
+        argv = [(h.h, desc.d, m, m, nnz, 1.0, lal.ptr(c[0]), lal.ptr(c[1]), lal.ptr(c[2]),
+                 lal.ptr(c[3]), 0.0, lal.ptr(c[4])) for c in copies]
+        for k in range(4):
+            f(*argv[k % 3])
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for k in range(calls):
+            f(*argv[k % 3])
+        e1.record()
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1) / 1e3 / calls
+        # same loop on one copy (Infinity Cache resident)
+        e0.record()
+        for k in range(calls):
+            f(*argv[0])
+        e1.record()
+        torch.cuda.synchronize()
+        t_hot = e0.elapsed_time(e1) / 1e3 / calls
+        byts = spmv_bytes(m, m, nnz, vb, vb)
+        del copies
+        return t, t_hot, byts
+
+    t64, t64_hot, b64 = spmv_loop(torch.float64, args.spmv_calls)
+    t32, t32_hot, b32 = spmv_loop(torch.float32, args.spmv_calls)
+    print(f"# spmv fp64 {t64 * 1e6:.2f} us ({b64 / t64 / 1e9:.0f} GB/s, cache-resident "
+          f"{b64 / t64_hot / 1e9:.0f}); fp32 {t32 * 1e6:.2f} us ({b32 / t32 / 1e9:.0f} GB/s)",
+          file=sys.stderr, flush=True)
+    if args.spmv_only:
+        return
+
+    # ---------------------------------------------------------------- setup
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    H = lal.Hierarchy(h, m, nnz, RP, CI, V, build_f32=1)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - ts
+    levels = []
+    for lvl in range(H.levels + 1):
+        arp, aci, _, lm, _ = H.level(lvl, "A")
+        pnnz = len(H.level(lvl, "P")[1]) if lvl < H.levels else 0
+        levels.append((lm, len(aci), pnnz))
+    print(f"# setup {setup_s:.3f} s, levels {[(l[0], l[1]) for l in levels]}", file=sys.stderr,
+          flush=True)
+
+    # ---------------------------------------------------------------- solves
+    X = torch.zeros(m, dtype=torch.float64, device=dev)
+
+    def run(kind):
+        X.zero_()
+        if kind == "fp64":
+            return H.pcg(B, X, rel_tol=1e-10, abs_tol=1e-300, max_iter=1000, n1=1, n2=1)
+        return H.ir(B, X, rel_tol=1e-10, abs_tol=1e-300, n1=1, n2=1, inner_rel_tol=1e-4,
+                    inner_max_iter=200, max_outer=30)
+
+    results = {}
+    for kind in ("fp64", "mixed"):
+        for _ in range(args.warmup):
+            run(kind)
+        torch.cuda.synchronize()
+        times = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            s0 = time.perf_counter()
+            info = run(kind)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - s0)
+        total = time.perf_counter() - t0
+        x = X.cpu().numpy()
+        r = b - (problems.rhs_from_solution(rp, ci, v, x))
+        results[kind] = {
+            "total_s": total,
+            "median_s": statistics.median(times),
+            "iters": info.iterations,
+            "outer": info.outer_iterations,
+            "converged": bool(info.converged),
+            "rel_res_inf": float(np.abs(r).max() / np.abs(b).max()),
+            "rel_err": float(np.linalg.norm(x - xs) / np.linalg.norm(xs)),
+        }
+        print(f"# {kind}: {results[kind]}", file=sys.stderr, flush=True)
+
+    f64, mx = results["fp64"], results["mixed"]
+    solve64_gbs = solve_bytes(levels, f64["iters"], 8) / f64["median_s"] / 1e9
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        try:
+            cpu = cpu_baseline(rp, ci, v, m, b, threads)
+        except Exception as e:  # the baseline is context only
+            cpu = {"value": None, "unit": "s", "cores": threads, "kind": "reference",
+                   "sample": f"not run: {e}"}
+
+    achieved = b64 / t64 / 1e9
+    line = {
+        "metric": "SA-AMG-PCG solve sec to 1e-10 & SpMV GB/s: mixed vs FP64, 1 MI355X",
+        "value": round(mx["total_s"] / args.steps, 6),
+        "unit": "s per solve (mixed FP32/FP64 SA-AMG-PCG to rel. residual 1e-10)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mx["total_s"] / args.steps * 1e3, 4),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"3D 7-point Poisson {nx}^3 SA-AMG-PCG to 1e-10 (1+1 Jacobi V-cycle)",
+            "n": m, "nnz": nnz, "levels": H.levels,
+            "level_sizes": [[l[0], l[1]] for l in levels],
+            "parallelism": "single GPU",
+        },
+        "fp64_solve_s": round(f64["total_s"] / args.steps, 6),
+        "mixed_solve_s": round(mx["total_s"] / args.steps, 6),
+        "speedup_mixed_vs_fp64": round(f64["total_s"] / mx["total_s"], 3),
+        "fp64_iterations": f64["iters"],
+        "mixed_inner_iterations": mx["iters"],
+        "mixed_outer_iterations": mx["outer"],
+        "fp64_rel_residual_inf": f64["rel_res_inf"],
+        "mixed_rel_residual_inf": mx["rel_res_inf"],
+        "fp64_rel_err_vs_xstar": f64["rel_err"],
+        "mixed_rel_err_vs_xstar": mx["rel_err"],
+        "setup_s": round(setup_s, 4),
+        "spmv_fp64_gbs": round(achieved, 1),
+        "spmv_fp32_gbs": round(b32 / t32 / 1e9, 1),
+        "spmv_fp64_cache_resident_gbs": round(b64 / t64_hot / 1e9, 1),
+        "spmv_fp64_us": round(t64 * 1e6, 3),
+        "solve_fp64_algorithmic_gbs": round(solve64_gbs, 1),
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "csrmv_kernel<double> (FP64 CSR SpMV, 3 rotating copies)",
+            "bytes_per_launch": b64,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()
