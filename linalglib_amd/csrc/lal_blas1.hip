@@ -63,17 +63,33 @@ namespace
         __shared__ double sh[LAL_RED_THREADS / 64];
         __shared__ bool   last;
         const int         tid = threadIdx.x;
-        double            acc = 0.0;
-        for(int i = blockIdx.x * LAL_RED_THREADS + tid; i < n; i += gridDim.x * LAL_RED_THREADS)
-        {
+        double            acc    = 0.0;
+        const int         stride = gridDim.x * LAL_RED_THREADS;
+        auto              add    = [&](T a, T b) {
             if(OP == 0)
-                acc = acc + (double)x[i] * (double)y[i];
+                acc = acc + (double)a * (double)b;
             else
             {
-                double a = fabs((double)x[i]);
-                acc      = (a < acc) ? acc : a;
+                double t = fabs((double)a);
+                acc      = (t < acc) ? acc : t;
             }
+        };
+        int i = blockIdx.x * LAL_RED_THREADS + tid;
+        for(; i + 3 * stride < n; i += 4 * stride)
+        {
+            T a[4], b[4];
+#pragma unroll
+            for(int k = 0; k < 4; k++)
+            {
+                a[k] = x[i + k * stride];
+                b[k] = OP == 0 ? y[i + k * stride] : a[k];
+            }
+#pragma unroll
+            for(int k = 0; k < 4; k++)
+                add(a[k], b[k]);
         }
+        for(; i < n; i += stride)
+            add(x[i], OP == 0 ? y[i] : x[i]);
         acc = OP == 0 ? lal::wave_sum(acc) : lal::wave_max(acc);
         if((tid & 63) == 0)
             sh[tid >> 6] = acc;

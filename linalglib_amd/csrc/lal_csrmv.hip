@@ -5,7 +5,7 @@
 // fma(alpha, s, beta*y)); r = b - A*x follows :35-57 (s += val*x, two
 // roundings).  The row-block layout and LDS staging are described in
 // lal_spmv.h.
-#include "lal_spmv.h"
+#include "lal_rowvec.h"
 
 #include <vector>
 
@@ -123,6 +123,12 @@ namespace
         LAL_REQUIRE(nnz == 0 || (ci != nullptr && v != nullptr && x != nullptr));
         if(d)
             LAL_REQUIRE(d->m == m && d->nnz == nnz);
+        if(d && !d->exact)
+        {
+            lal::rv_args a{m,    d->d_blocks, d->nblocks,     rp,           ci, v, x, nullptr,
+                           b,    y,           nullptr,        (double)alpha, (double)beta};
+            return lal::rowvec_launch<T>(c, MODE == 0 ? lal::RV_SPMV : lal::RV_RESIDUAL, a);
+        }
         int nb = lal::spmv_grid(d, m);
         csrmv_kernel<T, ACC, MODE><<<nb, SPMV_THREADS, 0, c->stream>>>(
             d ? d->d_blocks : nullptr, nb, m, rp, ci, v, x, alpha, beta, b, y,

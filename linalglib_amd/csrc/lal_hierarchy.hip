@@ -71,7 +71,8 @@ namespace
         T* t    = nullptr;
         T* b    = nullptr;
         T* r    = nullptr;
-        T* diag = nullptr;
+        T* diag = nullptr; // a_ii (exact path)
+        T* dinv = nullptr; // 1/a_ii (fast path)
     };
 
     struct level_t
@@ -118,8 +119,15 @@ struct lal_hierarchy_t
     float*  q32   = nullptr;
     float*  d32   = nullptr;
     double* s     = nullptr; // LAL_SOLVER_SLOTS scalars
-    double* h_s   = nullptr; // pinned mirror
-    bool    has32 = false;
+    double* part  = nullptr; // 4 x PCG_BLOCKS reduction partials
+    // residual norms reach the host through mapped pinned memory: a value
+    // and a sequence number the host polls
+    double*   h_note     = nullptr;
+    double*   d_note     = nullptr;
+    unsigned* h_seq      = nullptr;
+    unsigned* d_seq      = nullptr;
+    unsigned  seq        = 0;
+    bool      has32      = false;
 
     ~lal_hierarchy_t()
     {
@@ -128,16 +136,19 @@ struct lal_hierarchy_t
             L[l].A.free_all();
             L[l].P.free_all();
             L[l].R.free_all();
-            for(double* ptr : {L[l].w64.x, L[l].w64.t, L[l].w64.b, L[l].w64.r, L[l].w64.diag})
+            for(double* ptr : {L[l].w64.x, L[l].w64.t, L[l].w64.b, L[l].w64.r, L[l].w64.diag,
+                               L[l].w64.dinv})
                 (void)hipFree(ptr);
-            for(float* ptr : {L[l].w32.x, L[l].w32.t, L[l].w32.b, L[l].w32.r, L[l].w32.diag})
+            for(float* ptr : {L[l].w32.x, L[l].w32.t, L[l].w32.b, L[l].w32.r, L[l].w32.diag,
+                              L[l].w32.dinv})
                 (void)hipFree(ptr);
         }
         for(void* ptr : {(void*)coarse_inv, (void*)res, (void*)z, (void*)p, (void*)q,
-                         (void*)res32, (void*)z32, (void*)p32, (void*)q32, (void*)d32, (void*)s})
+                         (void*)res32, (void*)z32, (void*)p32, (void*)q32, (void*)d32, (void*)s,
+                         (void*)part})
             (void)hipFree(ptr);
-        if(h_s)
-            (void)hipHostFree(h_s);
+        if(h_note)
+            (void)hipHostFree(h_note);
     }
 
     template <typename T>
@@ -243,7 +254,9 @@ namespace
             LAL_TRY(alloc_n(&L.w64.b, L.A.m));
             LAL_TRY(alloc_n(&L.w64.r, L.A.m));
             LAL_TRY(alloc_n(&L.w64.diag, L.A.m));
-            LAL_TRY(lal::last_diag<double>(c, L.A.m, L.A.rp, L.A.ci, L.A.v, L.w64.diag));
+            LAL_TRY(alloc_n(&L.w64.dinv, L.A.m));
+            LAL_TRY(lal::diag_inv<double>(c, L.A.m, L.A.rp, L.A.ci, L.A.v, L.w64.diag,
+                                          L.w64.dinv));
             if(l < h->levels)
             {
                 LAL_TRY(lal::csrmv_analysis_dev(c, L.P.m, L.P.n, L.P.nnz, L.P.rp, L.P.ci, exact,
@@ -264,7 +277,9 @@ namespace
                 LAL_TRY(alloc_n(&L.w32.b, L.A.m));
                 LAL_TRY(alloc_n(&L.w32.r, L.A.m));
                 LAL_TRY(alloc_n(&L.w32.diag, L.A.m));
-                LAL_TRY(lal::last_diag<float>(c, L.A.m, L.A.rp, L.A.ci, L.A.v32, L.w32.diag));
+                LAL_TRY(alloc_n(&L.w32.dinv, L.A.m));
+                LAL_TRY(lal::diag_inv<float>(c, L.A.m, L.A.rp, L.A.ci, L.A.v32, L.w32.diag,
+                                             L.w32.dinv));
             }
         }
         // coarsest level
@@ -312,8 +327,12 @@ namespace
         LAL_TRY(alloc_n(&h->p, n));
         LAL_TRY(alloc_n(&h->q, n));
         LAL_TRY(alloc_n(&h->s, lal::LAL_SOLVER_SLOTS));
-        LAL_HIP(hipHostMalloc((void**)&h->h_s, sizeof(double) * lal::LAL_SOLVER_SLOTS,
-                              hipHostMallocDefault));
+        LAL_TRY(alloc_n(&h->part, 4 * lal::PCG_BLOCKS));
+        LAL_HIP(hipHostMalloc((void**)&h->h_note, 64, hipHostMallocMapped | hipHostMallocCoherent));
+        std::memset(h->h_note, 0, 64);
+        h->h_seq = reinterpret_cast<unsigned*>(h->h_note + 1);
+        LAL_HIP(hipHostGetDevicePointer((void**)&h->d_note, h->h_note, 0));
+        h->d_seq = reinterpret_cast<unsigned*>(h->d_note + 1);
         if(h->opts.build_f32)
         {
             LAL_TRY(alloc_n(&h->res32, n));
@@ -452,12 +471,14 @@ namespace
         }
         else
         {
-            LAL_TRY(lal::jacobi0_residual<T>(c, A.d, A.m, A.rp, A.ci, Av, W.diag, b, cur, W.r));
+            LAL_TRY(lal::jacobi0_residual<T>(c, A.d, A.m, A.rp, A.ci, Av, W.diag, W.dinv, b, cur,
+                                             W.r));
             if(n1 > 1)
             {
                 for(int k = 1; k < n1; k++)
                 {
-                    LAL_TRY(lal::jacobi_sweep<T>(c, A.d, A.m, A.rp, A.ci, Av, cur, b, other));
+                    LAL_TRY(lal::jacobi_sweep<T>(c, A.d, A.m, A.rp, A.ci, Av, W.diag, W.dinv,
+                                                 cur, b, other));
                     std::swap(cur, other);
                 }
                 if(sizeof(T) == 8)
@@ -482,7 +503,8 @@ namespace
         for(int k = 0; k < n2; k++)
         {
             T* o = (k == n2 - 1 && dst) ? dst : other;
-            LAL_TRY(lal::jacobi_sweep<T>(c, A.d, A.m, A.rp, A.ci, Av, cur, b, o));
+            LAL_TRY(lal::jacobi_sweep<T>(c, A.d, A.m, A.rp, A.ci, Av, W.diag, W.dinv, cur, b,
+                                         o));
             other = cur;
             cur   = o;
         }
@@ -514,13 +536,42 @@ namespace
         return false;
     }
 
-    int read_slot(lal_hierarchy_t* h, int slot, double* v)
+    // Reduces PCG_BLOCKS partials (op 0 sum, 1 max) into s[slot] and hands the
+    // value to the host through the mapped note; returns the sequence number
+    // to wait for.
+    int post_scalar(lal_hierarchy_t* h, int op, const double* part, int slot, unsigned* seq)
     {
-        LAL_HIP(hipMemcpyAsync(h->h_s + slot, h->s + slot, sizeof(double),
-                               hipMemcpyDeviceToHost, h->c->stream));
-        LAL_HIP(hipStreamSynchronize(h->c->stream));
-        *v = h->h_s[slot];
-        return LAL_OK;
+        *seq = ++h->seq;
+        return lal::finish_partials(h->c, op, part, h->s + slot, h->d_note, h->d_seq, *seq);
+    }
+
+    // Waits until the note carries `seq`.  The host polls the mapped word
+    // and checks the stream now and then so that a failed kernel turns into
+    // an error instead of a hang.
+    int wait_scalar(lal_hierarchy_t* h, unsigned seq, int slot, double* v)
+    {
+        for(long spins = 1;; spins++)
+        {
+            if(__atomic_load_n(h->h_seq, __ATOMIC_ACQUIRE) == seq)
+            {
+                *v = *reinterpret_cast<volatile double*>(h->h_note);
+                return LAL_OK;
+            }
+            if((spins & 1023) == 0)
+            {
+                hipError_t e = hipStreamQuery(h->c->stream);
+                if(e == hipSuccess)
+                {
+                    if(__atomic_load_n(h->h_seq, __ATOMIC_ACQUIRE) == seq)
+                        continue;
+                    // the stream has drained: read the device copy instead
+                    LAL_HIP(hipMemcpy(v, h->s + slot, sizeof(double), hipMemcpyDeviceToHost));
+                    return LAL_OK;
+                }
+                if(e != hipErrorNotReady)
+                    LAL_HIP(e);
+            }
+        }
     }
 
     // cg_solver::solve_precond on the device.  T = double uses the FP64
@@ -540,29 +591,43 @@ namespace
         else
             LAL_TRY(lal::residual_f32(c, A.d, n, A.n, A.nnz, A.rp, A.ci, (const float*)Av,
                                       (const float*)x, (const float*)b, (float*)res));
-        double init = 0.0;
-        LAL_TRY(lal::norm_inf_slot<T>(c, n, res, s, lal::S_RNORM0));
-        LAL_TRY(read_slot(h, lal::S_RNORM0, &init));
+        // partial buffers: <q,p>, <z,r> (two generations), max|r|
+        double*  part_pq   = h->part;
+        double*  part_g[2] = {h->part + lal::PCG_BLOCKS, h->part + 2 * lal::PCG_BLOCKS};
+        double*  part_r    = h->part + 3 * lal::PCG_BLOCKS;
+        unsigned seq       = 0;
+        double   init      = 0.0;
+        LAL_TRY(lal::norm_partials<T>(c, n, res, part_r));
+        LAL_TRY(post_scalar(h, 1, part_r, lal::S_RNORM0, &seq));
+        LAL_TRY(wait_scalar(h, seq, lal::S_RNORM0, &init));
         LAL_TRY(precond<T>(h, res, z, n1, n2));
         LAL_HIP(hipMemcpyAsync(p, z, sizeof(T) * n, hipMemcpyDeviceToDevice, c->stream));
-        LAL_TRY(lal::dot_store<T>(c, n, z, res, s, lal::S_GAMMA));
+        int cur = 0;
+        LAL_TRY(lal::dot_partials<T>(c, n, z, res, part_g[cur]));
         double rn   = init;
         int    iter = 0;
         bool   conv = false;
         while(!(iter > max_iter))
         {
             LAL_TRY(lal::csrmv(c, A.d, n, A.n, A.nnz, 1.0, A.rp, A.ci, Av, p, 0.0, q));
-            LAL_TRY(lal::dot_alpha<T>(c, n, q, p, s));
-            LAL_TRY(lal::update_xr<T>(c, n, p, q, x, res, s));
-            LAL_TRY(read_slot(h, lal::S_RNORM, &rn));
+            LAL_TRY(lal::dot_partials<T>(c, n, q, p, part_pq));
+            LAL_TRY(lal::update_xr<T>(c, n, p, q, x, res, part_pq, part_g[cur], part_r,
+                                      s + lal::S_ALPHA));
+            LAL_TRY(post_scalar(h, 1, part_r, lal::S_RNORM, &seq));
+            // The next preconditioner application depends on r only: it is
+            // queued before the host looks at the norm, so the GPU never
+            // waits for the convergence test (its result is simply unused
+            // after the last iteration).
+            LAL_TRY(precond<T>(h, res, z, n1, n2));
+            LAL_TRY(wait_scalar(h, seq, lal::S_RNORM, &rn));
             if(converges(rn, init, rel_tol, abs_tol))
             {
                 conv = true;
                 break;
             }
-            LAL_TRY(precond<T>(h, res, z, n1, n2));
-            LAL_TRY(lal::dot_beta<T>(c, n, z, res, s));
-            LAL_TRY(lal::update_p<T>(c, n, z, p, s));
+            LAL_TRY(lal::dot_partials<T>(c, n, z, res, part_g[cur ^ 1]));
+            LAL_TRY(lal::update_p<T>(c, n, z, p, part_g[cur ^ 1], part_g[cur]));
+            cur ^= 1;
             iter++;
         }
         if(info)
@@ -703,9 +768,12 @@ extern "C" int lal_ir_solve(lal_handle c, lal_hierarchy h, const double* b, doub
     const int      n = A.m;
     double*        r = h->q; // FP64 outer residual lives in the FP64 q buffer
     LAL_TRY(lal::residual_f64(c, A.d, n, A.n, A.nnz, A.rp, A.ci, A.v, x, b, r));
-    LAL_TRY(lal::norm_inf_slot<double>(c, n, r, h->s, lal::S_RNORM0));
-    double init = 0.0;
-    LAL_TRY(read_slot(h, lal::S_RNORM0, &init));
+    double*  part_r = h->part + 3 * lal::PCG_BLOCKS;
+    unsigned seq    = 0;
+    double   init   = 0.0;
+    LAL_TRY(lal::norm_partials<double>(c, n, r, part_r));
+    LAL_TRY(post_scalar(h, 1, part_r, lal::S_RNORM0, &seq));
+    LAL_TRY(wait_scalar(h, seq, lal::S_RNORM0, &init));
     double rn    = init;
     int    outer = 0, inner_total = 0;
     bool   conv  = false;
@@ -730,8 +798,9 @@ extern "C" int lal_ir_solve(lal_handle c, lal_hierarchy h, const double* b, doub
         // x += |r| d in FP64, then the true FP64 residual
         LAL_TRY(lal::axpy_f32_f64(c, n, rn, h->d32, x));
         LAL_TRY(lal::residual_f64(c, A.d, n, A.n, A.nnz, A.rp, A.ci, A.v, x, b, r));
-        LAL_TRY(lal::norm_inf_slot<double>(c, n, r, h->s, lal::S_RNORM));
-        LAL_TRY(read_slot(h, lal::S_RNORM, &rn));
+        LAL_TRY(lal::norm_partials<double>(c, n, r, part_r));
+        LAL_TRY(post_scalar(h, 1, part_r, lal::S_RNORM, &seq));
+        LAL_TRY(wait_scalar(h, seq, lal::S_RNORM, &rn));
         outer++;
     }
     if(info)

@@ -17,7 +17,7 @@ struct lal_csrmv_info
     int  m        = 0;
     int  nnz      = 0;
     int  nblocks  = 0;
-    int  exact    = 0;
+    int  exact    = 0;       // 1: sequential row sums (bit-identical to the host)
     int* d_blocks = nullptr; // nblocks + 1 row offsets
 };
 

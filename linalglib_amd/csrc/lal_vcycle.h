@@ -1,7 +1,7 @@
 // Internal launchers for the smoothers, coarse solves and PCG updates.
 #pragma once
 
-#include "lal_spmv.h"
+#include "lal_rowvec.h"
 
 namespace lal
 {
@@ -46,14 +46,20 @@ namespace lal
         return csrmv_f32(c, d, m, n, nnz, (float)alpha, rp, ci, v, x, (float)beta, y);
     }
 
+    // Grid of the PCG reduction and update kernels (one partial per block).
+    constexpr int PCG_BLOCKS = 512;
+
     template <typename T>
     int jacobi_sweep(lal_context* c, const lal_csrmv_info* d, int m, const int* rp,
-                     const int* ci, const T* v, const T* xold, const T* b, T* xnew);
+                     const int* ci, const T* v, const T* diag, const T* dinv, const T* xold,
+                     const T* b, T* xnew);
     template <typename T>
     int jacobi0_residual(lal_context* c, const lal_csrmv_info* d, int m, const int* rp,
-                         const int* ci, const T* v, const T* diag, const T* b, T* x, T* r);
+                         const int* ci, const T* v, const T* diag, const T* dinv, const T* b,
+                         T* x, T* r);
+    // d = the last diagonal entry of each row, dinv = 1/d.
     template <typename T>
-    int last_diag(lal_context* c, int m, const int* rp, const int* ci, const T* v, T* d);
+    int diag_inv(lal_context* c, int m, const int* rp, const int* ci, const T* v, T* d, T* dinv);
     template <typename T>
     int coarse_scalar(lal_context* c, const T* v, const T* b, T* x);
     template <typename T>
@@ -61,18 +67,22 @@ namespace lal
                   T* x);
     template <typename T>
     int coarse_dense(lal_context* c, int n, const double* inv, const T* b, T* x);
+    // PCG_BLOCKS partials of <x, y> / max|x| (FP64 accumulation).
     template <typename T>
-    int dot_store(lal_context* c, int n, const T* x, const T* y, double* s, int out);
+    int dot_partials(lal_context* c, int n, const T* x, const T* y, double* part);
     template <typename T>
-    int dot_alpha(lal_context* c, int n, const T* q, const T* p, double* s);
+    int norm_partials(lal_context* c, int n, const T* x, double* part);
+    // op 0: sum, 1: max of the partials into *slot (device) and, when host_val
+    // is given (mapped pinned memory), into *host_val then *host_seq = seq.
+    int finish_partials(lal_context* c, int op, const double* part, double* slot,
+                        double* host_val, unsigned* host_seq, unsigned seq);
     template <typename T>
-    int dot_beta(lal_context* c, int n, const T* z, const T* r, double* s);
+    int update_xr(lal_context* c, int n, const T* p, const T* q, T* x, T* r,
+                  const double* part_pq, const double* part_gamma, double* part_r,
+                  double* s_alpha);
     template <typename T>
-    int update_xr(lal_context* c, int n, const T* p, const T* q, T* x, T* r, double* s);
-    template <typename T>
-    int update_p(lal_context* c, int n, const T* z, T* p, const double* s);
-    template <typename T>
-    int norm_inf_slot(lal_context* c, int n, const T* x, double* s, int out);
+    int update_p(lal_context* c, int n, const T* z, T* p, const double* part_new,
+                 const double* part_old);
     int scale_cast_f64_f32(lal_context* c, int n, const double* x, double scale, float* y);
     int axpy_f32_f64(lal_context* c, int n, double alpha, const float* d, double* x);
 }

@@ -1,18 +1,26 @@
 // V-cycle kernels: Jacobi smoothing (from zero, fused with the residual, and
-// general sweeps), the coarse-level solves, and the fused PCG vector updates
-// with deterministic reductions.
+// general sweeps), the coarse-level solves, and the PCG vector updates.
 //
 // Smoother arithmetic follows iterative_solvers/classic/jacobi.cpp:43-75:
 // sigma = sigma + a_jk*xold_k over k != j in CSR order (two roundings per
 // term), x_j = (b_j - sigma)/a_jj with a_jj the row's last diagonal entry.  A
 // sweep from x = 0 gives sigma = +0 exactly, so it is x_j = b_j/a_jj; the
 // fused kernel recomputes the neighbours' x_k = b_k/a_kk on the fly, which is
-// bit-identical to smoothing first and forming r = b - A x afterwards.
+// bit-identical to smoothing first and forming r = b - A x afterwards.  These
+// row-block kernels are the exact path; the fast path (lal_rowvec.hip) forms
+// the same sweep as x + (b - A x) / a_ii.
+//
+// PCG reductions never use atomics or device-scope fences (on a multi-XCD
+// part a device-scope release writes the L2 back, which costs more than the
+// reduction itself).  Each reduction kernel writes one partial per block;
+// the kernels that need the scalar re-reduce the partials themselves, every
+// block in the same fixed order, so all blocks see bitwise the same value.
 #include "lal_vcycle.h"
 
 namespace
 {
     using lal::SPMV_THREADS;
+    constexpr int RB = lal::PCG_BLOCKS;
 
     // MODE 0: x_new = Jacobi sweep of xold ; MODE 1: x = b/d and r = b - A x
     template <typename T, int MODE>
@@ -103,11 +111,12 @@ namespace
         }
     }
 
-    // a_ii as the Jacobi sweep sees it: the last diagonal entry of the row.
+    // a_ii as the Jacobi sweep sees it (the last diagonal entry of the row)
+    // and its reciprocal for the fast path.
     template <typename T>
     __global__ void __launch_bounds__(256)
-        last_diag_kernel(int m, const int* __restrict__ rp, const int* __restrict__ ci,
-                         const T* __restrict__ val, T* __restrict__ d)
+        diag_kernel(int m, const int* __restrict__ rp, const int* __restrict__ ci,
+                    const T* __restrict__ val, T* __restrict__ d, T* __restrict__ dinv)
     {
         int i = blockIdx.x * 256 + threadIdx.x;
         if(i >= m)
@@ -116,7 +125,8 @@ namespace
         for(int j = rp[i]; j < rp[i + 1]; j++)
             if(ci[j] == i)
                 a = val[j];
-        d[i] = a;
+        d[i]    = a;
+        dinv[i] = (T)1 / a;
     }
 
     // Coarsest level with one unknown: x = b / a (amg.cpp:211-216).
@@ -128,8 +138,7 @@ namespace
             x[0] = b[0] / val[0];
     }
 
-    // 1000 in-place Gauss-Seidel sweeps from x = 0 (amg.cpp:236-264), one
-    // lane, the matrix staged in LDS when it fits.
+    // 1000 in-place Gauss-Seidel sweeps from x = 0 (amg.cpp:236-264), one lane.
     template <typename T>
     __global__ void __launch_bounds__(64)
         coarse_gs_kernel(int n, const int* __restrict__ rp, const int* __restrict__ ci,
@@ -176,145 +185,166 @@ namespace
     }
 
     // ---------------------------------------------------- PCG vector updates
-    // Reduction epilogues run by the last block on the device scalars.
-    enum
-    {
-        EPI_STORE = 0, // s[out] = v
-        EPI_ALPHA = 1, // s[PQ] = v ; s[ALPHA] = s[GAMMA] / v
-        EPI_BETA  = 2  // s[BETA] = v / s[GAMMA] ; s[GAMMA] = v
-    };
-
     template <int OP>
-    __device__ __forceinline__ double red_combine(double a, double b)
+    __device__ __forceinline__ double combine(double a, double b)
     {
         return OP == 0 ? a + b : ((b < a) ? a : b);
     }
 
-    // Block-level finish of a deterministic two-stage reduction: returns true
-    // in thread 0 of the last block, with the total in *total.
+    // Block total of one value per lane (all lanes receive it).
     template <int OP>
-    __device__ bool reduce_finish(double acc, double* __restrict__ part,
-                                  unsigned* __restrict__ ticket, double* total)
+    __device__ double block_total(double v)
     {
         __shared__ double sh[SPMV_THREADS / 64];
-        __shared__ bool   last;
-        const int         tid = threadIdx.x;
-        acc                   = OP == 0 ? lal::wave_sum(acc) : lal::wave_max(acc);
-        if((tid & 63) == 0)
-            sh[tid >> 6] = acc;
-        __syncthreads();
-        if(tid == 0)
-        {
-            double t = sh[0];
-            for(int w = 1; w < SPMV_THREADS / 64; w++)
-                t = red_combine<OP>(t, sh[w]);
-            part[blockIdx.x] = t;
-            __threadfence();
-            unsigned k = atomicAdd(ticket, 1u);
-            last       = (k == gridDim.x - 1);
-        }
-        __syncthreads();
-        if(!last)
-            return false;
-        __threadfence();
-        double v = 0.0;
-        for(int k = tid; k < (int)gridDim.x; k += SPMV_THREADS)
-            v = red_combine<OP>(
-                v, __hip_atomic_load(&part[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
         v = OP == 0 ? lal::wave_sum(v) : lal::wave_max(v);
         __syncthreads();
-        if((tid & 63) == 0)
-            sh[tid >> 6] = v;
+        if((threadIdx.x & 63) == 0)
+            sh[threadIdx.x >> 6] = v;
         __syncthreads();
-        if(tid != 0)
-            return false;
         double t = sh[0];
+#pragma unroll
         for(int w = 1; w < SPMV_THREADS / 64; w++)
-            t = red_combine<OP>(t, sh[w]);
-        *total  = t;
-        *ticket = 0u;
-        return true;
+            t = combine<OP>(t, sh[w]);
+        return t;
     }
 
-    // dot(x, y) accumulated in FP64 with an epilogue on the solver scalars.
-    template <typename T, int EPI>
-    __global__ void __launch_bounds__(SPMV_THREADS)
-        dot_epi_kernel(int n, const T* __restrict__ x, const T* __restrict__ y,
-                       double* __restrict__ part, unsigned* __restrict__ ticket,
-                       double* __restrict__ s, int out)
+    // Sum of the RB partials of a previous reduction, identical in every block.
+    __device__ double sum_partials(const double* __restrict__ part)
     {
-        double acc = 0.0;
-        for(int i = blockIdx.x * SPMV_THREADS + threadIdx.x; i < n;
-            i += gridDim.x * SPMV_THREADS)
-            acc = acc + (double)x[i] * (double)y[i];
-        double v;
-        if(!reduce_finish<0>(acc, part, ticket, &v))
-            return;
-        if(EPI == EPI_STORE)
-            s[out] = v;
-        else if(EPI == EPI_ALPHA)
+        double v = 0.0;
+#pragma unroll
+        for(int k = threadIdx.x; k < RB; k += SPMV_THREADS)
+            v += part[k];
+        return block_total<0>(v);
+    }
+
+    // part[block] = partial of sum x_i*y_i (OP 0) or max |x_i| (OP 1).
+    template <typename T, int OP>
+    __global__ void __launch_bounds__(SPMV_THREADS)
+        partial_kernel(int n, const T* __restrict__ x, const T* __restrict__ y,
+                       double* __restrict__ part)
+    {
+        double    acc    = 0.0;
+        const int stride = gridDim.x * SPMV_THREADS;
+        int       i      = blockIdx.x * SPMV_THREADS + threadIdx.x;
+        for(; i + 3 * stride < n; i += 4 * stride)
         {
-            s[lal::S_PQ]    = v;
-            s[lal::S_ALPHA] = s[lal::S_GAMMA] / v;
+            T a[4], b[4];
+#pragma unroll
+            for(int k = 0; k < 4; k++)
+            {
+                a[k] = x[i + k * stride];
+                b[k] = OP == 0 ? y[i + k * stride] : a[k];
+            }
+#pragma unroll
+            for(int k = 0; k < 4; k++)
+                acc = OP == 0 ? acc + (double)a[k] * (double)b[k]
+                              : combine<1>(acc, fabs((double)a[k]));
         }
-        else
+        for(; i < n; i += stride)
+            acc = OP == 0 ? acc + (double)x[i] * (double)y[i] : combine<1>(acc, fabs((double)x[i]));
+        const double t = block_total<OP>(acc);
+        if(threadIdx.x == 0)
+            part[blockIdx.x] = t;
+    }
+
+    // One block: total of RB partials into a device slot and, when given, a
+    // mapped host word (followed by a sequence number the host polls).
+    template <int OP>
+    __global__ void __launch_bounds__(SPMV_THREADS)
+        finish_kernel(const double* __restrict__ part, double* __restrict__ slot,
+                      double* host_val, unsigned* host_seq, unsigned seq)
+    {
+        double v = OP == 0 ? 0.0 : 0.0;
+        for(int k = threadIdx.x; k < RB; k += SPMV_THREADS)
+            v = combine<OP>(v, part[k]);
+        const double t = block_total<OP>(v);
+        if(threadIdx.x == 0)
         {
-            s[lal::S_BETA]  = v / s[lal::S_GAMMA];
-            s[lal::S_GAMMA] = v;
+            *slot = t;
+            if(host_val)
+            {
+                __hip_atomic_store(host_val, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                __hip_atomic_store(host_seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
         }
     }
 
-    // x += alpha p ; r += (-1*alpha) q ; s[RNORM] = max |r_i|   (cg.cpp:189-195)
+    // alpha = gamma / <q, p>; x += alpha p; r += (-1*alpha) q (cg.cpp:185-193);
+    // part_r[block] = max |r_i|.
     template <typename T>
     __global__ void __launch_bounds__(SPMV_THREADS)
         update_xr_kernel(int n, const T* __restrict__ p, const T* __restrict__ q,
-                         T* __restrict__ x, T* __restrict__ r, double* __restrict__ part,
-                         unsigned* __restrict__ ticket, double* __restrict__ s)
+                         T* __restrict__ x, T* __restrict__ r,
+                         const double* __restrict__ part_pq,
+                         const double* __restrict__ part_gamma, double* __restrict__ part_r,
+                         double* __restrict__ s_alpha)
     {
-        const T alpha  = (T)s[lal::S_ALPHA];
-        const T malpha = (T)(-1.0 * s[lal::S_ALPHA]);
-        double  acc    = 0.0;
-        for(int i = blockIdx.x * SPMV_THREADS + threadIdx.x; i < n;
-            i += gridDim.x * SPMV_THREADS)
+        const int stride = gridDim.x * SPMV_THREADS;
+        int       i      = blockIdx.x * SPMV_THREADS + threadIdx.x;
+        const double pq    = sum_partials(part_pq);
+        const double gamma = sum_partials(part_gamma);
+        const double alpha = gamma / pq;
+        if(blockIdx.x == 0 && threadIdx.x == 0)
+            *s_alpha = alpha;
+        const T a  = (T)alpha;
+        const T ma = (T)(-1.0 * alpha);
+        double  acc = 0.0;
+        auto    upd = [&](T pi, T qi, T xi, T ri, int at) {
+            x[at]    = a == (T)1 ? pi + xi : a * pi + xi;
+            T rn     = ma == (T)1 ? qi + ri : ma * qi + ri;
+            r[at]    = rn;
+            acc      = combine<1>(acc, fabs((double)rn));
+        };
+        for(; i + 3 * stride < n; i += 4 * stride)
         {
-            x[i]     = alpha == (T)1 ? p[i] + x[i] : alpha * p[i] + x[i];
-            T ri     = malpha == (T)1 ? q[i] + r[i] : malpha * q[i] + r[i];
-            r[i]     = ri;
-            double a = fabs((double)ri);
-            acc      = (a < acc) ? acc : a;
+            T pp[4], qq[4], xx[4], rr[4];
+#pragma unroll
+            for(int k = 0; k < 4; k++)
+            {
+                pp[k] = p[i + k * stride];
+                qq[k] = q[i + k * stride];
+                xx[k] = x[i + k * stride];
+                rr[k] = r[i + k * stride];
+            }
+#pragma unroll
+            for(int k = 0; k < 4; k++)
+                upd(pp[k], qq[k], xx[k], rr[k], i + k * stride);
         }
-        double v;
-        if(reduce_finish<1>(acc, part, ticket, &v))
-            s[lal::S_RNORM] = v;
+        for(; i < n; i += stride)
+            upd(p[i], q[i], x[i], r[i], i);
+        const double t = block_total<1>(acc);
+        if(threadIdx.x == 0)
+            part_r[blockIdx.x] = t;
     }
 
-    // p = 1*z + beta*p  (axpby, cg.cpp:210-211)
-    template <typename T>
-    __global__ void __launch_bounds__(256)
-        update_p_kernel(int n, const T* __restrict__ z, T* __restrict__ p,
-                        const double* __restrict__ s)
-    {
-        const T beta = (T)s[lal::S_BETA];
-        for(int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
-            p[i] = beta == (T)0 ? z[i] : z[i] + beta * p[i];
-    }
-
-    // s[out] = max |x_i| (FP64 result for either precision)
+    // beta = gamma_new / gamma_old; p = 1*z + beta*p (cg.cpp:205-211).
     template <typename T>
     __global__ void __launch_bounds__(SPMV_THREADS)
-        norm_inf_kernel(int n, const T* __restrict__ x, double* __restrict__ part,
-                        unsigned* __restrict__ ticket, double* __restrict__ s, int out)
+        update_p_kernel(int n, const T* __restrict__ z, T* __restrict__ p,
+                        const double* __restrict__ part_new,
+                        const double* __restrict__ part_old)
     {
-        double acc = 0.0;
-        for(int i = blockIdx.x * SPMV_THREADS + threadIdx.x; i < n;
-            i += gridDim.x * SPMV_THREADS)
+        const double g_new = sum_partials(part_new);
+        const double g_old = sum_partials(part_old);
+        const T      beta  = (T)(g_new / g_old);
+        const int    stride = gridDim.x * SPMV_THREADS;
+        int          i      = blockIdx.x * SPMV_THREADS + threadIdx.x;
+        for(; i + 3 * stride < n; i += 4 * stride)
         {
-            double a = fabs((double)x[i]);
-            acc      = (a < acc) ? acc : a;
+            T zz[4], pp[4];
+#pragma unroll
+            for(int k = 0; k < 4; k++)
+            {
+                zz[k] = z[i + k * stride];
+                pp[k] = p[i + k * stride];
+            }
+#pragma unroll
+            for(int k = 0; k < 4; k++)
+                p[i + k * stride] = beta == (T)0 ? zz[k] : zz[k] + beta * pp[k];
         }
-        double v;
-        if(reduce_finish<1>(acc, part, ticket, &v))
-            s[out] = v;
+        for(; i < n; i += stride)
+            p[i] = beta == (T)0 ? z[i] : z[i] + beta * p[i];
     }
 
     // y32 = (float)(scale * x64): the refinement residual, normalised
@@ -353,24 +383,38 @@ namespace lal
 {
     template <typename T>
     int jacobi_sweep(lal_context* c, const lal_csrmv_info* d, int m, const int* rp,
-                     const int* ci, const T* v, const T* xold, const T* b, T* xnew)
+                     const int* ci, const T* v, const T* diag, const T* dinv, const T* xold,
+                     const T* b, T* xnew)
     {
         if(m == 0)
             return LAL_OK;
+        if(d && !d->exact)
+        {
+            rv_args a{m, d->d_blocks, d->nblocks, rp, ci, v, xold, dinv, b, xnew, nullptr, 0.0,
+                      0.0};
+            return rowvec_launch<T>(c, RV_JACOBI, a);
+        }
         int nb = spmv_grid(d, m);
         jacobi_kernel<T, 0><<<nb, SPMV_THREADS, 0, c->stream>>>(
             d ? d->d_blocks : nullptr, nb, m, rp, ci, v, xold, nullptr, b, xnew, nullptr,
             d ? d->exact : 0);
         LAL_LAUNCHED();
+        (void)diag;
         return LAL_OK;
     }
 
     template <typename T>
     int jacobi0_residual(lal_context* c, const lal_csrmv_info* d, int m, const int* rp,
-                         const int* ci, const T* v, const T* diag, const T* b, T* x, T* r)
+                         const int* ci, const T* v, const T* diag, const T* dinv, const T* b,
+                         T* x, T* r)
     {
         if(m == 0)
             return LAL_OK;
+        if(d && !d->exact)
+        {
+            rv_args a{m, d->d_blocks, d->nblocks, rp, ci, v, nullptr, dinv, b, x, r, 0.0, 0.0};
+            return rowvec_launch<T>(c, RV_JACOBI0, a);
+        }
         int nb = spmv_grid(d, m);
         jacobi_kernel<T, 1><<<nb, SPMV_THREADS, 0, c->stream>>>(
             d ? d->d_blocks : nullptr, nb, m, rp, ci, v, nullptr, diag, b, x, r,
@@ -380,11 +424,11 @@ namespace lal
     }
 
     template <typename T>
-    int last_diag(lal_context* c, int m, const int* rp, const int* ci, const T* v, T* d)
+    int diag_inv(lal_context* c, int m, const int* rp, const int* ci, const T* v, T* d, T* dinv)
     {
         if(m == 0)
             return LAL_OK;
-        last_diag_kernel<T><<<grid_for(m, 256), 256, 0, c->stream>>>(m, rp, ci, v, d);
+        diag_kernel<T><<<grid_for(m, 256), 256, 0, c->stream>>>(m, rp, ci, v, d, dinv);
         LAL_LAUNCHED();
         return LAL_OK;
     }
@@ -414,60 +458,51 @@ namespace lal
         return LAL_OK;
     }
 
-    static int red_grid(int n)
-    {
-        return n <= 0 ? 1 : grid_for(n, SPMV_THREADS, LAL_RED_BLOCKS);
-    }
-
     template <typename T>
-    int dot_store(lal_context* c, int n, const T* x, const T* y, double* s, int out)
+    int dot_partials(lal_context* c, int n, const T* x, const T* y, double* part)
     {
-        dot_epi_kernel<T, EPI_STORE><<<red_grid(n), SPMV_THREADS, 0, c->stream>>>(
-            n, x, y, c->d_part, c->d_ticket, s, out);
+        partial_kernel<T, 0><<<RB, SPMV_THREADS, 0, c->stream>>>(n, x, y, part);
         LAL_LAUNCHED();
         return LAL_OK;
     }
 
     template <typename T>
-    int dot_alpha(lal_context* c, int n, const T* q, const T* p, double* s)
+    int norm_partials(lal_context* c, int n, const T* x, double* part)
     {
-        dot_epi_kernel<T, EPI_ALPHA><<<red_grid(n), SPMV_THREADS, 0, c->stream>>>(
-            n, q, p, c->d_part, c->d_ticket, s, 0);
+        partial_kernel<T, 1><<<RB, SPMV_THREADS, 0, c->stream>>>(n, x, x, part);
+        LAL_LAUNCHED();
+        return LAL_OK;
+    }
+
+    int finish_partials(lal_context* c, int op, const double* part, double* slot,
+                        double* host_val, unsigned* host_seq, unsigned seq)
+    {
+        if(op == 0)
+            finish_kernel<0><<<1, SPMV_THREADS, 0, c->stream>>>(part, slot, host_val, host_seq,
+                                                                seq);
+        else
+            finish_kernel<1><<<1, SPMV_THREADS, 0, c->stream>>>(part, slot, host_val, host_seq,
+                                                                seq);
         LAL_LAUNCHED();
         return LAL_OK;
     }
 
     template <typename T>
-    int dot_beta(lal_context* c, int n, const T* z, const T* r, double* s)
+    int update_xr(lal_context* c, int n, const T* p, const T* q, T* x, T* r,
+                  const double* part_pq, const double* part_gamma, double* part_r,
+                  double* s_alpha)
     {
-        dot_epi_kernel<T, EPI_BETA><<<red_grid(n), SPMV_THREADS, 0, c->stream>>>(
-            n, z, r, c->d_part, c->d_ticket, s, 0);
+        update_xr_kernel<T><<<RB, SPMV_THREADS, 0, c->stream>>>(n, p, q, x, r, part_pq,
+                                                               part_gamma, part_r, s_alpha);
         LAL_LAUNCHED();
         return LAL_OK;
     }
 
     template <typename T>
-    int update_xr(lal_context* c, int n, const T* p, const T* q, T* x, T* r, double* s)
+    int update_p(lal_context* c, int n, const T* z, T* p, const double* part_new,
+                 const double* part_old)
     {
-        update_xr_kernel<T><<<red_grid(n), SPMV_THREADS, 0, c->stream>>>(
-            n, p, q, x, r, c->d_part, c->d_ticket, s);
-        LAL_LAUNCHED();
-        return LAL_OK;
-    }
-
-    template <typename T>
-    int update_p(lal_context* c, int n, const T* z, T* p, const double* s)
-    {
-        update_p_kernel<T><<<grid_for(n, 256, 16384), 256, 0, c->stream>>>(n, z, p, s);
-        LAL_LAUNCHED();
-        return LAL_OK;
-    }
-
-    template <typename T>
-    int norm_inf_slot(lal_context* c, int n, const T* x, double* s, int out)
-    {
-        norm_inf_kernel<T><<<red_grid(n), SPMV_THREADS, 0, c->stream>>>(n, x, c->d_part,
-                                                                       c->d_ticket, s, out);
+        update_p_kernel<T><<<RB, SPMV_THREADS, 0, c->stream>>>(n, z, p, part_new, part_old);
         LAL_LAUNCHED();
         return LAL_OK;
     }
@@ -496,22 +531,23 @@ namespace lal
         return LAL_OK;
     }
 
-#define LAL_INST(T)                                                                         \
-    template int jacobi_sweep<T>(lal_context*, const lal_csrmv_info*, int, const int*,      \
-                                 const int*, const T*, const T*, const T*, T*);             \
-    template int jacobi0_residual<T>(lal_context*, const lal_csrmv_info*, int, const int*,  \
-                                     const int*, const T*, const T*, const T*, T*, T*);     \
-    template int last_diag<T>(lal_context*, int, const int*, const int*, const T*, T*);     \
-    template int coarse_scalar<T>(lal_context*, const T*, const T*, T*);                    \
+#define LAL_INST(T)                                                                          \
+    template int jacobi_sweep<T>(lal_context*, const lal_csrmv_info*, int, const int*,       \
+                                 const int*, const T*, const T*, const T*, const T*,         \
+                                 const T*, T*);                                              \
+    template int jacobi0_residual<T>(lal_context*, const lal_csrmv_info*, int, const int*,   \
+                                     const int*, const T*, const T*, const T*, const T*, T*, \
+                                     T*);                                                    \
+    template int diag_inv<T>(lal_context*, int, const int*, const int*, const T*, T*, T*);   \
+    template int coarse_scalar<T>(lal_context*, const T*, const T*, T*);                     \
     template int coarse_gs<T>(lal_context*, int, const int*, const int*, const T*, const T*, \
-                              T*);                                                          \
-    template int coarse_dense<T>(lal_context*, int, const double*, const T*, T*);           \
-    template int dot_store<T>(lal_context*, int, const T*, const T*, double*, int);         \
-    template int dot_alpha<T>(lal_context*, int, const T*, const T*, double*);              \
-    template int dot_beta<T>(lal_context*, int, const T*, const T*, double*);               \
-    template int update_xr<T>(lal_context*, int, const T*, const T*, T*, T*, double*);      \
-    template int update_p<T>(lal_context*, int, const T*, T*, const double*);              \
-    template int norm_inf_slot<T>(lal_context*, int, const T*, double*, int);
+                              T*);                                                           \
+    template int coarse_dense<T>(lal_context*, int, const double*, const T*, T*);            \
+    template int dot_partials<T>(lal_context*, int, const T*, const T*, double*);            \
+    template int norm_partials<T>(lal_context*, int, const T*, double*);                     \
+    template int update_xr<T>(lal_context*, int, const T*, const T*, T*, T*, const double*,  \
+                              const double*, double*, double*);                              \
+    template int update_p<T>(lal_context*, int, const T*, T*, const double*, const double*);
     LAL_INST(double)
     LAL_INST(float)
 #undef LAL_INST

@@ -61,10 +61,13 @@ def test_csrmv_f64(h, oracle, exact):
             if exact:
                 assert bits_equal(got, ref), (name, alpha, beta)
             else:
-                # split rows: within 1e-14 of the row's absolute sum
+                # rounded products summed in another order: the recursive
+                # summation bound (len + 2) u sum|a_ij x_j| (u = 2^-53)
                 absr = oracle.csrmv(CSR(A.row_ptr, A.col_ind, np.abs(A.val), A.m, A.n),
                                     np.abs(x)) * abs(alpha) + abs(beta * y0)
-                assert np.all(np.abs(got - ref) <= 1e-14 * absr + 1e-300), (name, alpha, beta)
+                lens = np.diff(A.row_ptr)
+                tol = (lens + 3) * 1.12e-16 * absr + 1e-300
+                assert np.all(np.abs(got - ref) <= tol), (name, alpha, beta)
         # without a descriptor: 256 rows per block
         y = empty(A.m, d.dtype)
         lal.call("lal_csrmv_f64", h.h, None, A.m, A.n, A.nnz, 1.0, lal.ptr(d.row_ptr),
