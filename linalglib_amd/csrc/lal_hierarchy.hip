@@ -12,6 +12,7 @@
 // device memory and are formed by the last block of each reduction, so a PCG
 // iteration hands one 8-byte residual norm to the host for the convergence
 // test and nothing else.
+#include "lal_coarse.h"
 #include "lal_vcycle.h"
 
 #include <algorithm>
@@ -128,6 +129,10 @@ struct lal_hierarchy_t
     unsigned* d_seq      = nullptr;
     unsigned  seq        = 0;
     bool      has32      = false;
+    // coarse tail: levels tail_first..levels run in one workgroup (fast path)
+    int                          tail_first = -1;
+    lal::coarse_level<double>*   d_tail64   = nullptr;
+    lal::coarse_level<float>*    d_tail32   = nullptr;
 
     ~lal_hierarchy_t()
     {
@@ -145,7 +150,7 @@ struct lal_hierarchy_t
         }
         for(void* ptr : {(void*)coarse_inv, (void*)res, (void*)z, (void*)p, (void*)q,
                          (void*)res32, (void*)z32, (void*)p32, (void*)q32, (void*)d32, (void*)s,
-                         (void*)part})
+                         (void*)part, (void*)d_tail64, (void*)d_tail32})
             (void)hipFree(ptr);
         if(h_note)
             (void)hipHostFree(h_note);
@@ -179,6 +184,30 @@ namespace
     const float* vals<float>(const csr_dev& A)
     {
         return A.v32;
+    }
+
+    template <typename T>
+    const lal::coarse_level<T>* tail_levels(const lal_hierarchy_t* h);
+    template <>
+    const lal::coarse_level<double>* tail_levels<double>(const lal_hierarchy_t* h)
+    {
+        return h->d_tail64;
+    }
+    template <>
+    const lal::coarse_level<float>* tail_levels<float>(const lal_hierarchy_t* h)
+    {
+        return h->d_tail32;
+    }
+
+    // r = b - A x in the level's precision
+    template <typename T>
+    int residual_t(lal_context* c, const csr_dev& A, const T* Av, const T* x, const T* b, T* r)
+    {
+        if(sizeof(T) == 8)
+            return lal::residual_f64(c, A.d, A.m, A.n, A.nnz, A.rp, A.ci, (const double*)Av,
+                                     (const double*)x, (const double*)b, (double*)r);
+        return lal::residual_f32(c, A.d, A.m, A.n, A.nnz, A.rp, A.ci, (const float*)Av,
+                                 (const float*)x, (const float*)b, (float*)r);
     }
 
     // Dense inverse of the coarsest operator through its Cholesky factor
@@ -230,6 +259,43 @@ namespace
                 inv[(size_t)i * n + e] = col[i];
         }
         return true;
+    }
+
+    // Device array describing levels tail_first..levels for the coarse tail.
+    template <typename T>
+    int build_tail_levels(lal_hierarchy_t* h, lal::coarse_level<T>** out)
+    {
+        std::vector<lal::coarse_level<T>> v;
+        for(int l = h->tail_first; l <= h->levels; l++)
+        {
+            level_t&             L = h->L[l];
+            work<T>&             W = h->w<T>(l);
+            lal::coarse_level<T> d{};
+            d.m    = L.A.m;
+            d.rp   = L.A.rp;
+            d.ci   = L.A.ci;
+            d.v    = vals<T>(L.A);
+            d.dinv = W.dinv;
+            if(l < h->levels)
+            {
+                d.prp = L.P.rp;
+                d.pci = L.P.ci;
+                d.pv  = vals<T>(L.P);
+                d.rrp = L.R.rp;
+                d.rci = L.R.ci;
+                d.rv  = vals<T>(L.R);
+            }
+            d.x = W.x;
+            d.t = W.t;
+            d.b = W.b;
+            d.r = W.r;
+            v.push_back(d);
+        }
+        LAL_TRY(alloc_n(out, v.size()));
+        LAL_HIP(hipMemcpyAsync(*out, v.data(), sizeof(lal::coarse_level<T>) * v.size(),
+                               hipMemcpyHostToDevice, h->c->stream));
+        LAL_HIP(hipStreamSynchronize(h->c->stream));
+        return LAL_OK;
     }
 
     int to_f32(lal_context* c, const double* x, float** y, size_t n)
@@ -320,6 +386,23 @@ namespace
         }
         else
             h->coarse_kind = 1;
+
+        // coarse tail: the deepest run of levels with few nonzeros (at least
+        // two levels, never level 0), on the fast path with a direct coarsest
+        // solve
+        if(!exact && h->coarse_kind != 1)
+        {
+            int first = h->levels;
+            while(first > 1 && h->L[first - 1].A.nnz <= lal::COARSE_TAIL_NNZ)
+                first--;
+            if(first < h->levels && h->L[first].A.nnz <= lal::COARSE_TAIL_NNZ)
+            {
+                h->tail_first = first;
+                LAL_TRY(build_tail_levels<double>(h, &h->d_tail64));
+                if(h->opts.build_f32)
+                    LAL_TRY(build_tail_levels<float>(h, &h->d_tail32));
+            }
+        }
 
         const int n = h->L[0].A.m;
         LAL_TRY(alloc_n(&h->res, n));
@@ -442,13 +525,27 @@ namespace
     // One V-cycle at level l for rhs b with the initial guess zero.  The
     // result lands in dst when given, else in a level buffer; *out gets it.
     template <typename T>
-    int vcycle(lal_hierarchy_t* h, int l, const T* b, T* dst, int n1, int n2, T** out)
+    int vcycle(lal_hierarchy_t* h, int l, const T* b, T* dst, int n1, int n2, bool xinit,
+               T** out)
     {
         lal_context* c = h->c;
         level_t&     L = h->L[l];
         work<T>&     W = h->w<T>(l);
         const csr_dev& A = L.A;
         const T*       Av = vals<T>(A);
+        // Fast path with one pre-sweep: the sweep from zero is x = b/a_ii,
+        // written by whoever produced b (the restriction above, or the PCG
+        // update for level 0) when xinit is set; this level then only forms
+        // r = b - A x.
+        const bool fast = !h->opts.exact && n1 == 1;
+        const lal::coarse_level<T>* tail = tail_levels<T>(h);
+        if(fast && n2 == 1 && l == h->tail_first && tail && b == W.b)
+        {
+            LAL_TRY(lal::coarse_tail<T>(c, tail, h->levels - l + 1, h->coarse_kind == 0 ? 0 : 1,
+                                        h->coarse_inv, dst));
+            *out = dst ? dst : W.t;
+            return LAL_OK;
+        }
         if(l == h->levels)
         {
             T* x = dst ? dst : W.x;
@@ -469,6 +566,12 @@ namespace
             LAL_HIP(hipMemsetAsync(cur, 0, sizeof(T) * A.m, c->stream));
             LAL_HIP(hipMemcpyAsync(W.r, b, sizeof(T) * A.m, hipMemcpyDeviceToDevice, c->stream));
         }
+        else if(fast)
+        {
+            if(!xinit)
+                LAL_TRY(lal::mul_elem<T>(c, A.m, b, W.dinv, cur));
+            LAL_TRY(residual_t<T>(c, A, Av, cur, b, W.r));
+        }
         else
         {
             LAL_TRY(lal::jacobi0_residual<T>(c, A.d, A.m, A.rp, A.ci, Av, W.diag, W.dinv, b, cur,
@@ -481,22 +584,20 @@ namespace
                                                  cur, b, other));
                     std::swap(cur, other);
                 }
-                if(sizeof(T) == 8)
-                    LAL_TRY(lal::residual_f64(c, A.d, A.m, A.n, A.nnz, A.rp, A.ci,
-                                              (const double*)Av, (const double*)cur,
-                                              (const double*)b, (double*)W.r));
-                else
-                    LAL_TRY(lal::residual_f32(c, A.d, A.m, A.n, A.nnz, A.rp, A.ci,
-                                              (const float*)Av, (const float*)cur,
-                                              (const float*)b, (float*)W.r));
+                LAL_TRY(residual_t<T>(c, A, Av, cur, b, W.r));
             }
         }
-        // restriction: b_{l+1} = R r
+        // restriction: b_{l+1} = R r (and, on the fast path, x_{l+1} = b_{l+1}/a_ii)
         const csr_dev& R  = L.R;
         work<T>&       Wc = h->w<T>(l + 1);
-        LAL_TRY(lal::csrmv(c, R.d, R.m, R.n, R.nnz, 1.0, R.rp, R.ci, vals<T>(R), W.r, 0.0, Wc.b));
+        if(fast)
+            LAL_TRY(lal::restrict_init<T>(c, R.d, R.m, R.rp, R.ci, vals<T>(R), W.r, Wc.dinv,
+                                          Wc.b, Wc.x));
+        else
+            LAL_TRY(lal::csrmv(c, R.d, R.m, R.n, R.nnz, 1.0, R.rp, R.ci, vals<T>(R), W.r, 0.0,
+                               Wc.b));
         T* ec = nullptr;
-        LAL_TRY(vcycle<T>(h, l + 1, Wc.b, nullptr, n1, n2, &ec));
+        LAL_TRY(vcycle<T>(h, l + 1, Wc.b, nullptr, n1, n2, fast, &ec));
         // prolongation and correction: x += P ec
         const csr_dev& P = L.P;
         LAL_TRY(lal::csrmv(c, P.d, P.m, P.n, P.nnz, 1.0, P.rp, P.ci, vals<T>(P), ec, 1.0, cur));
@@ -519,10 +620,10 @@ namespace
     }
 
     template <typename T>
-    int precond(lal_hierarchy_t* h, const T* r, T* z, int n1, int n2)
+    int precond(lal_hierarchy_t* h, const T* r, T* z, int n1, int n2, bool xinit = false)
     {
         T* out = nullptr;
-        return vcycle<T>(h, 0, r, z, n1, n2, &out);
+        return vcycle<T>(h, 0, r, z, n1, n2, xinit, &out);
     }
 
     bool converges(double res, double init, double rel_tol, double abs_tol)
@@ -611,14 +712,18 @@ namespace
         {
             LAL_TRY(lal::csrmv(c, A.d, n, A.n, A.nnz, 1.0, A.rp, A.ci, Av, p, 0.0, q));
             LAL_TRY(lal::dot_partials<T>(c, n, q, p, part_pq));
+            // with one pre-sweep on the fast path, the update also writes the
+            // V-cycle's first Jacobi iterate x0 = r / a_ii
+            const bool fast = !h->opts.exact && n1 == 1 && h->levels > 0;
+            work<T>&   W0   = h->w<T>(0);
             LAL_TRY(lal::update_xr<T>(c, n, p, q, x, res, part_pq, part_g[cur], part_r,
-                                      s + lal::S_ALPHA));
+                                      s + lal::S_ALPHA, W0.dinv, fast ? W0.x : nullptr));
             LAL_TRY(post_scalar(h, 1, part_r, lal::S_RNORM, &seq));
             // The next preconditioner application depends on r only: it is
             // queued before the host looks at the norm, so the GPU never
             // waits for the convergence test (its result is simply unused
             // after the last iteration).
-            LAL_TRY(precond<T>(h, res, z, n1, n2));
+            LAL_TRY(precond<T>(h, res, z, n1, n2, fast));
             LAL_TRY(wait_scalar(h, seq, lal::S_RNORM, &rn));
             if(converges(rn, init, rel_tol, abs_tol))
             {

@@ -17,6 +17,8 @@
 //   JACOBI    y = x + (b - A x) * dinv     (= (b - sum_{k!=i} a_ik x_k)/a_ii)
 //   JACOBI0   y = b * dinv and r = b - A (b * dinv)  (one sweep from x = 0,
 //                                           fused with the residual)
+//   RESTRICT  y = A x and r = y * dinv     (restriction that also writes the
+//                                           coarse level's first sweep)
 // dinv holds 1/a_ii, so no division is left in the inner loops.
 #pragma once
 
@@ -29,7 +31,9 @@ namespace lal
         RV_SPMV     = 0,
         RV_RESIDUAL = 1,
         RV_JACOBI   = 2,
-        RV_JACOBI0  = 3
+        RV_JACOBI0  = 3,
+        RV_RESTRICT = 4 // y = A x and r = y * dinv: the coarse right-hand side
+                        // and the first Jacobi iterate from zero on it
     };
 
     struct rv_args

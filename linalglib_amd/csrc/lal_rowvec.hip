@@ -88,6 +88,11 @@ namespace
             return;
         if(MODE == lal::RV_SPMV)
             y[row] = beta == (T)0 ? alpha * acc : fma(alpha, acc, beta * y[row]);
+        else if(MODE == lal::RV_RESTRICT)
+        {
+            y[row] = acc;
+            r[row] = acc * d[row];
+        }
         else if(MODE == lal::RV_RESIDUAL)
             y[row] = b[row] - acc;
         else if(MODE == lal::RV_JACOBI)
@@ -122,6 +127,7 @@ namespace lal
         switch(mode)
         {
         case RV_SPMV: return launch<T, RV_SPMV>(c, a);
+        case RV_RESTRICT: return launch<T, RV_RESTRICT>(c, a);
         case RV_RESIDUAL: return launch<T, RV_RESIDUAL>(c, a);
         case RV_JACOBI: return launch<T, RV_JACOBI>(c, a);
         default: return launch<T, RV_JACOBI0>(c, a);

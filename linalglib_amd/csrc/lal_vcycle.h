@@ -76,10 +76,18 @@ namespace lal
     // is given (mapped pinned memory), into *host_val then *host_seq = seq.
     int finish_partials(lal_context* c, int op, const double* part, double* slot,
                         double* host_val, unsigned* host_seq, unsigned seq);
+    // x += alpha p, r -= alpha q, partial max|r|; with x0 also x0 = r * dinv.
     template <typename T>
     int update_xr(lal_context* c, int n, const T* p, const T* q, T* x, T* r,
                   const double* part_pq, const double* part_gamma, double* part_r,
-                  double* s_alpha);
+                  double* s_alpha, const T* dinv, T* x0);
+    // x = b * dinv
+    template <typename T>
+    int mul_elem(lal_context* c, int n, const T* b, const T* dinv, T* x);
+    // b_c = R r and x_c = b_c * dinv_c (fast path only: needs a descriptor)
+    template <typename T>
+    int restrict_init(lal_context* c, const lal_csrmv_info* d, int m, const int* rp,
+                      const int* ci, const T* v, const T* r, const T* dinv_c, T* b_c, T* x_c);
     template <typename T>
     int update_p(lal_context* c, int n, const T* z, T* p, const double* part_new,
                  const double* part_old);

@@ -272,13 +272,16 @@ namespace
 
     // alpha = gamma / <q, p>; x += alpha p; r += (-1*alpha) q (cg.cpp:185-193);
     // part_r[block] = max |r_i|.
+    // When x0 is given, also x0 = r * dinv: the first Jacobi iterate from zero
+    // of the preconditioner applied next to r.
     template <typename T>
     __global__ void __launch_bounds__(SPMV_THREADS)
         update_xr_kernel(int n, const T* __restrict__ p, const T* __restrict__ q,
                          T* __restrict__ x, T* __restrict__ r,
                          const double* __restrict__ part_pq,
                          const double* __restrict__ part_gamma, double* __restrict__ part_r,
-                         double* __restrict__ s_alpha)
+                         double* __restrict__ s_alpha, const T* __restrict__ dinv,
+                         T* __restrict__ x0)
     {
         const int stride = gridDim.x * SPMV_THREADS;
         int       i      = blockIdx.x * SPMV_THREADS + threadIdx.x;
@@ -290,15 +293,17 @@ namespace
         const T a  = (T)alpha;
         const T ma = (T)(-1.0 * alpha);
         double  acc = 0.0;
-        auto    upd = [&](T pi, T qi, T xi, T ri, int at) {
+        auto    upd = [&](T pi, T qi, T xi, T ri, T di, int at) {
             x[at]    = a == (T)1 ? pi + xi : a * pi + xi;
             T rn     = ma == (T)1 ? qi + ri : ma * qi + ri;
             r[at]    = rn;
-            acc      = combine<1>(acc, fabs((double)rn));
+            if(x0)
+                x0[at] = rn * di;
+            acc = combine<1>(acc, fabs((double)rn));
         };
         for(; i + 3 * stride < n; i += 4 * stride)
         {
-            T pp[4], qq[4], xx[4], rr[4];
+            T pp[4], qq[4], xx[4], rr[4], dd[4];
 #pragma unroll
             for(int k = 0; k < 4; k++)
             {
@@ -306,13 +311,14 @@ namespace
                 qq[k] = q[i + k * stride];
                 xx[k] = x[i + k * stride];
                 rr[k] = r[i + k * stride];
+                dd[k] = x0 ? dinv[i + k * stride] : (T)0;
             }
 #pragma unroll
             for(int k = 0; k < 4; k++)
-                upd(pp[k], qq[k], xx[k], rr[k], i + k * stride);
+                upd(pp[k], qq[k], xx[k], rr[k], dd[k], i + k * stride);
         }
         for(; i < n; i += stride)
-            upd(p[i], q[i], x[i], r[i], i);
+            upd(p[i], q[i], x[i], r[i], x0 ? dinv[i] : (T)0, i);
         const double t = block_total<1>(acc);
         if(threadIdx.x == 0)
             part_r[blockIdx.x] = t;
@@ -345,6 +351,16 @@ namespace
         }
         for(; i < n; i += stride)
             p[i] = beta == (T)0 ? z[i] : z[i] + beta * p[i];
+    }
+
+    // x = b * dinv (first Jacobi iterate from zero)
+    template <typename T>
+    __global__ void __launch_bounds__(256)
+        mul_kernel(int n, const T* __restrict__ b, const T* __restrict__ dinv,
+                   T* __restrict__ x)
+    {
+        for(int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+            x[i] = b[i] * dinv[i];
     }
 
     // y32 = (float)(scale * x64): the refinement residual, normalised
@@ -490,12 +506,32 @@ namespace lal
     template <typename T>
     int update_xr(lal_context* c, int n, const T* p, const T* q, T* x, T* r,
                   const double* part_pq, const double* part_gamma, double* part_r,
-                  double* s_alpha)
+                  double* s_alpha, const T* dinv, T* x0)
     {
-        update_xr_kernel<T><<<RB, SPMV_THREADS, 0, c->stream>>>(n, p, q, x, r, part_pq,
-                                                               part_gamma, part_r, s_alpha);
+        update_xr_kernel<T><<<RB, SPMV_THREADS, 0, c->stream>>>(
+            n, p, q, x, r, part_pq, part_gamma, part_r, s_alpha, dinv, x0);
         LAL_LAUNCHED();
         return LAL_OK;
+    }
+
+    template <typename T>
+    int mul_elem(lal_context* c, int n, const T* b, const T* dinv, T* x)
+    {
+        if(n == 0)
+            return LAL_OK;
+        mul_kernel<T><<<grid_for(n, 256, 16384), 256, 0, c->stream>>>(n, b, dinv, x);
+        LAL_LAUNCHED();
+        return LAL_OK;
+    }
+
+    template <typename T>
+    int restrict_init(lal_context* c, const lal_csrmv_info* d, int m, const int* rp,
+                      const int* ci, const T* v, const T* r, const T* dinv_c, T* b_c, T* x_c)
+    {
+        if(m == 0)
+            return LAL_OK;
+        rv_args a{m, d->d_blocks, d->nblocks, rp, ci, v, r, dinv_c, nullptr, b_c, x_c, 1.0, 0.0};
+        return rowvec_launch<T>(c, RV_RESTRICT, a);
     }
 
     template <typename T>
@@ -546,8 +582,11 @@ namespace lal
     template int dot_partials<T>(lal_context*, int, const T*, const T*, double*);            \
     template int norm_partials<T>(lal_context*, int, const T*, double*);                     \
     template int update_xr<T>(lal_context*, int, const T*, const T*, T*, T*, const double*,  \
-                              const double*, double*, double*);                              \
-    template int update_p<T>(lal_context*, int, const T*, T*, const double*, const double*);
+                              const double*, double*, double*, const T*, T*);                \
+    template int update_p<T>(lal_context*, int, const T*, T*, const double*, const double*);  \
+    template int mul_elem<T>(lal_context*, int, const T*, const T*, T*);                     \
+    template int restrict_init<T>(lal_context*, const lal_csrmv_info*, int, const int*,      \
+                                  const int*, const T*, const T*, const T*, T*, T*);
     LAL_INST(double)
     LAL_INST(float)
 #undef LAL_INST
