@@ -66,6 +66,18 @@ def solve_bytes(levels, iters, vbytes):
     return iters * per_iter
 
 
+def measured_traffic():
+    """HBM bytes per FP64 SpMV launch at 128^3 from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (profiles/spmv_poisson128_hbm_traffic.md)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "spmv_poisson128_traffic.txt")
+    try:
+        kv = dict(line.strip().split("=", 1) for line in open(path) if "=" in line)
+        return int(float(kv["traffic_bytes_fp64"]))
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(rp, ci, v, m, b, threads):
     """LinAlgLib's own host path (oracle/_ref, compiled from its sources) on
     the same 128^3 problem when it was built, else the C oracle; SA setup and
@@ -167,6 +179,14 @@ def main():
           f"{b64 / t64_hot / 1e9:.0f}); fp32 {t32 * 1e6:.2f} us ({b32 / t32 / 1e9:.0f} GB/s)",
           file=sys.stderr, flush=True)
     if args.spmv_only:
+        # calibration for hardware byte counters: a plain copy of 512 MiB
+        src = torch.ones(64 << 20, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"# copy calibration: 5 x {src.numel() * 8} bytes read and written",
+              file=sys.stderr, flush=True)
         return
 
     # ---------------------------------------------------------------- setup
@@ -177,9 +197,9 @@ def main():
     setup_s = time.perf_counter() - ts
     levels = []
     for lvl in range(H.levels + 1):
-        arp, aci, _, lm, _ = H.level(lvl, "A")
-        pnnz = len(H.level(lvl, "P")[1]) if lvl < H.levels else 0
-        levels.append((lm, len(aci), pnnz))
+        lm, _, annz = H.dims(lvl, "A")
+        pnnz = H.dims(lvl, "P")[2] if lvl < H.levels else 0
+        levels.append((lm, annz, pnnz))
     print(f"# setup {setup_s:.3f} s, levels {[(l[0], l[1]) for l in levels]}", file=sys.stderr,
           flush=True)
 
@@ -232,6 +252,7 @@ def main():
                    "sample": f"not run: {e}"}
 
     achieved = b64 / t64 / 1e9
+    traffic = measured_traffic() if nx == 128 else None
     line = {
         "metric": "SA-AMG-PCG solve sec to 1e-10 & SpMV GB/s: mixed vs FP64, 1 MI355X",
         "value": round(mx["total_s"] / args.steps, 6),
@@ -273,7 +294,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "kernel": "csrmv_kernel<double> (FP64 CSR SpMV, 3 rotating copies)",
             "bytes_per_launch": b64,
         },

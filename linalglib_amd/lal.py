@@ -246,6 +246,15 @@ class Hierarchy:
         check(load().lal_hierarchy_levels(self.hp, C.byref(n)), "lal_hierarchy_levels")
         return n.value
 
+    def dims(self, level, kind):
+        """(m, n, nnz) of A/P/R at a level, without copying the arrays."""
+        m, n, nnz = i32(), i32(), i32()
+        rp, ci, v = vp(), vp(), vp()
+        check(load().lal_hierarchy_level(self.hp, level, self.KIND[kind], C.byref(m), C.byref(n),
+                                         C.byref(nnz), C.byref(rp), C.byref(ci), C.byref(v)),
+              "lal_hierarchy_level")
+        return m.value, n.value, nnz.value
+
     def level(self, level, kind):
         """Host copy (row_ptr, col_ind, val, m, n) of A/P/R at a level."""
         import numpy as np
