@@ -249,7 +249,14 @@ int lal_ir_solve(lal_handle h, lal_hierarchy hier, const double* b, double* x,
 /* SPD, dL/db = lambda with A lambda = g, and dL/dA_ij = -lambda_i x_j over  */
 /* A's pattern.                                                              */
 /* ------------------------------------------------------------------------ */
-/* vals[k] = scale * u[row(k)] * v[col(k)] over the CSR pattern. */
+/* A lambda = g with the hierarchy built for A: A is SPD, so the adjoint
+ * system is the forward one and its preconditioner is reused.  lambda is
+ * overwritten (zero initial guess).  mixed = 1 runs the FP32-inner /
+ * FP64-outer refinement (hierarchy built with build_f32), else FP64 PCG. */
+int lal_adjoint_solve(lal_handle h, lal_hierarchy hier, const double* g, double* lambda,
+                      int mixed, const lal_solve_options* opts, lal_solve_info* info);
+/* vals[k] = scale * u[row(k)] * v[col(k)] over the CSR pattern; with
+ * scale = -1, u = lambda, v = x this is dL/dA_ij = -lambda_i x_j. */
 int lal_sddmm_f64(lal_handle h, int m, int nnz, const int* row_ptr, const int* col_ind,
                   double scale, const double* u, const double* v, double* vals);
 

@@ -1,0 +1,553 @@
+// lal_linalg.hpp -- LinAlgLib's host C++ API for the SA-AMG-PCG path, on the
+// MI355X C ABI (lal.h).  Header only; links against liblal.so, no HIP headers.
+//
+// The names and argument meaning follow LinAlgLib (paths relative to its
+// library/ tree) so host code written against the reference reads the same:
+//   csr_matrix<T>        include/csr_matrix.h:48-449   (device resident here)
+//   vector<T>            include/vector.h:57-210       (device resident here)
+//   iter_control         include/iterative_solvers/iter_control.h:43-108
+//   hierarchy            include/iterative_solvers/amg/amg.h:65-144
+//   saamg_setup          include/iterative_solvers/amg/saamg.h:195-196
+//   amg_solve            include/iterative_solvers/amg/amg.h (src amg.cpp:516-571)
+//   preconditioner       include/iterative_solvers/preconditioner/preconditioner.h:59-99
+//   cg_solver            include/iterative_solvers/krylov/cg.h (src cg.cpp:45-241)
+// saamg_precond is new: the reference has no AMG preconditioner class; it is
+// the composition x.zeros(); amg_solve(h, x, rhs, 1, 1, vcycle, jacobi,
+// {max_cycle = 0}) executed on the device.
+//
+// Errors keep the reference's convention -- print and carry on -- except
+// that a failed device call also throws std::runtime_error, since there is no
+// host path to fall back to.
+#pragma once
+
+#include "lal.h"
+
+#include <cmath>
+#include <iostream>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace lal_linalg
+{
+    inline lal_handle context()
+    {
+        static lal_handle h = [] {
+            lal_handle t = nullptr;
+            int        s = lal_create(&t);
+            if(s != LAL_OK)
+                throw std::runtime_error(std::string("lal_create: ") + lal_status_string(s));
+            return t;
+        }();
+        return h;
+    }
+
+    inline void check(int s, const char* where)
+    {
+        if(s != LAL_OK)
+        {
+            std::cout << "Error: " << where << ": " << lal_status_string(s) << std::endl;
+            throw std::runtime_error(std::string(where) + ": " + lal_status_string(s));
+        }
+    }
+
+    inline void synchronize()
+    {
+        check(lal_sync(context()), "synchronize");
+    }
+
+    // ------------------------------------------------------------------ vector
+    template <typename T>
+    class vector
+    {
+        T*     d    = nullptr;
+        size_t size = 0;
+
+    public:
+        vector() = default;
+        explicit vector(size_t n)
+        {
+            resize(n);
+        }
+        vector(size_t n, T val)
+        {
+            resize(n);
+            fill(val);
+        }
+        explicit vector(const std::vector<T>& v)
+        {
+            resize(v.size());
+            check(lal_memcpy_h2d(context(), d, v.data(), sizeof(T) * size), "vector(h2d)");
+        }
+        vector(const vector&)            = delete;
+        vector& operator=(const vector&) = delete;
+        vector(vector&& o) noexcept : d(o.d), size(o.size)
+        {
+            o.d    = nullptr;
+            o.size = 0;
+        }
+        ~vector()
+        {
+            lal_free(d);
+        }
+
+        bool is_on_host() const
+        {
+            return false;
+        }
+        size_t get_size() const
+        {
+            return size;
+        }
+        T* get_vec()
+        {
+            return d;
+        }
+        const T* get_vec() const
+        {
+            return d;
+        }
+        void resize(size_t n)
+        {
+            if(n == size)
+                return;
+            lal_free(d);
+            d    = nullptr;
+            size = n;
+            check(lal_alloc((void**)&d, sizeof(T) * n), "vector::resize");
+        }
+        void fill(T val);
+        void zeros()
+        {
+            fill(T(0));
+        }
+        void ones()
+        {
+            fill(T(1));
+        }
+        void copy_from(const vector& x)
+        {
+            resize(x.get_size());
+            check(lal_memcpy_d2d(context(), d, x.d, sizeof(T) * size), "vector::copy_from");
+        }
+        std::vector<T> to_host() const
+        {
+            std::vector<T> h(size);
+            check(lal_memcpy_d2h(context(), h.data(), d, sizeof(T) * size), "vector::to_host");
+            return h;
+        }
+    };
+
+    template <>
+    inline void vector<double>::fill(double v)
+    {
+        check(lal_fill_f64(context(), d, size, v), "vector::fill");
+    }
+    template <>
+    inline void vector<float>::fill(float v)
+    {
+        check(lal_fill_f32(context(), d, size, v), "vector::fill");
+    }
+    template <>
+    inline void vector<int>::fill(int v)
+    {
+        check(lal_fill_i32(context(), d, size, v), "vector::fill");
+    }
+    template <>
+    inline void vector<int64_t>::fill(int64_t v)
+    {
+        check(lal_fill_i64(context(), d, size, v), "vector::fill");
+    }
+
+    // ------------------------------------------------------------- math (BLAS-1)
+    inline void axpy(double alpha, const vector<double>& x, vector<double>& y)
+    {
+        check(lal_axpy_f64(context(), (int)x.get_size(), alpha, x.get_vec(), y.get_vec()), "axpy");
+    }
+    inline void axpby(double alpha, const vector<double>& x, double beta, vector<double>& y)
+    {
+        check(lal_axpby_f64(context(), (int)x.get_size(), alpha, x.get_vec(), beta, y.get_vec()),
+              "axpby");
+    }
+    inline double dot_product(const vector<double>& x, const vector<double>& y)
+    {
+        double r = 0;
+        check(lal_dot_f64(context(), (int)x.get_size(), x.get_vec(), y.get_vec(), &r),
+              "dot_product");
+        return r;
+    }
+    inline double norm_inf(const vector<double>& x)
+    {
+        double r = 0;
+        check(lal_norm_inf_f64(context(), (int)x.get_size(), x.get_vec(), &r), "norm_inf");
+        return r;
+    }
+    template <typename T>
+    inline void exclusive_scan(vector<T>& x);
+    template <>
+    inline void exclusive_scan<int>(vector<int>& x)
+    {
+        check(lal_exclusive_scan_i32(context(), (int)x.get_size(), x.get_vec()), "exclusive_scan");
+    }
+    template <>
+    inline void exclusive_scan<int64_t>(vector<int64_t>& x)
+    {
+        check(lal_exclusive_scan_i64(context(), (int)x.get_size(), x.get_vec()), "exclusive_scan");
+    }
+
+    // -------------------------------------------------------------- csr_matrix
+    template <typename T>
+    class csr_matrix;
+
+    template <>
+    class csr_matrix<double>
+    {
+        int             m = 0, n = 0, nnz = 0;
+        vector<int>     rp, ci;
+        vector<double>  v;
+        mutable lal_csrmv_descr descr = nullptr;
+
+        lal_csrmv_descr analysed() const
+        {
+            if(!descr)
+                check(lal_csrmv_analysis(context(), m, n, nnz, rp.get_vec(), ci.get_vec(), 0,
+                                         &descr),
+                      "csrmv_analysis");
+            return descr;
+        }
+
+    public:
+        csr_matrix() = default;
+        csr_matrix(const std::vector<int>& row_ptr, const std::vector<int>& col_ind,
+                   const std::vector<double>& val, int m_, int n_, int nnz_)
+            : m(m_), n(n_), nnz(nnz_), rp(row_ptr), ci(col_ind), v(val)
+        {
+        }
+        csr_matrix(const csr_matrix&)            = delete;
+        csr_matrix& operator=(const csr_matrix&) = delete;
+        ~csr_matrix()
+        {
+            lal_csrmv_descr_destroy(descr);
+        }
+
+        bool is_on_host() const
+        {
+            return false;
+        }
+        int get_m() const
+        {
+            return m;
+        }
+        int get_n() const
+        {
+            return n;
+        }
+        int get_nnz() const
+        {
+            return nnz;
+        }
+        const int* get_row_ptr() const
+        {
+            return rp.get_vec();
+        }
+        const int* get_col_ind() const
+        {
+            return ci.get_vec();
+        }
+        const double* get_val() const
+        {
+            return v.get_vec();
+        }
+        int* get_row_ptr()
+        {
+            return rp.get_vec();
+        }
+        int* get_col_ind()
+        {
+            return ci.get_vec();
+        }
+        double* get_val()
+        {
+            return v.get_vec();
+        }
+
+        void resize(int m_, int n_, int nnz_)
+        {
+            lal_csrmv_descr_destroy(descr);
+            descr = nullptr;
+            m     = m_;
+            n     = n_;
+            nnz   = nnz_;
+            rp.resize(m + 1);
+            ci.resize(nnz);
+            v.resize(nnz);
+        }
+
+        // y = A x  (csr_matrix.cpp:302-315)
+        void multiply_by_vector(vector<double>& y, const vector<double>& x) const
+        {
+            check(lal_csrmv_f64(context(), analysed(), m, n, nnz, 1.0, get_row_ptr(),
+                                get_col_ind(), get_val(), x.get_vec(), 0.0, y.get_vec()),
+                  "multiply_by_vector");
+        }
+        // y = A x + y  (csr_matrix.cpp:317-330)
+        void multiply_by_vector_and_add(vector<double>& y, const vector<double>& x) const
+        {
+            check(lal_csrmv_f64(context(), analysed(), m, n, nnz, 1.0, get_row_ptr(),
+                                get_col_ind(), get_val(), x.get_vec(), 1.0, y.get_vec()),
+                  "multiply_by_vector_and_add");
+        }
+        void extract_diagonal(vector<double>& diag) const
+        {
+            check(lal_diagonal_f64(context(), m, get_row_ptr(), get_col_ind(), get_val(),
+                                   diag.get_vec()),
+                  "extract_diagonal");
+        }
+        // C = this * B  (csr_matrix.cpp:332-353)
+        void multiply_by_matrix(csr_matrix& C, const csr_matrix& B) const
+        {
+            C.resize(m, B.n, 0);
+            int nnz_c = 0;
+            check(lal_csrgemm_nnz(context(), m, n, B.n, nnz, get_row_ptr(), get_col_ind(), B.nnz,
+                                  B.get_row_ptr(), B.get_col_ind(), C.get_row_ptr(), &nnz_c),
+                  "csrgemm_nnz");
+            C.nnz = nnz_c;
+            C.ci.resize(nnz_c);
+            C.v.resize(nnz_c);
+            check(lal_csrgemm_solve_f64(context(), m, n, B.n, nnz, get_row_ptr(), get_col_ind(),
+                                        get_val(), B.nnz, B.get_row_ptr(), B.get_col_ind(),
+                                        B.get_val(), C.get_row_ptr(), C.get_col_ind(),
+                                        C.get_val()),
+                  "csrgemm_solve");
+        }
+        // T = this^T  (csr_matrix.cpp:437-447)
+        void transpose(csr_matrix& Tr) const
+        {
+            Tr.resize(n, m, nnz);
+            check(lal_transpose_f64(context(), m, n, nnz, get_row_ptr(), get_col_ind(), get_val(),
+                                    Tr.get_row_ptr(), Tr.get_col_ind(), Tr.get_val()),
+                  "transpose");
+        }
+    };
+
+    // r = b - A x  (linalg_math.cpp:110-119)
+    inline void compute_residual(const csr_matrix<double>& A, const vector<double>& x,
+                                 const vector<double>& b, vector<double>& r)
+    {
+        check(lal_residual_f64(context(), nullptr, A.get_m(), A.get_n(), A.get_nnz(),
+                               A.get_row_ptr(), A.get_col_ind(), A.get_val(), x.get_vec(),
+                               b.get_vec(), r.get_vec()),
+              "compute_residual");
+    }
+
+    // ------------------------------------------------------------ iter_control
+    struct iter_control
+    {
+        double rel_tol   = 1e-06;
+        double abs_tol   = 1e-06;
+        int    max_iter  = 1000;
+        int    max_cycle = 200;
+
+        // iterative_solvers/iter_control.cpp:38-78
+        bool residual_converges(double residual_norm, double initial_residual_norm) const
+        {
+            if(std::isnan(residual_norm))
+            {
+                std::cout << "Error: NaN detected in residual norm" << std::endl;
+                return false;
+            }
+            if(std::isnan(initial_residual_norm))
+            {
+                std::cout << "Error: NaN detected in initial residual norm" << std::endl;
+                return false;
+            }
+            if(residual_norm <= abs_tol)
+                return true;
+            return residual_norm / initial_residual_norm <= rel_tol;
+        }
+        bool exceed_max_iter(int iter) const
+        {
+            return iter > max_iter;
+        }
+        bool exceed_max_cycle(int cycle) const
+        {
+            return cycle > max_cycle;
+        }
+    };
+
+    // --------------------------------------------------------------------- AMG
+    enum class cycle
+    {
+        vcycle
+    };
+    enum class smoother
+    {
+        jacobi
+    };
+
+    struct hierarchy
+    {
+        lal_hierarchy h            = nullptr;
+        int           total_levels = 0;
+        hierarchy()                = default;
+        hierarchy(const hierarchy&) = delete;
+        hierarchy& operator=(const hierarchy&) = delete;
+        ~hierarchy()
+        {
+            lal_hierarchy_destroy(h);
+        }
+        bool is_on_host() const
+        {
+            return false;
+        }
+    };
+
+    // saamg.cpp:210-282 on the device; eps = 0.08, relax = 2/3 as the reference.
+    inline void saamg_setup(const csr_matrix<double>& A, int max_level, hierarchy& h,
+                            bool build_f32 = false)
+    {
+        lal_hierarchy_destroy(h.h);
+        h.h = nullptr;
+        lal_saamg_options o;
+        lal_saamg_default_options(&o);
+        o.max_levels = max_level < 20 ? max_level : 20;
+        o.build_f32  = build_f32 ? 1 : 0;
+        check(lal_saamg_setup(context(), A.get_m(), A.get_n(), A.get_nnz(), A.get_row_ptr(),
+                              A.get_col_ind(), A.get_val(), &o, &h.h),
+              "saamg_setup");
+        check(lal_hierarchy_levels(h.h, &h.total_levels), "saamg_setup");
+    }
+
+    // amg.cpp:516-571 as a solver: cycles x += M (b - A x) until converged or
+    // max_cycle; returns the cycle count.
+    inline int amg_solve(const hierarchy& h, vector<double>& x, const vector<double>& b, int n1,
+                         int n2, cycle, smoother, iter_control control)
+    {
+        int            m = (int)b.get_size();
+        vector<double> r(m), z(m);
+        int            mm, nn, nnz;
+        const int *    arp, *aci;
+        const double*  av;
+        check(lal_hierarchy_level(h.h, 0, 0, &mm, &nn, &nnz, &arp, &aci, &av), "amg_solve");
+        auto residual = [&] {
+            check(lal_residual_f64(context(), nullptr, mm, nn, nnz, arp, aci, av, x.get_vec(),
+                                   b.get_vec(), r.get_vec()),
+                  "amg_solve");
+        };
+        residual();
+        double init = norm_inf(r);
+        int    cycle_count = 0;
+        while(!control.exceed_max_cycle(cycle_count))
+        {
+            check(lal_amg_precond_f64(context(), h.h, r.get_vec(), z.get_vec(), n1, n2),
+                  "amg_solve");
+            axpy(1.0, z, x);
+            residual();
+            if(control.residual_converges(norm_inf(r), init))
+                break;
+            cycle_count++;
+        }
+        return cycle_count;
+    }
+
+    // ---------------------------------------------------------- preconditioner
+    class preconditioner
+    {
+    public:
+        virtual ~preconditioner() {}
+        virtual void build(const csr_matrix<double>& A)                        = 0;
+        virtual void solve(const vector<double>& rhs, vector<double>& x) const = 0;
+        virtual void move_to_device() {}
+        virtual void move_to_host() {}
+        virtual bool is_on_host() const
+        {
+            return false;
+        }
+    };
+
+    // One SA-AMG V-cycle (1 pre + 1 post undamped Jacobi sweep) from x = 0.
+    class saamg_precond : public preconditioner
+    {
+    public:
+        hierarchy h;
+        int       n1 = 1, n2 = 1;
+        void build(const csr_matrix<double>& A) override
+        {
+            saamg_setup(A, 20, h);
+        }
+        void solve(const vector<double>& rhs, vector<double>& x) const override
+        {
+            check(lal_amg_precond_f64(context(), h.h, rhs.get_vec(), x.get_vec(), n1, n2),
+                  "saamg_precond::solve");
+        }
+    };
+
+    // --------------------------------------------------------------- cg_solver
+    class cg_solver
+    {
+        vector<double> z, p, res;
+
+    public:
+        void build(const csr_matrix<double>& A)
+        {
+            p.resize(A.get_m());
+            z.resize(A.get_m());
+            res.resize(A.get_m());
+        }
+
+        // cg.cpp:136-223, every step a device call.  With an saamg_precond
+        // the whole loop runs in the library (lal_pcg_solve_f64) instead.
+        int solve_precond(const csr_matrix<double>& A, vector<double>& x,
+                          const vector<double>& b, const preconditioner* precond,
+                          iter_control control)
+        {
+            if(auto* sa = dynamic_cast<const saamg_precond*>(precond))
+            {
+                lal_solve_options o;
+                lal_solve_default_options(&o);
+                o.rel_tol  = control.rel_tol;
+                o.abs_tol  = control.abs_tol;
+                o.max_iter = control.max_iter;
+                o.n1       = sa->n1;
+                o.n2       = sa->n2;
+                lal_solve_info info{};
+                check(lal_pcg_solve_f64(context(), sa->h.h, b.get_vec(), x.get_vec(), &o, &info),
+                      "cg_solver::solve_precond");
+                return info.iterations;
+            }
+            compute_residual(A, x, b, res);
+            double initial_res_norm = norm_inf(res);
+            precond->solve(res, z);
+            p.copy_from(z);
+            double gamma = dot_product(z, res);
+            int    iter  = 0;
+            while(!control.exceed_max_iter(iter))
+            {
+                A.multiply_by_vector(z, p);
+                double alpha = gamma / dot_product(z, p);
+                axpy(alpha, p, x);
+                axpy(-1.0 * alpha, z, res);
+                double res_norm = norm_inf(res);
+                if(control.residual_converges(res_norm, initial_res_norm))
+                    break;
+                precond->solve(res, z);
+                double old_gamma = gamma;
+                gamma            = dot_product(z, res);
+                double beta      = gamma / old_gamma;
+                axpby(1.0, z, beta, p);
+                iter++;
+            }
+            return iter;
+        }
+
+        int solve(const csr_matrix<double>& A, vector<double>& x, const vector<double>& b,
+                  const preconditioner* precond, iter_control control)
+        {
+            if(!precond)
+                throw std::runtime_error("cg_solver: this backend runs the preconditioned path");
+            return solve_precond(A, x, b, precond, control);
+        }
+    };
+}

@@ -75,9 +75,23 @@ def build(verbose=False, jobs=8):
     return LIB
 
 
+def build_cpp_api_test():
+    """The host C++ test of include/lal_linalg.hpp (g++ only, links liblal.so)."""
+    src = os.path.join(ROOT, "tests", "cpp", "test_linalg_api.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "test_linalg_api")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(ROOT, 'include')}", src,
+           f"-L{LIB_DIR}", "-llal", "-Wl,-rpath,$ORIGIN/../../linalglib_amd/_lib", "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        print(r.stdout + r.stderr, file=sys.stderr)
+        raise RuntimeError("g++ failed on the C++ API test")
+    return out
+
+
 def clean():
     shutil.rmtree(OBJ, ignore_errors=True)
 
 
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_cpp_api_test())

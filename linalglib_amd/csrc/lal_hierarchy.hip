@@ -8,10 +8,9 @@
 //   PCG        iterative_solvers/krylov/cg.cpp:136-223, convergence test of
 //              iterative_solvers/iter_control.cpp:38-91
 //
-// Everything stays on the device.  Solver scalars (gamma, alpha, beta) live in
-// device memory and are formed by the last block of each reduction, so a PCG
-// iteration hands one 8-byte residual norm to the host for the convergence
-// test and nothing else.
+// Everything stays on the device.  Solver scalars (gamma, alpha, beta) are
+// formed on the device from per-block partials, so a PCG iteration hands one
+// 8-byte residual norm to the host for the convergence test and nothing else.
 #include "lal_coarse.h"
 #include "lal_vcycle.h"
 
@@ -917,4 +916,14 @@ extern "C" int lal_ir_solve(lal_handle c, lal_hierarchy h, const double* b, doub
         info->final_res_norm   = rn;
     }
     return LAL_OK;
+}
+
+extern "C" int lal_adjoint_solve(lal_handle c, lal_hierarchy h, const double* g, double* lambda,
+                                 int mixed, const lal_solve_options* opts, lal_solve_info* info)
+{
+    LAL_REQUIRE(c != nullptr && h != nullptr && g != nullptr && lambda != nullptr && c == h->c);
+    LAL_HIP(hipMemsetAsync(lambda, 0, sizeof(double) * h->L[0].A.m, c->stream));
+    if(mixed)
+        return lal_ir_solve(c, h, g, lambda, opts, info);
+    return lal_pcg_solve_f64(c, h, g, lambda, opts, info);
 }

@@ -104,6 +104,7 @@ _SIGS = {
     "lal_pcg_solve_f64": [vp, vp, vp, vp, C.POINTER(SolveOptions), C.POINTER(SolveInfo)],
     "lal_ir_solve": [vp, vp, vp, vp, C.POINTER(SolveOptions), C.POINTER(SolveInfo)],
     "lal_sddmm_f64": [vp, i32, i32, vp, vp, f64, vp, vp, vp],
+    "lal_adjoint_solve": [vp, vp, vp, vp, i32, C.POINTER(SolveOptions), C.POINTER(SolveInfo)],
     "lal_status_string": [i32],
     "lal_last_error": [],
 }

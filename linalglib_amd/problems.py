@@ -122,6 +122,94 @@ def diffusion3d(nx, cell_kappa=None, face_kappa=None):
     return _grid_laplacian(shape, 3, face_k=face_kappa)
 
 
+def diffusion_faces(nx):
+    """7-point FV diffusion on an nx^3 grid parameterised by one conductance
+    per face: 3 (nx-1) nx^2 interior faces, then 6 nx^2 Dirichlet boundary
+    faces.  Returns (row_ptr, col_ind, m, n_faces, entry, face, sign): the
+    matrix values for face conductances k are
+
+        val = zeros(nnz); val[entry] += sign * k[face]
+
+    i.e. a_pq = -k_f for the face f between cells p and q, and a_pp = sum of
+    k_f over the six faces of cell p.  A(k) is SPD for k > 0.
+    """
+    n = nx
+    m = n ** 3
+    idx = np.arange(m, dtype=np.int64).reshape(n, n, n)  # [z, y, x]
+    nin = (n - 1) * n * n
+    nb = n * n
+    # interior face ids per axis, on the lower cell of the pair
+    ids = {}
+    off = 0
+    for ax_np, ax in ((2, 0), (1, 1), (0, 2)):
+        shape = [n, n, n]
+        shape[ax_np] = n - 1
+        ids[ax] = off + np.arange(nin, dtype=np.int64).reshape(shape)
+        off += nin
+    bfirst = off
+    n_faces = off + 6 * nb
+    # face of every cell in every direction (interior or boundary)
+    F = {}
+    bk = 0
+    for ax_np, ax in ((2, 0), (1, 1), (0, 2)):
+        for st in (-1, 1):
+            f = np.empty((n, n, n), dtype=np.int64)
+            lo = [slice(None)] * 3
+            hi = [slice(None)] * 3
+            if st == 1:
+                lo[ax_np] = slice(0, n - 1)
+                f[tuple(lo)] = ids[ax]
+                hi[ax_np] = n - 1
+                f[tuple(hi)] = bfirst + bk * nb + np.arange(nb).reshape(n, n)
+            else:
+                lo[ax_np] = slice(1, n)
+                f[tuple(lo)] = ids[ax]
+                hi[ax_np] = 0
+                f[tuple(hi)] = bfirst + bk * nb + np.arange(nb).reshape(n, n)
+            bk += 1
+            F[(ax, st)] = f
+    rp, ci, _, _ = _grid_laplacian((n, n, n), 3)
+    # entry order inside a row: (2,-1) (1,-1) (0,-1) diag (0,1) (1,1) (2,1)
+    order = [(2, -1), (1, -1), (0, -1), None, (0, 1), (1, 1), (2, 1)]
+    coords = {0: np.broadcast_to(np.arange(n)[None, None, :], (n, n, n)),
+              1: np.broadcast_to(np.arange(n)[None, :, None], (n, n, n)),
+              2: np.broadcast_to(np.arange(n)[:, None, None], (n, n, n))}
+    present = []
+    for o in order:
+        if o is None:
+            present.append(np.ones((n, n, n), bool))
+        else:
+            c = coords[o[0]]
+            present.append((c + o[1] >= 0) & (c + o[1] < n))
+    P = np.stack([p.reshape(-1) for p in present], axis=1)
+    epos = np.full(P.shape, -1, dtype=np.int64)
+    epos[P] = np.arange(int(P.sum()))
+    entry, face, sign = [], [], []
+    for s, o in enumerate(order):
+        if o is None:
+            continue
+        ok = P[:, s]
+        entry.append(epos[ok, s])
+        face.append(F[o].reshape(-1)[ok])
+        sign.append(-np.ones(int(ok.sum())))
+    dpos = epos[:, 3]
+    for o in order:
+        if o is None:
+            continue
+        entry.append(dpos)
+        face.append(F[o].reshape(-1))
+        sign.append(np.ones(m))
+    return (rp, ci, m, n_faces, np.concatenate(entry), np.concatenate(face),
+            np.concatenate(sign))
+
+
+def assemble_faces(nnz, entry, face, sign, k):
+    """Values of A(k) for a diffusion_faces pattern (numpy)."""
+    val = np.zeros(nnz)
+    np.add.at(val, entry, sign * k[face])
+    return val
+
+
 def rhs_from_solution(row_ptr, col_ind, val, x):
     """b = A x in float64 (row-sequential sums, like the reference SpMV)."""
     m = len(row_ptr) - 1

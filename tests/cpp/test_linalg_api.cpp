@@ -1,0 +1,174 @@
+// Host C++ written against LinAlgLib's API names, running on the MI355X
+// backend through include/lal_linalg.hpp.  Mirrors the shape of the
+// reference's own SAAMG / CG tests (clients/testing/test_functions_amg.cpp,
+// test_functions_krylov.cpp): build a SPD problem, solve, check the residual.
+// Exit status 0 = all checks passed.
+#include "lal_linalg.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <random>
+#include <vector>
+
+using namespace lal_linalg;
+
+namespace
+{
+    int failures = 0;
+
+    void expect(bool ok, const char* what)
+    {
+        std::printf("%s %s\n", ok ? "OK  " : "FAIL", what);
+        if(!ok)
+            failures++;
+    }
+
+    // 3D 7-point Poisson, lexicographic rows, ascending columns
+    void poisson3d(int n, std::vector<int>& rp, std::vector<int>& ci, std::vector<double>& v)
+    {
+        int m = n * n * n;
+        rp.assign(m + 1, 0);
+        ci.clear();
+        v.clear();
+        for(int k = 0; k < n; k++)
+            for(int j = 0; j < n; j++)
+                for(int i = 0; i < n; i++)
+                {
+                    int r = i + n * (j + n * k);
+                    rp[r] = (int)ci.size();
+                    auto add = [&](int c, double a) {
+                        ci.push_back(c);
+                        v.push_back(a);
+                    };
+                    if(k > 0)
+                        add(r - n * n, -1);
+                    if(j > 0)
+                        add(r - n, -1);
+                    if(i > 0)
+                        add(r - 1, -1);
+                    add(r, 6);
+                    if(i < n - 1)
+                        add(r + 1, -1);
+                    if(j < n - 1)
+                        add(r + n, -1);
+                    if(k < n - 1)
+                        add(r + n * n, -1);
+                }
+        rp[m] = (int)ci.size();
+    }
+
+    std::vector<double> host_spmv(const std::vector<int>& rp, const std::vector<int>& ci,
+                                  const std::vector<double>& v, const std::vector<double>& x)
+    {
+        std::vector<double> y(rp.size() - 1, 0.0);
+        for(size_t i = 0; i + 1 < rp.size(); i++)
+            for(int j = rp[i]; j < rp[i + 1]; j++)
+                y[i] += v[j] * x[ci[j]];
+        return y;
+    }
+
+    double inf_norm_diff(const std::vector<double>& a, const std::vector<double>& b)
+    {
+        double d = 0;
+        for(size_t i = 0; i < a.size(); i++)
+            d = std::fmax(d, std::fabs(a[i] - b[i]));
+        return d;
+    }
+
+    // The V-cycle behind the generic preconditioner interface, so cg_solver
+    // takes its host-driven loop (every step one C-ABI call).
+    class generic_vcycle : public preconditioner
+    {
+    public:
+        const saamg_precond* inner = nullptr;
+        void build(const csr_matrix<double>&) override {}
+        void solve(const vector<double>& rhs, vector<double>& x) const override
+        {
+            inner->solve(rhs, x);
+        }
+    };
+}
+
+int main()
+{
+    const int           n = 24;
+    std::vector<int>    rp, ci;
+    std::vector<double> v;
+    poisson3d(n, rp, ci, v);
+    const int m = n * n * n, nnz = rp[m];
+
+    std::mt19937                           gen(1234567);
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    std::vector<double>                    xs(m);
+    for(auto& t : xs)
+        t = U(gen);
+    std::vector<double> hb = host_spmv(rp, ci, v, xs);
+
+    csr_matrix<double> A(rp, ci, v, m, m, nnz);
+    vector<double>     b(hb), x(m);
+
+    // SpMV and residual against the host
+    {
+        vector<double> xd(xs), y(m), r(m);
+        A.multiply_by_vector(y, xd);
+        expect(inf_norm_diff(y.to_host(), hb) < 1e-12, "multiply_by_vector matches host");
+        compute_residual(A, xd, b, r);
+        expect(inf_norm_diff(r.to_host(), std::vector<double>(m, 0.0)) < 1e-12,
+               "compute_residual of the exact solution is 0");
+        vector<double> ones(m, 1.0);
+        expect(std::fabs(dot_product(ones, ones) - m) == 0.0, "dot_product closed form");
+    }
+    // transpose and SpGEMM against the host on A itself
+    {
+        csr_matrix<double> T, C;
+        A.transpose(T);
+        vector<double> xd(xs), y1(m), y2(m);
+        T.multiply_by_vector(y1, xd);
+        A.multiply_by_vector(y2, xd);
+        expect(inf_norm_diff(y1.to_host(), y2.to_host()) < 1e-12, "transpose of symmetric A");
+        A.multiply_by_matrix(C, A);
+        C.multiply_by_vector(y1, xd);
+        A.multiply_by_vector(y2, xd);
+        vector<double> y3(m);
+        A.multiply_by_vector(y3, y2);
+        expect(inf_norm_diff(y1.to_host(), y3.to_host()) < 1e-10, "multiply_by_matrix (A*A) x");
+    }
+
+    iter_control control;
+    control.rel_tol  = 1e-10;
+    control.abs_tol  = 1e-300;
+    control.max_iter = 500;
+
+    // SA-AMG-PCG, whole loop on the device
+    saamg_precond precond;
+    precond.build(A);
+    cg_solver cg;
+    cg.build(A);
+    x.zeros();
+    int  it_native = cg.solve(A, x, b, &precond, control);
+    auto xh        = x.to_host();
+    expect(inf_norm_diff(xh, xs) < 1e-8, "SA-AMG-PCG solution (device loop)");
+    std::printf("     levels %d, iterations %d\n", precond.h.total_levels, it_native);
+
+    // the same method through the generic preconditioner interface
+    generic_vcycle gv;
+    gv.inner = &precond;
+    x.zeros();
+    int it_generic = cg.solve(A, x, b, &gv, control);
+    expect(std::abs(it_generic - it_native) <= 1, "host-driven PCG loop: same iteration count");
+    expect(inf_norm_diff(x.to_host(), xs) < 1e-8, "host-driven PCG loop: solution");
+
+    // AMG as a stand-alone solver (V-cycles)
+    iter_control c2;
+    c2.rel_tol   = 1e-8;
+    c2.abs_tol   = 1e-300;
+    c2.max_cycle = 200;
+    x.zeros();
+    int cycles = amg_solve(precond.h, x, b, 1, 1, cycle::vcycle, smoother::jacobi, c2);
+    expect(cycles < 200 && inf_norm_diff(x.to_host(), xs) < 1e-6, "amg_solve V-cycles converge");
+    std::printf("     amg_solve cycles %d\n", cycles);
+
+    synchronize();
+    std::printf("%s\n", failures ? "FAILED" : "ALL OK");
+    return failures ? 1 : 0;
+}
