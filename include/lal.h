@@ -243,6 +243,13 @@ int lal_pcg_solve_f64(lal_handle h, lal_hierarchy hier, const double* b, double*
  * inner SA-AMG-PCG (FP64 inner products) to inner_rel_tol; needs build_f32. */
 int lal_ir_solve(lal_handle h, lal_hierarchy hier, const double* b, double* x,
                  const lal_solve_options* opts, lal_solve_info* info);
+/* Mixed precision: cg_solver::solve_precond (iterative_solvers/krylov/cg.cpp:
+ * 136-223) with x, r, p, A p and all inner products in FP64 and the SA-AMG
+ * V-cycle (amg.cpp:144-267) applied on the FP32 levels; needs build_f32.
+ * Same options and convergence test as lal_pcg_solve_f64; residual entries
+ * must stay within FP32's normal range. */
+int lal_pcg_solve_mixed(lal_handle h, lal_hierarchy hier, const double* b, double* x,
+                        const lal_solve_options* opts, lal_solve_info* info);
 
 /* ------------------------------------------------------------------------ */
 /* Adjoint (no reference counterpart): for loss L(x) with x = A^-1 b and A   */
@@ -251,8 +258,9 @@ int lal_ir_solve(lal_handle h, lal_hierarchy hier, const double* b, double* x,
 /* ------------------------------------------------------------------------ */
 /* A lambda = g with the hierarchy built for A: A is SPD, so the adjoint
  * system is the forward one and its preconditioner is reused.  lambda is
- * overwritten (zero initial guess).  mixed = 1 runs the FP32-inner /
- * FP64-outer refinement (hierarchy built with build_f32), else FP64 PCG. */
+ * overwritten (zero initial guess).  mixed = 0: FP64 PCG; 1: FP32-inner /
+ * FP64-outer refinement; 2: FP64 PCG with the FP32 V-cycle (1 and 2 need a
+ * hierarchy built with build_f32). */
 int lal_adjoint_solve(lal_handle h, lal_hierarchy hier, const double* g, double* lambda,
                       int mixed, const lal_solve_options* opts, lal_solve_info* info);
 /* vals[k] = scale * u[row(k)] * v[col(k)] over the CSR pattern; with

@@ -703,20 +703,21 @@ namespace
         LAL_TRY(precond<T>(h, res, z, n1, n2));
         LAL_HIP(hipMemcpyAsync(p, z, sizeof(T) * n, hipMemcpyDeviceToDevice, c->stream));
         int cur = 0;
-        LAL_TRY(lal::dot_partials<T>(c, n, z, res, part_g[cur]));
+        LAL_TRY(lal::dot_partials<T, T>(c, n, z, res, part_g[cur]));
         double rn   = init;
         int    iter = 0;
         bool   conv = false;
         while(!(iter > max_iter))
         {
             LAL_TRY(lal::csrmv(c, A.d, n, A.n, A.nnz, 1.0, A.rp, A.ci, Av, p, 0.0, q));
-            LAL_TRY(lal::dot_partials<T>(c, n, q, p, part_pq));
+            LAL_TRY(lal::dot_partials<T, T>(c, n, q, p, part_pq));
             // with one pre-sweep on the fast path, the update also writes the
             // V-cycle's first Jacobi iterate x0 = r / a_ii
             const bool fast = !h->opts.exact && n1 == 1 && h->levels > 0;
             work<T>&   W0   = h->w<T>(0);
-            LAL_TRY(lal::update_xr<T>(c, n, p, q, x, res, part_pq, part_g[cur], part_r,
-                                      s + lal::S_ALPHA, W0.dinv, fast ? W0.x : nullptr));
+            LAL_TRY(lal::update_xr<T, T>(c, n, p, q, x, res, part_pq, part_g[cur], part_r,
+                                         s + lal::S_ALPHA, W0.dinv, fast ? W0.x : nullptr,
+                                         nullptr));
             LAL_TRY(post_scalar(h, 1, part_r, lal::S_RNORM, &seq));
             // The next preconditioner application depends on r only: it is
             // queued before the host looks at the norm, so the GPU never
@@ -729,14 +730,83 @@ namespace
                 conv = true;
                 break;
             }
-            LAL_TRY(lal::dot_partials<T>(c, n, z, res, part_g[cur ^ 1]));
-            LAL_TRY(lal::update_p<T>(c, n, z, p, part_g[cur ^ 1], part_g[cur]));
+            LAL_TRY(lal::dot_partials<T, T>(c, n, z, res, part_g[cur ^ 1]));
+            LAL_TRY(lal::update_p<T, T>(c, n, z, p, part_g[cur ^ 1], part_g[cur]));
             cur ^= 1;
             iter++;
         }
         if(info)
         {
             info->iterations       = iter;
+            info->converged        = conv;
+            info->initial_res_norm = init;
+            info->final_res_norm   = rn;
+        }
+        return LAL_OK;
+    }
+
+    // cg_solver::solve_precond with the Krylov vectors (x, r, p, q), the
+    // operator product and every inner product in FP64 and the V-cycle on the
+    // FP32 levels.  The residual's FP32 copy and the V-cycle's first Jacobi
+    // iterate are written by the FP64 update kernel, and <z, r> / p = z + beta p
+    // read the FP32 z directly, so the precision change costs no extra pass.
+    // The V-cycle input is r rounded to FP32: residual entries must lie in
+    // FP32's normal range.
+    int pcg_mixed(lal_hierarchy_t* h, const double* b, double* x, double rel_tol,
+                  double abs_tol, int max_iter, int n1, int n2, lal_solve_info* info)
+    {
+        lal_context*   c   = h->c;
+        const csr_dev& A   = h->L[0].A;
+        const int      n   = A.m;
+        double*        s   = h->s;
+        double*        res = h->res;
+        double*        p   = h->p;
+        double*        q   = h->q;
+        float*         r32 = h->res32;
+        float*         z32 = h->z32;
+        LAL_TRY(lal::residual_f64(c, A.d, n, A.n, A.nnz, A.rp, A.ci, A.v, x, b, res));
+        double*  part_pq   = h->part;
+        double*  part_g[2] = {h->part + lal::PCG_BLOCKS, h->part + 2 * lal::PCG_BLOCKS};
+        double*  part_r    = h->part + 3 * lal::PCG_BLOCKS;
+        unsigned seq       = 0;
+        double   init      = 0.0;
+        LAL_TRY(lal::norm_partials<double>(c, n, res, part_r));
+        LAL_TRY(post_scalar(h, 1, part_r, lal::S_RNORM0, &seq));
+        LAL_TRY(wait_scalar(h, seq, lal::S_RNORM0, &init));
+        LAL_TRY(lal::scale_cast_f64_f32(c, n, res, 1.0, r32));
+        LAL_TRY(precond<float>(h, r32, z32, n1, n2));
+        LAL_TRY(lal_convert_f32_f64(c, z32, p, (size_t)n));
+        int cur = 0;
+        LAL_TRY(lal::dot_partials<float, double>(c, n, z32, res, part_g[cur]));
+        double       rn   = init;
+        int          iter = 0;
+        bool         conv = false;
+        const bool   fast = !h->opts.exact && n1 == 1 && h->levels > 0;
+        work<float>& W0   = h->w<float>(0);
+        while(!(iter > max_iter))
+        {
+            LAL_TRY(lal::csrmv(c, A.d, n, A.n, A.nnz, 1.0, A.rp, A.ci, A.v, p, 0.0, q));
+            LAL_TRY(lal::dot_partials<double, double>(c, n, q, p, part_pq));
+            LAL_TRY(lal::update_xr<double, float>(c, n, p, q, x, res, part_pq, part_g[cur],
+                                                  part_r, s + lal::S_ALPHA, W0.dinv,
+                                                  fast ? W0.x : nullptr, r32));
+            LAL_TRY(post_scalar(h, 1, part_r, lal::S_RNORM, &seq));
+            LAL_TRY(precond<float>(h, r32, z32, n1, n2, fast));
+            LAL_TRY(wait_scalar(h, seq, lal::S_RNORM, &rn));
+            if(converges(rn, init, rel_tol, abs_tol))
+            {
+                conv = true;
+                break;
+            }
+            LAL_TRY(lal::dot_partials<float, double>(c, n, z32, res, part_g[cur ^ 1]));
+            LAL_TRY(lal::update_p<float, double>(c, n, z32, p, part_g[cur ^ 1], part_g[cur]));
+            cur ^= 1;
+            iter++;
+        }
+        if(info)
+        {
+            info->iterations       = iter;
+            info->outer_iterations = 0;
             info->converged        = conv;
             info->initial_res_norm = init;
             info->final_res_norm   = rn;
@@ -855,6 +925,23 @@ extern "C" int lal_pcg_solve_f64(lal_handle c, lal_hierarchy h, const double* b,
                        o.n1, o.n2, info);
 }
 
+extern "C" int lal_pcg_solve_mixed(lal_handle c, lal_hierarchy h, const double* b, double* x,
+                                   const lal_solve_options* opts, lal_solve_info* info)
+{
+    LAL_REQUIRE(c != nullptr && h != nullptr && b != nullptr && x != nullptr && c == h->c);
+    if(!h->has32)
+    {
+        lal::set_last_error("hierarchy was built without build_f32");
+        return LAL_ERR_INVALID;
+    }
+    lal_solve_options o;
+    lal_solve_default_options(&o);
+    if(opts)
+        o = *opts;
+    LAL_REQUIRE(o.max_iter >= 0 && o.n1 >= 0 && o.n2 >= 0);
+    return pcg_mixed(h, b, x, o.rel_tol, o.abs_tol, o.max_iter, o.n1, o.n2, info);
+}
+
 extern "C" int lal_ir_solve(lal_handle c, lal_hierarchy h, const double* b, double* x,
                             const lal_solve_options* opts, lal_solve_info* info)
 {
@@ -922,8 +1009,11 @@ extern "C" int lal_adjoint_solve(lal_handle c, lal_hierarchy h, const double* g,
                                  int mixed, const lal_solve_options* opts, lal_solve_info* info)
 {
     LAL_REQUIRE(c != nullptr && h != nullptr && g != nullptr && lambda != nullptr && c == h->c);
+    LAL_REQUIRE(mixed >= 0 && mixed <= 2);
     LAL_HIP(hipMemsetAsync(lambda, 0, sizeof(double) * h->L[0].A.m, c->stream));
-    if(mixed)
+    if(mixed == 1)
         return lal_ir_solve(c, h, g, lambda, opts, info);
+    if(mixed == 2)
+        return lal_pcg_solve_mixed(c, h, g, lambda, opts, info);
     return lal_pcg_solve_f64(c, h, g, lambda, opts, info);
 }

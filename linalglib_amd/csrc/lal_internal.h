@@ -51,10 +51,10 @@ namespace lal
             return lal::hip_status(lal_e_, "kernel launch", __FILE__, __LINE__);       \
     } while(0)
 
-#define LAL_TRY(expr)                                                                  \
+#define LAL_TRY(...)                                                                   \
     do                                                                                 \
     {                                                                                  \
-        int lal_s_ = (expr);                                                           \
+        int lal_s_ = (__VA_ARGS__);                                                    \
         if(lal_s_ != LAL_OK)                                                           \
             return lal_s_;                                                             \
     } while(0)

@@ -68,19 +68,20 @@ namespace lal
     template <typename T>
     int coarse_dense(lal_context* c, int n, const double* inv, const T* b, T* x);
     // PCG_BLOCKS partials of <x, y> / max|x| (FP64 accumulation).
-    template <typename T>
-    int dot_partials(lal_context* c, int n, const T* x, const T* y, double* part);
+    template <typename TX, typename TY>
+    int dot_partials(lal_context* c, int n, const TX* x, const TY* y, double* part);
     template <typename T>
     int norm_partials(lal_context* c, int n, const T* x, double* part);
     // op 0: sum, 1: max of the partials into *slot (device) and, when host_val
     // is given (mapped pinned memory), into *host_val then *host_seq = seq.
     int finish_partials(lal_context* c, int op, const double* part, double* slot,
                         double* host_val, unsigned* host_seq, unsigned seq);
-    // x += alpha p, r -= alpha q, partial max|r|; with x0 also x0 = r * dinv.
-    template <typename T>
+    // x += alpha p, r -= alpha q, partial max|r|; with x0 also x0 = (S)r * dinv,
+    // with rs also rs = (S)r (the preconditioner's precision S).
+    template <typename T, typename S>
     int update_xr(lal_context* c, int n, const T* p, const T* q, T* x, T* r,
                   const double* part_pq, const double* part_gamma, double* part_r,
-                  double* s_alpha, const T* dinv, T* x0);
+                  double* s_alpha, const S* dinv, S* x0, S* rs);
     // x = b * dinv
     template <typename T>
     int mul_elem(lal_context* c, int n, const T* b, const T* dinv, T* x);
@@ -88,8 +89,8 @@ namespace lal
     template <typename T>
     int restrict_init(lal_context* c, const lal_csrmv_info* d, int m, const int* rp,
                       const int* ci, const T* v, const T* r, const T* dinv_c, T* b_c, T* x_c);
-    template <typename T>
-    int update_p(lal_context* c, int n, const T* z, T* p, const double* part_new,
+    template <typename TZ, typename T>
+    int update_p(lal_context* c, int n, const TZ* z, T* p, const double* part_new,
                  const double* part_old);
     int scale_cast_f64_f32(lal_context* c, int n, const double* x, double scale, float* y);
     int axpy_f32_f64(lal_context* c, int n, double alpha, const float* d, double* x);

@@ -218,10 +218,11 @@ namespace
         return block_total<0>(v);
     }
 
-    // part[block] = partial of sum x_i*y_i (OP 0) or max |x_i| (OP 1).
-    template <typename T, int OP>
+    // part[block] = partial of sum x_i*y_i (OP 0) or max |x_i| (OP 1); x and y
+    // may differ in precision (FP32 preconditioned vector, FP64 residual).
+    template <typename TX, typename TY, int OP>
     __global__ void __launch_bounds__(SPMV_THREADS)
-        partial_kernel(int n, const T* __restrict__ x, const T* __restrict__ y,
+        partial_kernel(int n, const TX* __restrict__ x, const TY* __restrict__ y,
                        double* __restrict__ part)
     {
         double    acc    = 0.0;
@@ -229,12 +230,13 @@ namespace
         int       i      = blockIdx.x * SPMV_THREADS + threadIdx.x;
         for(; i + 3 * stride < n; i += 4 * stride)
         {
-            T a[4], b[4];
+            TX a[4];
+            TY b[4];
 #pragma unroll
             for(int k = 0; k < 4; k++)
             {
                 a[k] = x[i + k * stride];
-                b[k] = OP == 0 ? y[i + k * stride] : a[k];
+                b[k] = OP == 0 ? y[i + k * stride] : (TY)a[k];
             }
 #pragma unroll
             for(int k = 0; k < 4; k++)
@@ -273,15 +275,16 @@ namespace
     // alpha = gamma / <q, p>; x += alpha p; r += (-1*alpha) q (cg.cpp:185-193);
     // part_r[block] = max |r_i|.
     // When x0 is given, also x0 = r * dinv: the first Jacobi iterate from zero
-    // of the preconditioner applied next to r.
-    template <typename T>
+    // of the preconditioner applied next to r, in the preconditioner's
+    // precision S; when rs is given, rs = (S) r, the preconditioner's input.
+    template <typename T, typename S>
     __global__ void __launch_bounds__(SPMV_THREADS)
         update_xr_kernel(int n, const T* __restrict__ p, const T* __restrict__ q,
                          T* __restrict__ x, T* __restrict__ r,
                          const double* __restrict__ part_pq,
                          const double* __restrict__ part_gamma, double* __restrict__ part_r,
-                         double* __restrict__ s_alpha, const T* __restrict__ dinv,
-                         T* __restrict__ x0)
+                         double* __restrict__ s_alpha, const S* __restrict__ dinv,
+                         S* __restrict__ x0, S* __restrict__ rs)
     {
         const int stride = gridDim.x * SPMV_THREADS;
         int       i      = blockIdx.x * SPMV_THREADS + threadIdx.x;
@@ -293,17 +296,21 @@ namespace
         const T a  = (T)alpha;
         const T ma = (T)(-1.0 * alpha);
         double  acc = 0.0;
-        auto    upd = [&](T pi, T qi, T xi, T ri, T di, int at) {
+        auto    upd = [&](T pi, T qi, T xi, T ri, S di, int at) {
             x[at]    = a == (T)1 ? pi + xi : a * pi + xi;
             T rn     = ma == (T)1 ? qi + ri : ma * qi + ri;
             r[at]    = rn;
+            const S r_s = (S)rn;
+            if(rs)
+                rs[at] = r_s;
             if(x0)
-                x0[at] = rn * di;
+                x0[at] = r_s * di;
             acc = combine<1>(acc, fabs((double)rn));
         };
         for(; i + 3 * stride < n; i += 4 * stride)
         {
-            T pp[4], qq[4], xx[4], rr[4], dd[4];
+            T pp[4], qq[4], xx[4], rr[4];
+            S dd[4];
 #pragma unroll
             for(int k = 0; k < 4; k++)
             {
@@ -311,23 +318,24 @@ namespace
                 qq[k] = q[i + k * stride];
                 xx[k] = x[i + k * stride];
                 rr[k] = r[i + k * stride];
-                dd[k] = x0 ? dinv[i + k * stride] : (T)0;
+                dd[k] = x0 ? dinv[i + k * stride] : (S)0;
             }
 #pragma unroll
             for(int k = 0; k < 4; k++)
                 upd(pp[k], qq[k], xx[k], rr[k], dd[k], i + k * stride);
         }
         for(; i < n; i += stride)
-            upd(p[i], q[i], x[i], r[i], x0 ? dinv[i] : (T)0, i);
+            upd(p[i], q[i], x[i], r[i], x0 ? dinv[i] : (S)0, i);
         const double t = block_total<1>(acc);
         if(threadIdx.x == 0)
             part_r[blockIdx.x] = t;
     }
 
-    // beta = gamma_new / gamma_old; p = 1*z + beta*p (cg.cpp:205-211).
-    template <typename T>
+    // beta = gamma_new / gamma_old; p = 1*z + beta*p (cg.cpp:205-211); z may be
+    // the FP32 output of an FP32 preconditioner.
+    template <typename TZ, typename T>
     __global__ void __launch_bounds__(SPMV_THREADS)
-        update_p_kernel(int n, const T* __restrict__ z, T* __restrict__ p,
+        update_p_kernel(int n, const TZ* __restrict__ z, T* __restrict__ p,
                         const double* __restrict__ part_new,
                         const double* __restrict__ part_old)
     {
@@ -342,7 +350,7 @@ namespace
 #pragma unroll
             for(int k = 0; k < 4; k++)
             {
-                zz[k] = z[i + k * stride];
+                zz[k] = (T)z[i + k * stride];
                 pp[k] = p[i + k * stride];
             }
 #pragma unroll
@@ -350,7 +358,7 @@ namespace
                 p[i + k * stride] = beta == (T)0 ? zz[k] : zz[k] + beta * pp[k];
         }
         for(; i < n; i += stride)
-            p[i] = beta == (T)0 ? z[i] : z[i] + beta * p[i];
+            p[i] = beta == (T)0 ? (T)z[i] : (T)z[i] + beta * p[i];
     }
 
     // x = b * dinv (first Jacobi iterate from zero)
@@ -474,10 +482,10 @@ namespace lal
         return LAL_OK;
     }
 
-    template <typename T>
-    int dot_partials(lal_context* c, int n, const T* x, const T* y, double* part)
+    template <typename TX, typename TY>
+    int dot_partials(lal_context* c, int n, const TX* x, const TY* y, double* part)
     {
-        partial_kernel<T, 0><<<RB, SPMV_THREADS, 0, c->stream>>>(n, x, y, part);
+        partial_kernel<TX, TY, 0><<<RB, SPMV_THREADS, 0, c->stream>>>(n, x, y, part);
         LAL_LAUNCHED();
         return LAL_OK;
     }
@@ -485,7 +493,7 @@ namespace lal
     template <typename T>
     int norm_partials(lal_context* c, int n, const T* x, double* part)
     {
-        partial_kernel<T, 1><<<RB, SPMV_THREADS, 0, c->stream>>>(n, x, x, part);
+        partial_kernel<T, T, 1><<<RB, SPMV_THREADS, 0, c->stream>>>(n, x, x, part);
         LAL_LAUNCHED();
         return LAL_OK;
     }
@@ -503,13 +511,13 @@ namespace lal
         return LAL_OK;
     }
 
-    template <typename T>
+    template <typename T, typename S>
     int update_xr(lal_context* c, int n, const T* p, const T* q, T* x, T* r,
                   const double* part_pq, const double* part_gamma, double* part_r,
-                  double* s_alpha, const T* dinv, T* x0)
+                  double* s_alpha, const S* dinv, S* x0, S* rs)
     {
-        update_xr_kernel<T><<<RB, SPMV_THREADS, 0, c->stream>>>(
-            n, p, q, x, r, part_pq, part_gamma, part_r, s_alpha, dinv, x0);
+        update_xr_kernel<T, S><<<RB, SPMV_THREADS, 0, c->stream>>>(
+            n, p, q, x, r, part_pq, part_gamma, part_r, s_alpha, dinv, x0, rs);
         LAL_LAUNCHED();
         return LAL_OK;
     }
@@ -534,11 +542,12 @@ namespace lal
         return rowvec_launch<T>(c, RV_RESTRICT, a);
     }
 
-    template <typename T>
-    int update_p(lal_context* c, int n, const T* z, T* p, const double* part_new,
+    template <typename TZ, typename T>
+    int update_p(lal_context* c, int n, const TZ* z, T* p, const double* part_new,
                  const double* part_old)
     {
-        update_p_kernel<T><<<RB, SPMV_THREADS, 0, c->stream>>>(n, z, p, part_new, part_old);
+        update_p_kernel<TZ, T><<<RB, SPMV_THREADS, 0, c->stream>>>(n, z, p, part_new,
+                                                                   part_old);
         LAL_LAUNCHED();
         return LAL_OK;
     }
@@ -579,17 +588,27 @@ namespace lal
     template int coarse_gs<T>(lal_context*, int, const int*, const int*, const T*, const T*, \
                               T*);                                                           \
     template int coarse_dense<T>(lal_context*, int, const double*, const T*, T*);            \
-    template int dot_partials<T>(lal_context*, int, const T*, const T*, double*);            \
+    template int dot_partials<T, T>(lal_context*, int, const T*, const T*, double*);         \
     template int norm_partials<T>(lal_context*, int, const T*, double*);                     \
-    template int update_xr<T>(lal_context*, int, const T*, const T*, T*, T*, const double*,  \
-                              const double*, double*, double*, const T*, T*);                \
-    template int update_p<T>(lal_context*, int, const T*, T*, const double*, const double*);  \
+    template int update_xr<T, T>(lal_context*, int, const T*, const T*, T*, T*,              \
+                                 const double*, const double*, double*, double*, const T*,   \
+                                 T*, T*);                                                    \
+    template int update_p<T, T>(lal_context*, int, const T*, T*, const double*,              \
+                                const double*);                                              \
     template int mul_elem<T>(lal_context*, int, const T*, const T*, T*);                     \
     template int restrict_init<T>(lal_context*, const lal_csrmv_info*, int, const int*,      \
                                   const int*, const T*, const T*, const T*, T*, T*);
     LAL_INST(double)
     LAL_INST(float)
 #undef LAL_INST
+    // FP64 Krylov vectors with an FP32 preconditioner
+    template int dot_partials<float, double>(lal_context*, int, const float*, const double*,
+                                             double*);
+    template int update_xr<double, float>(lal_context*, int, const double*, const double*,
+                                          double*, double*, const double*, const double*,
+                                          double*, double*, const float*, float*, float*);
+    template int update_p<float, double>(lal_context*, int, const float*, double*,
+                                         const double*, const double*);
 }
 
 extern "C" int lal_sddmm_f64(lal_handle c, int m, int nnz, const int* rp, const int* ci,

@@ -103,6 +103,7 @@ _SIGS = {
     "lal_solve_default_options": [C.POINTER(SolveOptions)],
     "lal_pcg_solve_f64": [vp, vp, vp, vp, C.POINTER(SolveOptions), C.POINTER(SolveInfo)],
     "lal_ir_solve": [vp, vp, vp, vp, C.POINTER(SolveOptions), C.POINTER(SolveInfo)],
+    "lal_pcg_solve_mixed": [vp, vp, vp, vp, C.POINTER(SolveOptions), C.POINTER(SolveInfo)],
     "lal_sddmm_f64": [vp, i32, i32, vp, vp, f64, vp, vp, vp],
     "lal_adjoint_solve": [vp, vp, vp, vp, i32, C.POINTER(SolveOptions), C.POINTER(SolveInfo)],
     "lal_status_string": [i32],
@@ -283,6 +284,14 @@ class Hierarchy:
         o = solve_options(**opts)
         info = SolveInfo()
         call("lal_ir_solve", self.handle.h, self.hp, ptr(b), ptr(x), C.byref(o), C.byref(info))
+        return info
+
+    def pcg_mixed(self, b, x, **opts):
+        """FP64 PCG with the FP32 V-cycle as preconditioner (needs build_f32)."""
+        o = solve_options(**opts)
+        info = SolveInfo()
+        call("lal_pcg_solve_mixed", self.handle.h, self.hp, ptr(b), ptr(x), C.byref(o),
+             C.byref(info))
         return info
 
 

@@ -12,7 +12,9 @@ values `val`.  Backward, for a loss L with g = dL/dx:
 Gradients with respect to parameters that assemble `val` (face
 conductances, edge weights) follow from torch's own autograd through the
 assembly, e.g. `val = torch.zeros(nnz).index_add(0, entry, sign * k[face])`.
-With mixed=True both solves run FP32 inner SA-AMG-PCG inside FP64 refinement.
+mixed selects the precision of both solves: False (FP64 throughout), "vcycle"
+(FP64 PCG with the V-cycle on FP32 levels; also what True means) or "ir"
+(FP32 inner SA-AMG-PCG inside FP64 iterative refinement).
 """
 import ctypes as C
 
@@ -27,11 +29,12 @@ class _Solve(torch.autograd.Function):
         m = b.numel()
         nnz = val.numel()
         val_c = val.detach().contiguous()
-        H = lal.Hierarchy(handle, m, nnz, row_ptr, col_ind, val_c, build_f32=int(mixed))
+        H = lal.Hierarchy(handle, m, nnz, row_ptr, col_ind, val_c, build_f32=int(mixed != 0))
         x = torch.zeros(m, dtype=torch.float64, device=b.device)
         b_c = b.detach().contiguous()
         opts = dict(rel_tol=tol, abs_tol=1e-300, max_iter=1000)
-        info = H.ir(b_c, x, **opts) if mixed else H.pcg(b_c, x, **opts)
+        solve = {0: H.pcg, 1: H.ir, 2: H.pcg_mixed}[mixed]
+        info = solve(b_c, x, **opts)
         if not info.converged:
             raise lal.LalError(4, "saamg_solve forward",
                                f"residual {info.final_res_norm:.3e} after {info.iterations}")
@@ -80,4 +83,7 @@ def saamg_solve(val, b, row_ptr, col_ind, mixed=False, tol=1e-12):
     """x = A^-1 b with A = CSR(row_ptr, col_ind, val); differentiable in val and b."""
     if val.dtype != torch.float64 or b.dtype != torch.float64:
         raise TypeError("saamg_solve takes float64 values and right-hand side")
-    return _Solve.apply(val, b, row_ptr, col_ind, _handle(), bool(mixed), float(tol))
+    modes = {False: 0, True: 2, "ir": 1, "vcycle": 2}
+    if mixed not in modes:
+        raise ValueError(f"mixed must be False, True, 'ir' or 'vcycle', not {mixed!r}")
+    return _Solve.apply(val, b, row_ptr, col_ind, _handle(), modes[mixed], float(tol))

@@ -267,6 +267,41 @@ def test_mixed_precision_ir(h, oracle, name):
     assert np.linalg.norm(xg - xo) <= 1e-8 * np.linalg.norm(xo)
 
 
+@pytest.mark.parametrize("name", ["3d16", "diff12", "2d128"])
+def test_mixed_precision_pcg_f32_vcycle(h, oracle, name):
+    """FP64 PCG with the V-cycle on FP32 levels: the Krylov recurrence is
+    FP64, so the solution reaches the FP64 solver's accuracy and the
+    iteration count stays within a few of the FP64 PCG's."""
+    rp, ci, v, m = grid(name)
+    A = CSR(rp, ci, v, m, m)
+    b = pb.rhs_from_solution(rp, ci, v, pb.uniform(m, seed=1))
+    log(f"pcg_mixed {name}")
+    H = _hier(h, A, build_f32=1)
+    B = dev(b)
+    x = empty(m, torch.float64)
+    info = H.pcg_mixed(B, x, rel_tol=1e-10, abs_tol=1e-300, max_iter=500)
+    x64 = empty(m, torch.float64)
+    info64 = H.pcg(B, x64, rel_tol=1e-10, abs_tol=1e-300, max_iter=500)
+    assert info.converged and info64.converged
+    assert abs(info.iterations - info64.iterations) <= 3, (info.iterations, info64.iterations)
+    xg = host(x)
+    r = oracle.residual(A, xg, b)
+    assert np.abs(r).max() <= 1e-9 * np.abs(b).max()
+    xo, _, _ = oracle.saamg_setup(A).pcg(b, rel_tol=1e-12, coarse_solver=1)
+    assert np.linalg.norm(xg - xo) <= 1e-8 * np.linalg.norm(xo)
+
+
+def test_mixed_requires_f32_levels(h):
+    rp, ci, v, m = grid("2d8")
+    H = _hier(h, CSR(rp, ci, v, m, m))
+    B = dev(np.ones(m))
+    x = empty(m, torch.float64)
+    with pytest.raises(lal.LalError):
+        H.pcg_mixed(B, x)
+    with pytest.raises(lal.LalError):
+        H.ir(B, x)
+
+
 def test_sddmm(h):
     A = random_csr(np.random.default_rng(8), 200, 200, 9)
     d = DevCSR(A)
