@@ -125,7 +125,7 @@ namespace
             LAL_REQUIRE(d->m == m && d->nnz == nnz);
         if(d && !d->exact)
         {
-            lal::rv_args a{m,    d->d_blocks, d->nblocks,     rp,           ci, v, x, nullptr,
+            lal::rv_args a{m,    d->nnz, d->d_blocks, d->nblocks,     rp,           ci, v, x, nullptr,
                            b,    y,           nullptr,        (double)alpha, (double)beta};
             return lal::rowvec_launch<T>(c, MODE == 0 ? lal::RV_SPMV : lal::RV_RESIDUAL, a);
         }

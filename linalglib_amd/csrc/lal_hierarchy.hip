@@ -128,10 +128,13 @@ struct lal_hierarchy_t
     unsigned* d_seq      = nullptr;
     unsigned  seq        = 0;
     bool      has32      = false;
-    // coarse tail: levels tail_first..levels run in one workgroup (fast path)
-    int                          tail_first = -1;
-    lal::coarse_level<double>*   d_tail64   = nullptr;
-    lal::coarse_level<float>*    d_tail32   = nullptr;
+    // coarse tail: levels tail_first..levels run in one workgroup out of LDS
+    // (fast path); one packed device image of their arrays per precision
+    int            tail_first = -1;
+    int*           tail_img64 = nullptr;
+    int*           tail_img32 = nullptr;
+    lal::tail_desc tail64{};
+    lal::tail_desc tail32{};
 
     ~lal_hierarchy_t()
     {
@@ -149,7 +152,7 @@ struct lal_hierarchy_t
         }
         for(void* ptr : {(void*)coarse_inv, (void*)res, (void*)z, (void*)p, (void*)q,
                          (void*)res32, (void*)z32, (void*)p32, (void*)q32, (void*)d32, (void*)s,
-                         (void*)part, (void*)d_tail64, (void*)d_tail32})
+                         (void*)part, (void*)tail_img64, (void*)tail_img32})
             (void)hipFree(ptr);
         if(h_note)
             (void)hipHostFree(h_note);
@@ -186,16 +189,28 @@ namespace
     }
 
     template <typename T>
-    const lal::coarse_level<T>* tail_levels(const lal_hierarchy_t* h);
+    const lal::tail_desc& tail_of(const lal_hierarchy_t* h);
     template <>
-    const lal::coarse_level<double>* tail_levels<double>(const lal_hierarchy_t* h)
+    const lal::tail_desc& tail_of<double>(const lal_hierarchy_t* h)
     {
-        return h->d_tail64;
+        return h->tail64;
     }
     template <>
-    const lal::coarse_level<float>* tail_levels<float>(const lal_hierarchy_t* h)
+    const lal::tail_desc& tail_of<float>(const lal_hierarchy_t* h)
     {
-        return h->d_tail32;
+        return h->tail32;
+    }
+    template <typename T>
+    const int* tail_image(const lal_hierarchy_t* h);
+    template <>
+    const int* tail_image<double>(const lal_hierarchy_t* h)
+    {
+        return h->tail_img64;
+    }
+    template <>
+    const int* tail_image<float>(const lal_hierarchy_t* h)
+    {
+        return h->tail_img32;
     }
 
     // r = b - A x in the level's precision
@@ -260,41 +275,103 @@ namespace
         return true;
     }
 
-    // Device array describing levels tail_first..levels for the coarse tail.
+    // Layout of the coarse tail's LDS image for levels first..levels in
+    // precision T (word offsets; values 8-byte aligned for FP64).  With
+    // `img` the arrays are also copied into a fresh device image.  Returns
+    // false when the levels do not fit the tail's LDS.
     template <typename T>
-    int build_tail_levels(lal_hierarchy_t* h, lal::coarse_level<T>** out)
+    bool tail_layout(lal_hierarchy_t* h, int first, lal::tail_desc* d, int** img, int* status)
     {
-        std::vector<lal::coarse_level<T>> v;
-        for(int l = h->tail_first; l <= h->levels; l++)
+        constexpr int S = (int)sizeof(T) / 4;
+        *status         = LAL_OK;
+        const int nl    = h->levels - first + 1;
+        if(first < 1 || nl < 2 || nl > lal::TAIL_MAX_LEVELS)
+            return false;
+        std::memset(d, 0, sizeof(*d));
+        d->nl   = nl;
+        d->kind = h->coarse_kind == 0 ? 0 : 1;
+        long w  = 0;
+        struct piece
         {
-            level_t&             L = h->L[l];
-            work<T>&             W = h->w<T>(l);
-            lal::coarse_level<T> d{};
-            d.m    = L.A.m;
-            d.rp   = L.A.rp;
-            d.ci   = L.A.ci;
-            d.v    = vals<T>(L.A);
-            d.dinv = W.dinv;
+            long        off;
+            const void* src;
+            size_t      bytes;
+        };
+        std::vector<piece> pieces;
+        auto put = [&](const void* src, long words, int align) -> int {
+            w            = (w + align - 1) / align * align;
+            const long o = w;
+            if(words > 0)
+                pieces.push_back({o, src, (size_t)words * 4});
+            w += words;
+            return (int)o;
+        };
+        for(int k = 0; k < nl; k++)
+        {
+            const int       l  = first + k;
+            level_t&        L  = h->L[l];
+            work<T>&        W  = h->w<T>(l);
+            lal::tail_level& t = d->L[k];
+            t.m                = L.A.m;
+            t.rp               = put(L.A.rp, L.A.m + 1, 1);
+            t.ci               = put(L.A.ci, L.A.nnz, 1);
+            t.v                = put(vals<T>(L.A), (long)L.A.nnz * S, S);
+            t.dinv             = put(W.dinv, (long)L.A.m * S, S);
+            t.prp = t.pci = t.pv = t.rrp = t.rci = t.rv = -1;
             if(l < h->levels)
             {
-                d.prp = L.P.rp;
-                d.pci = L.P.ci;
-                d.pv  = vals<T>(L.P);
-                d.rrp = L.R.rp;
-                d.rci = L.R.ci;
-                d.rv  = vals<T>(L.R);
+                t.prp = put(L.P.rp, L.P.m + 1, 1);
+                t.pci = put(L.P.ci, L.P.nnz, 1);
+                t.pv  = put(vals<T>(L.P), (long)L.P.nnz * S, S);
+                t.rrp = put(L.R.rp, L.R.m + 1, 1);
+                t.rci = put(L.R.ci, L.R.nnz, 1);
+                t.rv  = put(vals<T>(L.R), (long)L.R.nnz * S, S);
             }
-            d.x = W.x;
-            d.t = W.t;
-            d.b = W.b;
-            d.r = W.r;
-            v.push_back(d);
         }
-        LAL_TRY(alloc_n(out, v.size()));
-        LAL_HIP(hipMemcpyAsync(*out, v.data(), sizeof(lal::coarse_level<T>) * v.size(),
-                               hipMemcpyHostToDevice, h->c->stream));
-        LAL_HIP(hipStreamSynchronize(h->c->stream));
-        return LAL_OK;
+        // the coarsest dense inverse joins the image when it fits
+        d->inv            = -1;
+        const long budget = lal::TAIL_LDS_BYTES / 4;
+        long       vec    = 0;
+        for(int k = 0; k < nl; k++)
+            vec += 4 * ((long)d->L[k].m * S + S);
+        const int mc = h->L[h->levels].A.m;
+        if(d->kind == 1 && h->coarse_inv)
+        {
+            const long inv_words = 2L * mc * mc;
+            if(((w + 1) / 2 * 2) + inv_words + vec + 8 <= budget)
+                d->inv = put(h->coarse_inv, inv_words, 2);
+        }
+        w            = (w + 3) / 4 * 4;
+        d->img_words = (int)w;
+        for(int k = 0; k < nl; k++)
+        {
+            lal::tail_level& t = d->L[k];
+            t.x                = put(nullptr, (long)t.m * S, S);
+            t.t                = put(nullptr, (long)t.m * S, S);
+            t.b                = put(nullptr, (long)t.m * S, S);
+            t.r                = put(nullptr, (long)t.m * S, S);
+        }
+        d->lds_words = (int)w;
+        if(w > budget)
+            return false;
+        if(img)
+        {
+            *status = alloc_n(img, (size_t)d->img_words);
+            if(*status != LAL_OK)
+                return false;
+            for(const piece& p : pieces)
+                if(p.src)
+                {
+                    hipError_t e = hipMemcpyAsync(*img + p.off, p.src, p.bytes,
+                                                  hipMemcpyDeviceToDevice, h->c->stream);
+                    if(e != hipSuccess)
+                    {
+                        *status = lal::hip_status(e, "tail image", __FILE__, __LINE__);
+                        return false;
+                    }
+                }
+        }
+        return true;
     }
 
     int to_f32(lal_context* c, const double* x, float** y, size_t n)
@@ -386,20 +463,29 @@ namespace
         else
             h->coarse_kind = 1;
 
-        // coarse tail: the deepest run of levels with few nonzeros (at least
-        // two levels, never level 0), on the fast path with a direct coarsest
-        // solve
-        if(!exact && h->coarse_kind != 1)
+        // coarse tail: the deepest run of levels (at least two, never level 0)
+        // whose FP64 arrays fit the tail's LDS, on the fast path with a direct
+        // coarsest solve
+        if(!exact && h->coarse_kind != 1 && h->levels >= 2)
         {
-            int first = h->levels;
-            while(first > 1 && h->L[first - 1].A.nnz <= lal::COARSE_TAIL_NNZ)
-                first--;
-            if(first < h->levels && h->L[first].A.nnz <= lal::COARSE_TAIL_NNZ)
+            lal::tail_desc probe{};
+            int            st    = LAL_OK;
+            int            first = -1;
+            for(int f = h->levels - 1; f >= 1; f--)
             {
+                if(!tail_layout<double>(h, f, &probe, nullptr, &st))
+                    break;
+                first = f;
+            }
+            if(first >= 1)
+            {
+                if(!tail_layout<double>(h, first, &h->tail64, &h->tail_img64, &st))
+                    return st != LAL_OK ? st : LAL_ERR_INVALID;
+                if(h->opts.build_f32 &&
+                   !tail_layout<float>(h, first, &h->tail32, &h->tail_img32, &st))
+                    return st != LAL_OK ? st : LAL_ERR_INVALID;
                 h->tail_first = first;
-                LAL_TRY(build_tail_levels<double>(h, &h->d_tail64));
-                if(h->opts.build_f32)
-                    LAL_TRY(build_tail_levels<float>(h, &h->d_tail32));
+                LAL_HIP(hipStreamSynchronize(h->c->stream));
             }
         }
 
@@ -537,12 +623,12 @@ namespace
         // update for level 0) when xinit is set; this level then only forms
         // r = b - A x.
         const bool fast = !h->opts.exact && n1 == 1;
-        const lal::coarse_level<T>* tail = tail_levels<T>(h);
-        if(fast && n2 == 1 && l == h->tail_first && tail && b == W.b)
+        if(fast && n2 == 1 && l == h->tail_first && tail_image<T>(h))
         {
-            LAL_TRY(lal::coarse_tail<T>(c, tail, h->levels - l + 1, h->coarse_kind == 0 ? 0 : 1,
-                                        h->coarse_inv, dst));
-            *out = dst ? dst : W.t;
+            T* res = dst ? dst : W.t;
+            LAL_TRY(lal::coarse_tail<T>(c, tail_of<T>(h), tail_image<T>(h), b, h->coarse_inv,
+                                        res));
+            *out = res;
             return LAL_OK;
         }
         if(l == h->levels)

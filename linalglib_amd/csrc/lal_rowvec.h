@@ -1,15 +1,19 @@
-// Fast (non-exact) CSR kernels: products staged in LDS.
+// Fast (non-exact) CSR kernels.
 //
 // A 256-lane workgroup owns one row block of the analysis (at most 256 rows,
-// at most CAP nonzeros unless a single row is longer).  Phase 1 streams the
-// block's column indices and values with non-temporal loads (they are read
-// once; x keeps the caches), gathers x and stores the products a_ij * x_j in
-// LDS -- 8 bytes per nonzero, so more workgroups fit on a CU than when value
-// and x are staged separately.  Phase 2 sums each row's products: one lane per
-// row in dense blocks, a group of G lanes with a fixed butterfly in sparse
-// ones.  The sum order differs from the host's sequential fma, so results
-// agree to rounding, deterministically; the bit-exact path is the row-block
-// kernel of lal_csrmv.hip.
+// at most 2048 nonzeros unless a single row is longer).  The main kernel
+// (16-byte aligned column and value arrays) copies the block's column indices
+// and values into LDS with 16-byte non-temporal loads, four nonzeros per lane,
+// and then lets every row -- one lane, or a group of G lanes with a fixed
+// butterfly in blocks of few long rows -- walk its own nonzeros in LDS and
+// gather x itself, eight (G = 1) or four gathers in flight per lane.  A gather
+// instruction then touches the same stencil offset of consecutive rows.
+// Unaligned arrays take the product-staging kernel: lanes gather x for
+// consecutive nonzeros and stage a_ij * x_j in LDS before the row sums.
+// Column indices and values are read once, so they are streamed
+// (non-temporal) and x keeps the caches.  The sum order differs from the
+// host's sequential fma, so results agree to rounding, deterministically; the
+// bit-exact path is the row-block kernel of lal_csrmv.hip.
 //
 // All level operations are one kernel with different gathers and epilogues:
 //   SPMV      y = alpha*(A x) + beta*y
@@ -39,6 +43,7 @@ namespace lal
     struct rv_args
     {
         int         m;
+        int         nnz;
         const int*  blocks; // analysis row blocks (nb + 1)
         int         nb;
         const int*  rp;

@@ -203,6 +203,71 @@ def diffusion_faces(nx):
             np.concatenate(sign))
 
 
+def _morton3(p, bits=10):
+    """Z-order key of points in [0,1)^3 on a 2^bits grid per axis."""
+    q = np.minimum((p * (1 << bits)).astype(np.int64), (1 << bits) - 1)
+    key = np.zeros(len(p), dtype=np.int64)
+    for b in range(bits):
+        for a in range(3):
+            key |= ((q[:, a] >> b) & 1) << (3 * b + a)
+    return key
+
+
+def knn_laplacian(n_points, k=12, seed=0, order="morton"):
+    """Weighted graph Laplacian of the symmetrised k-nearest-neighbour graph
+    of n_points uniform points in [0,1]^3, parameterised by one weight per
+    edge.  Returns (row_ptr, col_ind, m, edges, entry, edge, sign, diag_pos):
+    for edge weights w (w > 0) and a shift s > 0 the SPD matrix L(w) + s I has
+
+        val = zeros(nnz); val[entry] += sign * w[edge]; val[diag_pos] += s
+
+    i.e. a_pq = -w_e for the edge e = (p, q) and a_pp = s + sum of w_e over
+    the edges at p.  Columns are sorted within rows.  With order="morton"
+    the points are numbered along a Z-order curve, so graph neighbours get
+    nearby row numbers (the numbering a partitioner would produce); with
+    "random" they keep the sampling order.
+    """
+    from scipy.spatial import cKDTree
+
+    rng = np.random.default_rng(seed)
+    pts = rng.uniform(0.0, 1.0, (n_points, 3))
+    if order == "morton":
+        pts = pts[np.argsort(_morton3(pts), kind="stable")]
+    elif order != "random":
+        raise ValueError(order)
+    _, nb = cKDTree(pts).query(pts, k=k + 1)
+    i = np.repeat(np.arange(n_points, dtype=np.int64), k)
+    j = nb[:, 1:].reshape(-1).astype(np.int64)
+    keep = i != j
+    lo, hi = np.minimum(i[keep], j[keep]), np.maximum(i[keep], j[keep])
+    edges = np.unique(lo * n_points + hi)
+    ep, eq = edges // n_points, edges % n_points
+    ne = len(edges)
+    m = n_points
+    rows = np.concatenate([ep, eq, np.arange(m)])
+    cols = np.concatenate([eq, ep, np.arange(m)])
+    key = rows * m + cols
+    srt = np.argsort(key, kind="stable")
+    rows_s, cols_s = rows[srt], cols[srt]
+    row_ptr = np.zeros(m + 1, dtype=np.int64)
+    row_ptr[1:] = np.cumsum(np.bincount(rows_s, minlength=m))
+    pos = np.empty(len(srt), dtype=np.int64)
+    pos[srt] = np.arange(len(srt))
+    e_ids = np.arange(ne, dtype=np.int64)
+    dpos = pos[2 * ne:]
+    entry = np.concatenate([pos[:ne], pos[ne:2 * ne], dpos[ep], dpos[eq]])
+    edge = np.concatenate([e_ids, e_ids, e_ids, e_ids])
+    sign = np.concatenate([-np.ones(2 * ne), np.ones(2 * ne)])
+    return (row_ptr.astype(np.int32), cols_s.astype(np.int32), m,
+            np.stack([ep, eq], axis=1), entry, edge, sign, dpos)
+
+
+def assemble_knn(nnz, entry, edge, sign, diag_pos, w, shift=1e-2):
+    val = assemble_faces(nnz, entry, edge, sign, w)
+    val[diag_pos] += shift
+    return val
+
+
 def assemble_faces(nnz, entry, face, sign, k):
     """Values of A(k) for a diffusion_faces pattern (numpy)."""
     val = np.zeros(nnz)

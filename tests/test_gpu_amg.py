@@ -208,6 +208,28 @@ def test_vcycle_sweep_counts_bitwise(h, oracle, n1, n2):
     assert bits_equal(host(z), ho.precond(r, n1, n2, 0))
 
 
+@pytest.mark.parametrize("name,max_levels", [("3d16", 20), ("diff12", 20), ("2d128", 20),
+                                             ("3d8", 3), ("3d8", 2), ("2d128", 3),
+                                             ("diff12", 4)])
+def test_vcycle_fast_matches_oracle(h, oracle, name, max_levels):
+    """One fast-path V-cycle (1 + 1 Jacobi; the coarse levels in the
+    single-workgroup LDS tail where they fit; coarsest 1x1 or dense inverse)
+    against the oracle's V-cycle with its Cholesky coarsest solve: equal up
+    to summation order inside rows."""
+    rp, ci, v, m = grid(name)
+    A = CSR(rp, ci, v, m, m)
+    log(f"vcycle fast {name} max_levels={max_levels}")
+    H = _hier(h, A, max_levels=max_levels)
+    ho = oracle.saamg_setup(A, max_level=max_levels)
+    assert H.levels == ho.levels
+    r = np.random.default_rng(6).uniform(-1, 1, m)
+    R = dev(r)
+    z = empty(m, torch.float64)
+    H.precond(R, z, 1, 1)
+    zo = ho.precond(r, 1, 1, 1)
+    assert np.linalg.norm(host(z) - zo) <= 1e-10 * np.linalg.norm(zo)
+
+
 @pytest.mark.parametrize("name", ["2d128", "3d16", "diff12"])
 def test_pcg_matches_reference(h, golden, name):
     """FP64 SA-AMG-PCG: the GPU reductions sum in another order than the

@@ -95,6 +95,64 @@ def test_csrmv_f32(h, oracle):
         assert np.all(np.abs(got - ref) <= 1.2e-7 * (lens + 1) * absr + 1e-30), name
 
 
+def _fast_tol(A, x, alpha, beta, y0, oracle, u):
+    absr = oracle.csrmv(CSR(A.row_ptr, A.col_ind, np.abs(A.val), A.m, A.n),
+                        np.abs(x)) * abs(alpha) + abs(beta * y0)
+    return (np.diff(A.row_ptr) + 3) * u * absr + 1e-300
+
+
+@pytest.mark.parametrize("f32", [False, True])
+def test_csrmv_fast_tails_and_alignment(h, oracle, f32):
+    """The fast kernel loads four nonzeros per lane from 16-byte aligned
+    arrays: matrices whose nnz is not a multiple of four (and tiny ones) end in
+    a scalar tail; column / value arrays at an odd element offset take the
+    product-staging kernel.  Both must agree with the oracle to rounding."""
+    import torch
+
+    rng = np.random.default_rng(9)
+    cases = []
+    for nnz_target in (1, 2, 3, 5, 6, 7, 4097, 4098, 4099):
+        m = max(1, nnz_target // 3)
+        lens = np.zeros(m, np.int64)
+        lens[:] = nnz_target // m
+        lens[: nnz_target - lens.sum()] += 1
+        n = 50
+        rows = [np.sort(rng.choice(n, size=int(k), replace=False)) for k in lens]
+        rp = np.zeros(m + 1, np.int32)
+        rp[1:] = np.cumsum(lens)
+        cases.append(CSR(rp, np.concatenate(rows).astype(np.int32),
+                         rng.uniform(-1, 1, rp[-1]), m, n))
+    cases.append(irregular_csr(3))
+    vdt = torch.float32 if f32 else torch.float64
+    fn = "lal_csrmv_f32" if f32 else "lal_csrmv_f64"
+    u = 1.2e-7 if f32 else 1.12e-16
+    for A in cases:
+        for offset in (0, 1):
+            log(f"csrmv fast f32={f32} nnz={A.nnz} offset={offset}")
+            RP = dev(A.row_ptr, np.int32)
+            CIb = torch.zeros(A.nnz + 1, dtype=torch.int32, device="cuda")
+            Vb = torch.zeros(A.nnz + 1, dtype=vdt, device="cuda")
+            CIb[offset: offset + A.nnz] = dev(A.col_ind, np.int32)
+            Vb[offset: offset + A.nnz] = dev(A.val).to(vdt)
+            CI, V = CIb[offset:], Vb[offset:]
+            desc = lal.CsrDescr(h, A.m, A.n, A.nnz, RP, CI)
+            x = rng.uniform(-1, 1, A.n)
+            X = dev(x).to(vdt)
+            y = empty(A.m, vdt)
+            lal.call(fn, h.h, desc.d, A.m, A.n, A.nnz, 1.0, lal.ptr(RP), lal.ptr(CI),
+                     lal.ptr(V), lal.ptr(X), 0.0, lal.ptr(y))
+            Ar = A
+            xr = x
+            if f32:
+                Ar = CSR(A.row_ptr, A.col_ind, A.val.astype(np.float32).astype(np.float64),
+                         A.m, A.n)
+                xr = x.astype(np.float32).astype(np.float64)
+            ref = oracle.csrmv(Ar, xr)
+            got = host(y)[: A.m].astype(np.float64)
+            tol = _fast_tol(Ar, xr, 1.0, 0.0, np.zeros(A.m), oracle, u)
+            assert np.all(np.abs(got - ref) <= tol), (A.nnz, offset)
+
+
 def test_residual_and_diagonal(h, oracle):
     for name, A in matrices():
         if A.m != A.n:
