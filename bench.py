@@ -41,6 +41,11 @@ def parse():
                     help="only the standalone SpMV loop (for counter runs)")
     ap.add_argument("--mixed", choices=("vcycle", "ir"), default="vcycle",
                     help="mixed scheme reported as value")
+    ap.add_argument("--config", choices=("c2", "c1", "c3", "c4", "c5"), default="c2",
+                    help="c2 (default): 3D Poisson 128^3, the headline line; the others "
+                         "print one JSON line each for the remaining workloads: c1 2D "
+                         "Poisson 128^2 to 1e-8, c3 log-normal FV diffusion 160^3, c4 96^3 "
+                         "face-conductance gradient, c5 kNN graph Laplacian (2M points)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     return ap.parse_args()
@@ -116,6 +121,148 @@ def cpu_baseline(rp, ci, v, m, b, threads):
     }
 
 
+def timed_solves(H, B, X, kinds, steps, warmup, rel_tol):
+    """Median seconds and info of each solve kind ("fp64", "vcycle", "ir")
+    from x = 0, with the hierarchy resident."""
+    import torch
+
+    def run(kind):
+        X.zero_()
+        if kind == "fp64":
+            return H.pcg(B, X, rel_tol=rel_tol, abs_tol=1e-300, max_iter=1000)
+        if kind == "vcycle":
+            return H.pcg_mixed(B, X, rel_tol=rel_tol, abs_tol=1e-300, max_iter=1000)
+        return H.ir(B, X, rel_tol=rel_tol, abs_tol=1e-300, inner_rel_tol=1e-4,
+                    inner_max_iter=200, max_outer=30)
+
+    out = {}
+    for kind in kinds:
+        for _ in range(warmup):
+            run(kind)
+        torch.cuda.synchronize()
+        times = []
+        for _ in range(steps):
+            s0 = time.perf_counter()
+            info = run(kind)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - s0)
+        out[kind] = {"median_s": statistics.median(times), "iters": info.iterations,
+                     "outer": info.outer_iterations, "converged": bool(info.converged),
+                     "x": X.clone()}
+    return out
+
+
+def run_config(args):
+    """The workloads other than the headline 128^3 Poisson: one JSON line each
+    with FP64 and mixed solve times (and, for c4, the gradient)."""
+    import torch
+
+    from linalglib_amd import lal, problems
+    from linalglib_amd.torch_solve import saamg_solve
+
+    dev = torch.device("cuda:0")
+    h = lal.Handle()
+    cfg = args.config
+    rng = np.random.default_rng(2024)
+    if cfg == "c1":
+        rp, ci, v, m = problems.poisson2d(128)
+        b = problems.rhs_from_solution(rp, ci, v, problems.uniform(m, seed=1))
+        tol, name = 1e-8, "2D 5-point Poisson 128^2 to 1e-8"
+    elif cfg == "c3":
+        nx = 160
+        kappa = np.exp(1.5 * rng.standard_normal((nx, nx, nx)))
+        rp, ci, v, m = problems.diffusion3d(nx, cell_kappa=kappa)
+        b = rng.standard_normal(m)
+        tol, name = 1e-10, "3D FV diffusion 160^3, kappa = exp(N(0, 1.5^2)), to 1e-10"
+    elif cfg == "c5":
+        n = 2_000_000
+        rp, ci, m, E, entry, edge, sign, dpos = problems.knn_laplacian(n, seed=5)
+        w = rng.uniform(0.5, 2.0, len(E))
+        v = problems.assemble_knn(len(ci), entry, edge, sign, dpos, w)
+        b = rng.standard_normal(m)
+        tol, name = 1e-10, "kNN graph Laplacian + 1e-2 I, 2M points in [0,1]^3, 12-NN, to 1e-10"
+    else:
+        return run_gradient(args)
+    nnz = int(rp[-1])
+    RP, CI, V, B = (torch.from_numpy(a).to(dev) for a in (rp, ci, v, b))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    H = lal.Hierarchy(h, m, nnz, RP, CI, V, build_f32=1)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    X = torch.zeros(m, dtype=torch.float64, device=dev)
+    res = timed_solves(H, B, X, ("fp64", "vcycle", "ir"), args.steps, args.warmup, tol)
+    line = {"config": {"workload": name, "n": m, "nnz": nnz, "levels": H.levels},
+            "setup_s": round(setup_s, 4)}
+    for kind, r in res.items():
+        x = r.pop("x").cpu().numpy()
+        rr = b - problems.rhs_from_solution(rp, ci, v, x)
+        line[kind] = {"solve_s": round(r["median_s"], 6), "iterations": r["iters"],
+                      "outer": r["outer"], "converged": r["converged"],
+                      "rel_residual_inf": float(np.abs(rr).max() / np.abs(b).max())}
+    line["speedup_vcycle_vs_fp64"] = round(res["fp64"]["median_s"] / res["vcycle"]["median_s"], 3)
+    line["speedup_ir_vs_fp64"] = round(res["fp64"]["median_s"] / res["ir"]["median_s"], 3)
+    print(json.dumps(line), flush=True)
+
+
+def run_gradient(args):
+    """96^3 face-conductance diffusion: L = 0.5 ||x - x_t||^2, x = A(k)^-1 b;
+    forward and adjoint solves (FP64 and mixed) plus dL/dk, and a central
+    difference check of 64 seeded faces with the FP64 solver (h = 1e-6)."""
+    import torch
+
+    from linalglib_amd import problems
+    from linalglib_amd.torch_solve import saamg_solve
+
+    dev = torch.device("cuda:0")
+    nx = 96
+    rp, ci, m, nf, entry, face, sign = problems.diffusion_faces(nx)
+    rng = np.random.default_rng(96)
+    k = np.exp(rng.standard_normal(nf))
+    b = rng.standard_normal(m)
+    xt = rng.standard_normal(m)
+    RP, CI = torch.from_numpy(rp).to(dev), torch.from_numpy(ci).to(dev)
+    E, F, S = (torch.from_numpy(a).to(dev) for a in (entry, face, sign))
+    Bt, XT = torch.from_numpy(b).to(dev), torch.from_numpy(xt).to(dev)
+
+    def loss_grad(kk, mixed):
+        K = torch.tensor(kk, device=dev, requires_grad=True)
+        val = torch.zeros(len(ci), dtype=torch.float64, device=dev).index_add(0, E, S * K[F])
+        x = saamg_solve(val, Bt, RP, CI, mixed=mixed, tol=1e-10)
+        L = 0.5 * ((x - XT) ** 2).sum()
+        L.backward()
+        torch.cuda.synchronize()
+        return L.item(), K.grad
+
+    out = {"config": {"workload": "96^3 FV diffusion, k_f = exp(N(0,1)) per face, "
+                                  "L = 0.5 ||x - x_t||^2, dL/dk by adjoint + SDDMM",
+                      "n": m, "nnz": int(rp[-1]), "faces": nf}}
+    grads = {}
+    for mixed in (False, "vcycle"):
+        loss_grad(k, mixed)
+        times = []
+        for _ in range(max(1, args.steps)):
+            t0 = time.perf_counter()
+            L, g = loss_grad(k, mixed)
+            times.append(time.perf_counter() - t0)
+        grads[mixed] = g.cpu().numpy()
+        out["fp64" if mixed is False else "mixed"] = {
+            "forward_backward_s": round(statistics.median(times), 5), "loss": L}
+    g64, gmx = grads[False], grads["vcycle"]
+    out["mixed_grad_rel_diff"] = float(np.linalg.norm(gmx - g64) / np.linalg.norm(g64))
+    faces = rng.choice(nf, size=64, replace=False)
+    hstep = 1e-6
+    errs = []
+    for f in faces:
+        kp, km = k.copy(), k.copy()
+        kp[f] += hstep
+        km[f] -= hstep
+        fd = (loss_grad(kp, False)[0] - loss_grad(km, False)[0]) / (2 * hstep)
+        errs.append(abs(fd - g64[f]) / max(abs(g64[f]), 1e-3 * np.abs(g64).max()))
+    out["fd_max_rel_err_64_faces"] = float(max(errs))
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -124,6 +271,11 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import torch
+
+    if args.config != "c2":
+        sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+        run_config(args)
+        return
 
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from linalglib_amd import lal, problems
@@ -309,7 +461,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": "csrmv_kernel<double> (FP64 CSR SpMV, 3 rotating copies)",
+            "kernel": "quad_kernel<double, 0> (FP64 CSR SpMV, 3 rotating copies)",
             "bytes_per_launch": b64,
         },
         "cpu_baseline": cpu,

@@ -40,6 +40,9 @@ typedef struct lal_context* lal_handle;
 
 /* Human-readable text for a status code. */
 const char* lal_status_string(int status);
+/* Detail of the last failure on the calling thread (the reference prints
+ * its errors instead, common.cuh:58-97). */
+const char* lal_last_error(void);
 
 /* ------------------------------------------------------------------------ */
 /* context and memory -- replaces backend/device/device_memory.h:34-52        */

@@ -141,11 +141,17 @@ namespace
 namespace lal
 {
     // Greedy row packing: a block closes at 256 rows or when the next row
-    // would push it past CAP nonzeros; a row longer than CAP is a block of
-    // its own.
+    // would push it past `cap` nonzeros; a row longer than cap is a block of
+    // its own.  cap is 2048 (the kernels' LDS chunk) for large matrices; for
+    // small ones it shrinks, down to 256, so that a launch still has about
+    // 1024 workgroups: their time is one latency chain per workgroup, and
+    // more, shorter chains spread over all CUs finish sooner.
     int csrmv_analysis_host(const int* h_rp, int m, std::vector<int>& blocks)
     {
-        const int cap = spmv_cap<double>::value;
+        const long nnz = m > 0 ? h_rp[m] : 0;
+        int        cap = spmv_cap<double>::value;
+        while(cap > 256 && nnz / 1024 < cap)
+            cap >>= 1;
         blocks.clear();
         blocks.push_back(0);
         int r = 0;
