@@ -125,7 +125,7 @@ namespace
             LAL_REQUIRE(d->m == m && d->nnz == nnz);
         if(d && !d->exact)
         {
-            lal::rv_args a{m,    d->nnz, d->d_blocks, d->nblocks,     rp,           ci, v, x, nullptr,
+            lal::rv_args a{m,    d->nnz, d->by_nnz, d->d_blocks, d->nblocks,     rp,           ci, v, x, nullptr,
                            b,    y,           nullptr,        (double)alpha, (double)beta};
             return lal::rowvec_launch<T>(c, MODE == 0 ? lal::RV_SPMV : lal::RV_RESIDUAL, a);
         }
@@ -214,6 +214,7 @@ namespace lal
         d->m       = m;
         d->nnz     = nnz;
         d->exact   = exact;
+        d->by_nnz  = m != n;
         d->nblocks = (int)blocks.size() - 1;
         if(hipMalloc(&d->d_blocks, sizeof(int) * blocks.size()) != hipSuccess)
         {
@@ -224,7 +225,6 @@ namespace lal
                                hipMemcpyHostToDevice, c->stream));
         LAL_HIP(hipStreamSynchronize(c->stream));
         *out = d;
-        (void)n;
         return LAL_OK;
     }
 

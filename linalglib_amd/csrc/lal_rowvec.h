@@ -7,7 +7,10 @@
 // and then lets every row -- one lane, or a group of G lanes with a fixed
 // butterfly in blocks of few long rows -- walk its own nonzeros in LDS and
 // gather x itself, eight (G = 1) or four gathers in flight per lane.  A gather
-// instruction then touches the same stencil offset of consecutive rows.
+// instruction then touches the same stencil offset of consecutive rows.  For
+// the rectangular transfer operators (P: 4-7 nonzeros per row, R: their
+// transposes) the lanes instead gather x for the consecutive nonzeros they
+// loaded and stage the products, which wastes no lanes on short rows.
 // Unaligned arrays take the product-staging kernel: lanes gather x for
 // consecutive nonzeros and stage a_ij * x_j in LDS before the row sums.
 // Column indices and values are read once, so they are streamed
@@ -44,6 +47,7 @@ namespace lal
     {
         int         m;
         int         nnz;
+        int         by_nnz; // gather over consecutive nonzeros (transfer operators)
         const int*  blocks; // analysis row blocks (nb + 1)
         int         nb;
         const int*  rp;

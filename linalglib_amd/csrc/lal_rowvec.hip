@@ -141,6 +141,32 @@ namespace
         }
     }
 
+    // y (and r) of one row from its sum acc = (A u)_row, per operation.
+    template <typename T, int MODE>
+    __device__ __forceinline__ void row_epilogue(int row, T acc, const T* __restrict__ x,
+                                                 const T* __restrict__ d,
+                                                 const T* __restrict__ b, T alpha, T beta,
+                                                 T* __restrict__ y, T* __restrict__ r)
+    {
+        if(MODE == lal::RV_SPMV)
+            y[row] = beta == (T)0 ? alpha * acc : fma(alpha, acc, beta * y[row]);
+        else if(MODE == lal::RV_RESTRICT)
+        {
+            y[row] = acc;
+            r[row] = acc * d[row];
+        }
+        else if(MODE == lal::RV_RESIDUAL)
+            y[row] = b[row] - acc;
+        else if(MODE == lal::RV_JACOBI)
+            y[row] = x[row] + (b[row] - acc) * d[row];
+        else
+        {
+            const T bi = b[row];
+            y[row]     = bi * d[row];
+            r[row]     = bi - acc;
+        }
+    }
+
     // Row sum over the staged nonzeros [j, e) (stride g): UR column indices
     // and values come from LDS, then UR gathers are in flight at once.
     template <typename T, int MODE, int UR>
@@ -265,23 +291,77 @@ namespace
             acc = lal::group_sum(acc, g);
         if(!active || lig != 0)
             return;
-        if(MODE == lal::RV_SPMV)
-            y[row] = beta == (T)0 ? alpha * acc : fma(alpha, acc, beta * y[row]);
-        else if(MODE == lal::RV_RESTRICT)
+        row_epilogue<T, MODE>(row, acc, x, d, b, alpha, beta, y, r);
+    }
+
+    // Quad-staged kernel that gathers by nonzero (16-byte aligned ci and val):
+    // each lane gathers x for the four consecutive nonzeros it loaded and
+    // stores the products in LDS; the rows then sum their products.  Used for
+    // the rectangular transfer operators, whose short and uneven rows would
+    // leave lanes of the per-row gather idle.
+    template <typename T, int MODE>
+    __global__ void __launch_bounds__(SPMV_THREADS)
+        quad_nnz_kernel(int m, int nnz, const int* __restrict__ blocks, int nb,
+                        const int* __restrict__ rp, const int* __restrict__ ci,
+                        const T* __restrict__ val, const T* __restrict__ x,
+                        const T* __restrict__ d, const T* __restrict__ b, T alpha, T beta,
+                        T* __restrict__ y, T* __restrict__ r)
+    {
+        constexpr int WIN = 2048 + 4;
+        __shared__ __attribute__((aligned(16))) T s_p[WIN];
+
+        const int tid  = threadIdx.x;
+        const int tile = lal::xcd_tile(blockIdx.x, nb);
+        int       r0, r1;
+        lal::block_rows(blocks, m, tile, &r0, &r1);
+        const int  nrows  = r1 - r0;
+        const int  g      = lal::spmv_group(nrows, 0);
+        const int  lrow   = tid / g;
+        const int  lig    = tid & (g - 1);
+        const bool active = lrow < nrows;
+        const int  row    = r0 + lrow;
+        const int  base   = rp[r0];
+        const int  end    = rp[r1];
+        const int  rs     = active ? rp[row] : 0;
+        const int  re     = active ? rp[row + 1] : 0;
+        const int  nnz4   = nnz & ~3;
+
+        T acc = 0;
+        for(int ws = base & ~3; ws < end; ws += WIN)
         {
-            y[row] = acc;
-            r[row] = acc * d[row];
+            const int we   = min(ws + WIN, end);
+            const int vend = min((we + 3) & ~3, nnz4);
+            for(int q = ws + 4 * tid; q < vend; q += 4 * SPMV_THREADS)
+            {
+                const int4v cq = __builtin_nontemporal_load(reinterpret_cast<const int4v*>(ci + q));
+                quad<T>     vq;
+                vq.load(val + q);
+                T v4[4], x4[4];
+                vq.store(v4);
+#pragma unroll
+                for(int e = 0; e < 4; e++)
+                    x4[e] = gather<T, MODE>(x, d, b, cq[e]);
+#pragma unroll
+                for(int e = 0; e < 4; e++)
+                    s_p[q - ws + e] = v4[e] * x4[e];
+            }
+            for(int k = max(vend, ws) + tid; k < we; k += SPMV_THREADS)
+                s_p[k - ws] = val[k] * gather<T, MODE>(x, d, b, ci[k]);
+            __syncthreads();
+            if(active)
+            {
+                const int a = max(rs, ws) - ws + lig;
+                const int e = min(re, we) - ws;
+                for(int j = a; j < e; j += g)
+                    acc += s_p[j];
+            }
+            __syncthreads();
         }
-        else if(MODE == lal::RV_RESIDUAL)
-            y[row] = b[row] - acc;
-        else if(MODE == lal::RV_JACOBI)
-            y[row] = x[row] + (b[row] - acc) * d[row];
-        else
-        {
-            const T bi = b[row];
-            y[row]     = bi * d[row];
-            r[row]     = bi - acc;
-        }
+        if(g > 1)
+            acc = lal::group_sum(acc, g);
+        if(!active || lig != 0)
+            return;
+        row_epilogue<T, MODE>(row, acc, x, d, b, alpha, beta, y, r);
     }
 
     template <typename T, int MODE>
@@ -292,7 +372,11 @@ namespace
             return LAL_OK;
         const bool aligned = ((reinterpret_cast<uintptr_t>(a.ci) |
                                reinterpret_cast<uintptr_t>(a.val)) & 15) == 0;
-        if(aligned && a.blocks)
+        if(aligned && a.blocks && a.by_nnz)
+            quad_nnz_kernel<T, MODE><<<nb, SPMV_THREADS, 0, c->stream>>>(
+                a.m, a.nnz, a.blocks, nb, a.rp, a.ci, (const T*)a.val, (const T*)a.x,
+                (const T*)a.d, (const T*)a.b, (T)a.alpha, (T)a.beta, (T*)a.y, (T*)a.r);
+        else if(aligned && a.blocks)
             quad_kernel<T, MODE><<<nb, SPMV_THREADS, 0, c->stream>>>(
                 a.m, a.nnz, a.blocks, nb, a.rp, a.ci, (const T*)a.val, (const T*)a.x,
                 (const T*)a.d, (const T*)a.b, (T)a.alpha, (T)a.beta, (T*)a.y, (T*)a.r);

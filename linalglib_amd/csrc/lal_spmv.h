@@ -18,6 +18,8 @@ struct lal_csrmv_info
     int  nnz      = 0;
     int  nblocks  = 0;
     int  exact    = 0;       // 1: sequential row sums (bit-identical to the host)
+    int  by_nnz   = 0;       // fast path: lanes gather x over consecutive nonzeros
+                             // (rectangular transfer operators) instead of per row
     int* d_blocks = nullptr; // nblocks + 1 row offsets
 };
 
