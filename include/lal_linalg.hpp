@@ -22,8 +22,13 @@
 
 #include "lal.h"
 
+#include <algorithm>
 #include <cmath>
+#include <cstdint>
+#include <fstream>
+#include <iomanip>
 #include <iostream>
+#include <sstream>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -198,6 +203,137 @@ namespace lal_linalg
     }
 
     // -------------------------------------------------------------- csr_matrix
+    // Matrix Market coordinate files on the host (csr_matrix.cpp:523-794):
+    // "matrix coordinate real|integer general|symmetric" is read, 1-based
+    // entries become a CSR matrix with columns sorted inside rows, a symmetric
+    // file's (i, j), i != j, also yields (j, i), and repeated (i, j) entries
+    // are summed (in file order).  Other objects, formats, fields and
+    // symmetries are refused with a message on stderr, as in the reference.
+    namespace mtx
+    {
+        inline bool read(const std::string& filename, int& m, int& n, std::vector<int>& row_ptr,
+                         std::vector<int>& col_ind, std::vector<double>& val)
+        {
+            std::ifstream in(filename);
+            if(!in)
+            {
+                std::cout << "Error: Could not read file contents" << std::endl;
+                return false;
+            }
+            std::string line, tok;
+            std::getline(in, line);
+            std::istringstream h(line);
+            std::string        banner, object, format, field, symmetry;
+            h >> banner >> object >> format >> field >> symmetry;
+            if(object != "matrix")
+            {
+                std::cerr << "Error: Not a matrix market file." << std::endl;
+                return false;
+            }
+            if(format != "coordinate")
+            {
+                std::cerr << "Error: Only 'coordinate' format is supported (not 'array')."
+                          << std::endl;
+                return false;
+            }
+            if(field != "real" && field != "integer")
+            {
+                std::cerr << "Error: unsupported data type '" << field << "'." << std::endl;
+                return false;
+            }
+            if(symmetry != "general" && symmetry != "symmetric")
+            {
+                std::cerr << "Error: unsupported symmetry '" << symmetry << "'." << std::endl;
+                return false;
+            }
+            const bool sym = symmetry == "symmetric";
+            while(std::getline(in, line) && !line.empty() && line[0] == '%')
+            {
+            }
+            int64_t entries = 0;
+            {
+                std::istringstream d(line);
+                if(!(d >> m >> n >> entries))
+                {
+                    std::cerr << "Error: Failed to read matrix dimensions or number of non-zeros."
+                              << std::endl;
+                    return false;
+                }
+            }
+            struct entry
+            {
+                int    r, c;
+                double v;
+            };
+            std::vector<entry> t;
+            t.reserve(sym ? 2 * entries : entries);
+            for(int64_t k = 0; k < entries; k++)
+            {
+                if(!std::getline(in, line))
+                {
+                    std::cerr << "Error: Unexpected end of file while reading elements (expected "
+                              << entries << " elements)." << std::endl;
+                    return false;
+                }
+                std::istringstream e(line);
+                int    r, c;
+                double v;
+                if(!(e >> r >> c >> v) || r < 1 || c < 1 || r > m || c > n)
+                {
+                    std::cerr << "Error: malformed entry '" << line << "'." << std::endl;
+                    return false;
+                }
+                t.push_back({r - 1, c - 1, v});
+                if(sym && r != c)
+                    t.push_back({c - 1, r - 1, v});
+            }
+            std::stable_sort(t.begin(), t.end(), [](const entry& a, const entry& b) {
+                return a.r != b.r ? a.r < b.r : a.c < b.c;
+            });
+            row_ptr.assign(m + 1, 0);
+            col_ind.clear();
+            val.clear();
+            for(size_t k = 0; k < t.size(); k++)
+            {
+                if(!col_ind.empty() && k > 0 && t[k].r == t[k - 1].r && t[k].c == t[k - 1].c)
+                {
+                    val.back() += t[k].v;
+                    continue;
+                }
+                row_ptr[t[k].r + 1]++;
+                col_ind.push_back(t[k].c);
+                val.push_back(t[k].v);
+            }
+            for(int i = 0; i < m; i++)
+                row_ptr[i + 1] += row_ptr[i];
+            return true;
+        }
+
+        // Writes "real general" with 10 digits after the point, as the
+        // reference's writer does (csr_matrix.cpp:736-794).
+        inline bool write(const std::string& filename, int m, int n,
+                          const std::vector<int>& row_ptr, const std::vector<int>& col_ind,
+                          const std::vector<double>& val)
+        {
+            std::ofstream out(filename);
+            if(!out.is_open())
+            {
+                std::cerr << "Error: Could not open file " << filename << " for writing."
+                          << std::endl;
+                return false;
+            }
+            const int nnz = m > 0 ? row_ptr[m] : 0;
+            out << "%%MatrixMarket matrix coordinate real general\n";
+            out << "% Created by C++ CSR to Matrix Market writer\n";
+            out << m << " " << n << " " << nnz << "\n";
+            out << std::fixed << std::setprecision(10);
+            for(int i = 0; i < m; i++)
+                for(int j = row_ptr[i]; j < row_ptr[i + 1]; j++)
+                    out << (i + 1) << " " << (col_ind[j] + 1) << " " << val[j] << "\n";
+            return true;
+        }
+    }
+
     template <typename T>
     class csr_matrix;
 
@@ -322,6 +458,33 @@ namespace lal_linalg
                                         C.get_val()),
                   "csrgemm_solve");
         }
+        // csr_matrix::read_mtx / write_mtx (csr_matrix.cpp:523-794): parsed on
+        // the host, then the arrays move to the device.
+        bool read_mtx(const std::string& filename)
+        {
+            int                 m_ = 0, n_ = 0;
+            std::vector<int>    hrp, hci;
+            std::vector<double> hv;
+            if(!mtx::read(filename, m_, n_, hrp, hci, hv))
+                return false;
+            resize(m_, n_, (int)hci.size());
+            if(!hci.empty())
+            {
+                check(lal_memcpy_h2d(context(), ci.get_vec(), hci.data(), sizeof(int) * hci.size()),
+                      "read_mtx");
+                check(lal_memcpy_h2d(context(), v.get_vec(), hv.data(), sizeof(double) * hv.size()),
+                      "read_mtx");
+            }
+            check(lal_memcpy_h2d(context(), rp.get_vec(), hrp.data(), sizeof(int) * hrp.size()),
+                  "read_mtx");
+            return true;
+        }
+        bool write_mtx(const std::string& filename) const
+        {
+            return mtx::write(filename, m, n, rp.to_host(), nnz ? ci.to_host() : std::vector<int>(),
+                              nnz ? v.to_host() : std::vector<double>());
+        }
+
         // T = this^T  (csr_matrix.cpp:437-447)
         void transpose(csr_matrix& Tr) const
         {

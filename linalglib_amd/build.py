@@ -88,6 +88,19 @@ def build_cpp_api_test():
     return out
 
 
+def build_mtx_tool():
+    """Host-only Matrix Market driver for the CPU parity tests (g++ only)."""
+    src = os.path.join(ROOT, "tests", "cpp", "mtx_tool.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "mtx_tool")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(ROOT, 'include')}", src,
+           f"-L{LIB_DIR}", "-llal", "-Wl,-rpath,$ORIGIN/../../linalglib_amd/_lib", "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        print(r.stdout + r.stderr, file=sys.stderr)
+        raise RuntimeError("g++ failed on the Matrix Market tool")
+    return out
+
+
 def clean():
     shutil.rmtree(OBJ, ignore_errors=True)
 
@@ -95,3 +108,4 @@ def clean():
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
     print(build_cpp_api_test())
+    print(build_mtx_tool())

@@ -314,6 +314,10 @@ class Reference:
         L.ref_csr_dims.argtypes = [C.c_void_p, _ip, _ip, _ip]
         L.ref_csr_copy.argtypes = [C.c_void_p, _i32p, _i32p, _f64p]
         L.ref_csr_free.argtypes = [C.c_void_p]
+        L.ref_read_mtx.argtypes = [C.c_char_p]
+        L.ref_read_mtx.restype = C.c_void_p
+        L.ref_write_mtx.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, _i32p, _i32p, _f64p]
+        L.ref_write_mtx.restype = C.c_int
 
     @staticmethod
     def available():
@@ -362,6 +366,17 @@ class Reference:
 
     def transpose(self, A):
         return self._take(self.L.ref_transpose(A.m, A.n, A.nnz, A.row_ptr, A.col_ind, A.val))
+
+    def read_mtx(self, path):
+        """csr_matrix::read_mtx of the reference; None when it refuses the file."""
+        p = self.L.ref_read_mtx(str(path).encode())
+        return self._take(p) if p else None
+
+    def write_mtx(self, path, A):
+        ci = A.col_ind if A.nnz else np.zeros(1, np.int32)
+        v = A.val if A.nnz else np.zeros(1)
+        return bool(self.L.ref_write_mtx(str(path).encode(), A.m, A.n, A.nnz, A.row_ptr,
+                                         _i32(ci), _f64(v)))
 
     def saamg_setup(self, A, max_level=20):
         h = self.L.ref_saamg_setup(A.m, A.n, A.nnz, A.row_ptr, A.col_ind, A.val, max_level)

@@ -300,6 +300,30 @@ void* ref_transpose(int m, int n, int nnz, const int* rp, const int* ci, const d
     return T;
 }
 
+// csr_matrix::read_mtx into a new host matrix (nullptr when it refuses the file)
+void* ref_read_mtx(const char* filename)
+{
+    quiet q;
+    auto* A = new linalg::csr_matrix<double>();
+    if(!A->read_mtx(filename))
+    {
+        delete A;
+        return nullptr;
+    }
+    return A;
+}
+
+// csr_matrix::write_mtx of the given CSR arrays
+int ref_write_mtx(const char* filename, int m, int n, int nnz, const int* rp, const int* ci,
+                  const double* v)
+{
+    quiet q;
+    linalg::csr_matrix<double>* A  = make_csr(m, n, nnz, rp, ci, v);
+    const bool                  ok = A->write_mtx(filename);
+    delete A;
+    return ok ? 1 : 0;
+}
+
 void ref_csr_dims(void* p, int* m, int* n, int* nnz)
 {
     quiet q;

@@ -118,6 +118,18 @@ int main()
         vector<double> ones(m, 1.0);
         expect(std::fabs(dot_product(ones, ones) - m) == 0.0, "dot_product closed form");
     }
+    // Matrix Market round trip: write_mtx, read_mtx into a device matrix, SpMV
+    {
+        const char* path = "/tmp/lal_api_test_poisson.mtx";
+        expect(A.write_mtx(path), "write_mtx");
+        csr_matrix<double> B;
+        expect(B.read_mtx(path), "read_mtx");
+        expect(B.get_m() == m && B.get_n() == m && B.get_nnz() == nnz, "read_mtx dimensions");
+        vector<double> xd(xs), y(m);
+        B.multiply_by_vector(y, xd);
+        expect(inf_norm_diff(y.to_host(), hb) < 1e-12, "read_mtx matrix: multiply_by_vector");
+        std::remove(path);
+    }
     // transpose and SpGEMM against the host on A itself
     {
         csr_matrix<double> T, C;
