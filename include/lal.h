@@ -254,6 +254,20 @@ int lal_ir_solve(lal_handle h, lal_hierarchy hier, const double* b, double* x,
 int lal_pcg_solve_mixed(lal_handle h, lal_hierarchy hier, const double* b, double* x,
                         const lal_solve_options* opts, lal_solve_info* info);
 
+/* AMG as a stand-alone solver -- replaces linalg::amg_solve
+ * (iterative_solvers/amg/amg.cpp:516-571; enums of include/iterative_solvers/
+ * amg/amg.h:155-240).  cycle: 0 V, 1 W, 2 F; smoother: 0 Jacobi,
+ * 1 Gauss-Seidel, 2 symmetric Gauss-Seidel (a forward sweep, as the reference
+ * runs it), 3 SOR, 4 SSOR (omega = 0.5).  x is the initial guess on entry;
+ * cycles run until the inf-norm residual meets rel_tol / abs_tol or more
+ * than opts->max_iter cycles ran (iter_control::max_cycle);
+ * info->iterations is the reference's return value.  Gauss-Seidel-type sweeps
+ * run as level sets of the rows' dependency graph (built on first use per
+ * level), bit-identical to the sequential sweep; with a hierarchy built in
+ * exact mode the whole solve is bit-identical to the reference's. */
+int lal_amg_solve(lal_handle h, lal_hierarchy hier, const double* b, double* x, int n1, int n2,
+                  int cycle, int smoother, const lal_solve_options* opts, lal_solve_info* info);
+
 /* ------------------------------------------------------------------------ */
 /* Adjoint (no reference counterpart): for loss L(x) with x = A^-1 b and A   */
 /* SPD, dL/db = lambda with A lambda = g, and dL/dA_ij = -lambda_i x_j over  */

@@ -1,0 +1,168 @@
+// SA-AMG hierarchy as the device code sees it (shared by lal_hierarchy.hip
+// and lal_cycles.hip).
+#pragma once
+
+#include "lal_coarse.h"
+#include "lal_vcycle.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace lal
+{
+    struct csr_dev
+    {
+        int             m = 0, n = 0, nnz = 0;
+        int*            rp  = nullptr;
+        int*            ci  = nullptr;
+        double*         v   = nullptr;
+        float*          v32 = nullptr;
+        lal_csrmv_info* d   = nullptr;
+
+        void free_all()
+        {
+            (void)hipFree(rp);
+            (void)hipFree(ci);
+            (void)hipFree(v);
+            (void)hipFree(v32);
+            lal_csrmv_descr_destroy(d);
+            rp = ci = nullptr;
+            v       = nullptr;
+            v32     = nullptr;
+            d       = nullptr;
+        }
+    };
+
+    template <typename T>
+    struct work
+    {
+        T* x    = nullptr;
+        T* t    = nullptr;
+        T* b    = nullptr;
+        T* r    = nullptr;
+        T* diag = nullptr; // a_ii (exact path)
+        T* dinv = nullptr; // 1/a_ii (fast path)
+    };
+
+    // Level sets of the Gauss-Seidel sweeps of one operator: the rows of set k
+    // depend only on rows of sets < k, so a set is one launch and each row is
+    // updated in the order the sequential sweep would use (forward, and
+    // backward for the second half of SSOR).
+    struct gs_sched
+    {
+        int*             perm = nullptr; // rows grouped by set
+        std::vector<int> off;            // set k: perm[off[k] .. off[k+1])
+    };
+
+    struct level_t
+    {
+        csr_dev      A, P, R;
+        work<double> w64;
+        work<float>  w32;
+        gs_sched     gs_fwd, gs_bwd;
+    };
+
+    inline int alloc_bytes(void** p, size_t bytes)
+    {
+        *p = nullptr;
+        if(bytes == 0)
+            return LAL_OK;
+        LAL_HIP(hipMalloc(p, bytes));
+        return LAL_OK;
+    }
+
+    template <typename T>
+    int alloc_n(T** p, size_t n)
+    {
+        return alloc_bytes((void**)p, sizeof(T) * std::max<size_t>(n, 1));
+    }
+}
+
+struct lal_hierarchy_t
+{
+    lal_context*      c = nullptr;
+    lal_saamg_options opts{};
+    int               levels = 0;
+    lal::level_t      L[21];
+    // coarsest level
+    int     coarse_kind = 0; // 0 scalar, 1 Gauss-Seidel, 2 dense inverse
+    double* coarse_inv  = nullptr;
+    // PCG workspaces and device scalars
+    double* res = nullptr;
+    double* z   = nullptr;
+    double* p   = nullptr;
+    double* q   = nullptr;
+    float*  res32 = nullptr;
+    float*  z32   = nullptr;
+    float*  p32   = nullptr;
+    float*  q32   = nullptr;
+    float*  d32   = nullptr;
+    double* s     = nullptr; // LAL_SOLVER_SLOTS scalars
+    double* part  = nullptr; // 4 x PCG_BLOCKS reduction partials
+    // residual norms reach the host through mapped pinned memory: a value
+    // and a sequence number the host polls
+    double*   h_note     = nullptr;
+    double*   d_note     = nullptr;
+    unsigned* h_seq      = nullptr;
+    unsigned* d_seq      = nullptr;
+    unsigned  seq        = 0;
+    bool      has32      = false;
+    // coarse tail: levels tail_first..levels run in one workgroup out of LDS
+    // (fast path); one packed device image of their arrays per precision
+    int            tail_first = -1;
+    int*           tail_img64 = nullptr;
+    int*           tail_img32 = nullptr;
+    lal::tail_desc tail64{};
+    lal::tail_desc tail32{};
+
+    ~lal_hierarchy_t()
+    {
+        for(int l = 0; l <= 20; l++)
+        {
+            L[l].A.free_all();
+            L[l].P.free_all();
+            L[l].R.free_all();
+            for(double* ptr : {L[l].w64.x, L[l].w64.t, L[l].w64.b, L[l].w64.r, L[l].w64.diag,
+                               L[l].w64.dinv})
+                (void)hipFree(ptr);
+            for(float* ptr : {L[l].w32.x, L[l].w32.t, L[l].w32.b, L[l].w32.r, L[l].w32.diag,
+                              L[l].w32.dinv})
+                (void)hipFree(ptr);
+            (void)hipFree(L[l].gs_fwd.perm);
+            (void)hipFree(L[l].gs_bwd.perm);
+        }
+        for(void* ptr : {(void*)coarse_inv, (void*)res, (void*)z, (void*)p, (void*)q,
+                         (void*)res32, (void*)z32, (void*)p32, (void*)q32, (void*)d32, (void*)s,
+                         (void*)part, (void*)tail_img64, (void*)tail_img32})
+            (void)hipFree(ptr);
+        if(h_note)
+            (void)hipHostFree(h_note);
+    }
+
+    template <typename T>
+    lal::work<T>& w(int l);
+};
+
+template <>
+inline lal::work<double>& lal_hierarchy_t::w<double>(int l)
+{
+    return L[l].w64;
+}
+template <>
+inline lal::work<float>& lal_hierarchy_t::w<float>(int l)
+{
+    return L[l].w32;
+}
+
+
+namespace lal
+{
+    // iter_control::residual_converges (iter_control.cpp:38-78)
+    bool converges(double res, double init, double rel_tol, double abs_tol);
+    // Reduces PCG_BLOCKS partials (op 0 sum, 1 max) into the solver slot and
+    // hands the value to the host through the hierarchy's mapped note;
+    // returns the sequence number to wait for.
+    int post_scalar(lal_hierarchy_t* h, int op, const double* part, int slot, unsigned* seq);
+    // Waits for the note carrying `seq` (checks the stream now and then).
+    int wait_scalar(lal_hierarchy_t* h, unsigned seq, int slot, double* v);
+}

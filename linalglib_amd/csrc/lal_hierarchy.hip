@@ -11,8 +11,7 @@
 // Everything stays on the device.  Solver scalars (gamma, alpha, beta) are
 // formed on the device from per-block partials, so a PCG iteration hands one
 // 8-byte residual norm to the host for the convergence test and nothing else.
-#include "lal_coarse.h"
-#include "lal_vcycle.h"
+#include "lal_hier.h"
 
 #include <algorithm>
 #include <cmath>
@@ -39,139 +38,13 @@ namespace lal
                           const int* crp, int* cci, double* cv);
 }
 
-namespace
-{
-    struct csr_dev
-    {
-        int     m = 0, n = 0, nnz = 0;
-        int*    rp = nullptr;
-        int*    ci = nullptr;
-        double* v  = nullptr;
-        float*  v32 = nullptr;
-        lal_csrmv_info* d = nullptr;
-
-        void free_all()
-        {
-            (void)hipFree(rp);
-            (void)hipFree(ci);
-            (void)hipFree(v);
-            (void)hipFree(v32);
-            lal_csrmv_descr_destroy(d);
-            rp = ci = nullptr;
-            v       = nullptr;
-            v32     = nullptr;
-            d       = nullptr;
-        }
-    };
-
-    template <typename T>
-    struct work
-    {
-        T* x    = nullptr;
-        T* t    = nullptr;
-        T* b    = nullptr;
-        T* r    = nullptr;
-        T* diag = nullptr; // a_ii (exact path)
-        T* dinv = nullptr; // 1/a_ii (fast path)
-    };
-
-    struct level_t
-    {
-        csr_dev      A, P, R;
-        work<double> w64;
-        work<float>  w32;
-    };
-
-    int alloc_bytes(void** p, size_t bytes)
-    {
-        *p = nullptr;
-        if(bytes == 0)
-            return LAL_OK;
-        LAL_HIP(hipMalloc(p, bytes));
-        return LAL_OK;
-    }
-
-    template <typename T>
-    int alloc_n(T** p, size_t n)
-    {
-        return alloc_bytes((void**)p, sizeof(T) * std::max<size_t>(n, 1));
-    }
-
-}
-
-struct lal_hierarchy_t
-{
-    lal_context*      c = nullptr;
-    lal_saamg_options opts{};
-    int               levels = 0;
-    level_t           L[21];
-    // coarsest level
-    int     coarse_kind = 0; // 0 scalar, 1 Gauss-Seidel, 2 dense inverse
-    double* coarse_inv  = nullptr;
-    // PCG workspaces and device scalars
-    double* res = nullptr;
-    double* z   = nullptr;
-    double* p   = nullptr;
-    double* q   = nullptr;
-    float*  res32 = nullptr;
-    float*  z32   = nullptr;
-    float*  p32   = nullptr;
-    float*  q32   = nullptr;
-    float*  d32   = nullptr;
-    double* s     = nullptr; // LAL_SOLVER_SLOTS scalars
-    double* part  = nullptr; // 4 x PCG_BLOCKS reduction partials
-    // residual norms reach the host through mapped pinned memory: a value
-    // and a sequence number the host polls
-    double*   h_note     = nullptr;
-    double*   d_note     = nullptr;
-    unsigned* h_seq      = nullptr;
-    unsigned* d_seq      = nullptr;
-    unsigned  seq        = 0;
-    bool      has32      = false;
-    // coarse tail: levels tail_first..levels run in one workgroup out of LDS
-    // (fast path); one packed device image of their arrays per precision
-    int            tail_first = -1;
-    int*           tail_img64 = nullptr;
-    int*           tail_img32 = nullptr;
-    lal::tail_desc tail64{};
-    lal::tail_desc tail32{};
-
-    ~lal_hierarchy_t()
-    {
-        for(int l = 0; l <= 20; l++)
-        {
-            L[l].A.free_all();
-            L[l].P.free_all();
-            L[l].R.free_all();
-            for(double* ptr : {L[l].w64.x, L[l].w64.t, L[l].w64.b, L[l].w64.r, L[l].w64.diag,
-                               L[l].w64.dinv})
-                (void)hipFree(ptr);
-            for(float* ptr : {L[l].w32.x, L[l].w32.t, L[l].w32.b, L[l].w32.r, L[l].w32.diag,
-                              L[l].w32.dinv})
-                (void)hipFree(ptr);
-        }
-        for(void* ptr : {(void*)coarse_inv, (void*)res, (void*)z, (void*)p, (void*)q,
-                         (void*)res32, (void*)z32, (void*)p32, (void*)q32, (void*)d32, (void*)s,
-                         (void*)part, (void*)tail_img64, (void*)tail_img32})
-            (void)hipFree(ptr);
-        if(h_note)
-            (void)hipHostFree(h_note);
-    }
-
-    template <typename T>
-    work<T>& w(int l);
-};
-
-template <>
-work<double>& lal_hierarchy_t::w<double>(int l)
-{
-    return L[l].w64;
-}
-template <>
-work<float>& lal_hierarchy_t::w<float>(int l)
-{
-    return L[l].w32;
-}
+using lal::alloc_n;
+using lal::converges;
+using lal::post_scalar;
+using lal::wait_scalar;
+using lal::csr_dev;
+using lal::level_t;
+using lal::work;
 
 namespace
 {
@@ -711,55 +584,6 @@ namespace
         return vcycle<T>(h, 0, r, z, n1, n2, xinit, &out);
     }
 
-    bool converges(double res, double init, double rel_tol, double abs_tol)
-    {
-        if(std::isnan(res) || std::isnan(init))
-            return false;
-        if(res <= abs_tol)
-            return true;
-        if(res / init <= rel_tol)
-            return true;
-        return false;
-    }
-
-    // Reduces PCG_BLOCKS partials (op 0 sum, 1 max) into s[slot] and hands the
-    // value to the host through the mapped note; returns the sequence number
-    // to wait for.
-    int post_scalar(lal_hierarchy_t* h, int op, const double* part, int slot, unsigned* seq)
-    {
-        *seq = ++h->seq;
-        return lal::finish_partials(h->c, op, part, h->s + slot, h->d_note, h->d_seq, *seq);
-    }
-
-    // Waits until the note carries `seq`.  The host polls the mapped word
-    // and checks the stream now and then so that a failed kernel turns into
-    // an error instead of a hang.
-    int wait_scalar(lal_hierarchy_t* h, unsigned seq, int slot, double* v)
-    {
-        for(long spins = 1;; spins++)
-        {
-            if(__atomic_load_n(h->h_seq, __ATOMIC_ACQUIRE) == seq)
-            {
-                *v = *reinterpret_cast<volatile double*>(h->h_note);
-                return LAL_OK;
-            }
-            if((spins & 1023) == 0)
-            {
-                hipError_t e = hipStreamQuery(h->c->stream);
-                if(e == hipSuccess)
-                {
-                    if(__atomic_load_n(h->h_seq, __ATOMIC_ACQUIRE) == seq)
-                        continue;
-                    // the stream has drained: read the device copy instead
-                    LAL_HIP(hipMemcpy(v, h->s + slot, sizeof(double), hipMemcpyDeviceToHost));
-                    return LAL_OK;
-                }
-                if(e != hipErrorNotReady)
-                    LAL_HIP(e);
-            }
-        }
-    }
-
     // cg_solver::solve_precond on the device.  T = double uses the FP64
     // levels and workspaces, T = float the FP32 ones.
     template <typename T>
@@ -899,6 +723,59 @@ namespace
         }
         return LAL_OK;
     }
+}
+
+namespace lal
+{
+    bool converges(double res, double init, double rel_tol, double abs_tol)
+    {
+        if(std::isnan(res) || std::isnan(init))
+            return false;
+        if(res <= abs_tol)
+            return true;
+        if(res / init <= rel_tol)
+            return true;
+        return false;
+    }
+
+    // Reduces PCG_BLOCKS partials (op 0 sum, 1 max) into s[slot] and hands the
+    // value to the host through the mapped note; returns the sequence number
+    // to wait for.
+    int post_scalar(lal_hierarchy_t* h, int op, const double* part, int slot, unsigned* seq)
+    {
+        *seq = ++h->seq;
+        return lal::finish_partials(h->c, op, part, h->s + slot, h->d_note, h->d_seq, *seq);
+    }
+
+    // Waits until the note carries `seq`.  The host polls the mapped word
+    // and checks the stream now and then so that a failed kernel turns into
+    // an error instead of a hang.
+    int wait_scalar(lal_hierarchy_t* h, unsigned seq, int slot, double* v)
+    {
+        for(long spins = 1;; spins++)
+        {
+            if(__atomic_load_n(h->h_seq, __ATOMIC_ACQUIRE) == seq)
+            {
+                *v = *reinterpret_cast<volatile double*>(h->h_note);
+                return LAL_OK;
+            }
+            if((spins & 1023) == 0)
+            {
+                hipError_t e = hipStreamQuery(h->c->stream);
+                if(e == hipSuccess)
+                {
+                    if(__atomic_load_n(h->h_seq, __ATOMIC_ACQUIRE) == seq)
+                        continue;
+                    // the stream has drained: read the device copy instead
+                    LAL_HIP(hipMemcpy(v, h->s + slot, sizeof(double), hipMemcpyDeviceToHost));
+                    return LAL_OK;
+                }
+                if(e != hipErrorNotReady)
+                    LAL_HIP(e);
+            }
+        }
+    }
+
 }
 
 extern "C" void lal_saamg_default_options(lal_saamg_options* o)

@@ -67,6 +67,9 @@ namespace lal
                   T* x);
     template <typename T>
     int coarse_dense(lal_context* c, int n, const double* inv, const T* b, T* x);
+    // `sweeps` sequential Gauss-Seidel sweeps on x as given (one lane).
+    int coarse_gs_sweeps(lal_context* c, int n, const int* rp, const int* ci, const double* v,
+                         const double* b, double* x, int sweeps);
     // PCG_BLOCKS partials of <x, y> / max|x| (FP64 accumulation).
     template <typename TX, typename TY>
     int dot_partials(lal_context* c, int n, const TX* x, const TY* y, double* part);

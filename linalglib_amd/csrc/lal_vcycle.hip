@@ -143,12 +143,13 @@ namespace
     __global__ void __launch_bounds__(64)
         coarse_gs_kernel(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                          const T* __restrict__ val, const T* __restrict__ b, T* __restrict__ x,
-                         int sweeps)
+                         int sweeps, int zero)
     {
         if(threadIdx.x != 0)
             return;
-        for(int j = 0; j < n; j++)
-            x[j] = 0;
+        if(zero)
+            for(int j = 0; j < n; j++)
+                x[j] = 0;
         for(int it = 0; it < sweeps; it++)
             for(int j = 0; j < n; j++)
             {
@@ -469,7 +470,15 @@ namespace lal
     int coarse_gs(lal_context* c, int n, const int* rp, const int* ci, const T* v, const T* b,
                   T* x)
     {
-        coarse_gs_kernel<T><<<1, 64, 0, c->stream>>>(n, rp, ci, v, b, x, 1000);
+        coarse_gs_kernel<T><<<1, 64, 0, c->stream>>>(n, rp, ci, v, b, x, 1000, 1);
+        LAL_LAUNCHED();
+        return LAL_OK;
+    }
+
+    int coarse_gs_sweeps(lal_context* c, int n, const int* rp, const int* ci, const double* v,
+                         const double* b, double* x, int sweeps)
+    {
+        coarse_gs_kernel<double><<<1, 64, 0, c->stream>>>(n, rp, ci, v, b, x, sweeps, 0);
         LAL_LAUNCHED();
         return LAL_OK;
     }

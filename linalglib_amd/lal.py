@@ -104,6 +104,8 @@ _SIGS = {
     "lal_pcg_solve_f64": [vp, vp, vp, vp, C.POINTER(SolveOptions), C.POINTER(SolveInfo)],
     "lal_ir_solve": [vp, vp, vp, vp, C.POINTER(SolveOptions), C.POINTER(SolveInfo)],
     "lal_pcg_solve_mixed": [vp, vp, vp, vp, C.POINTER(SolveOptions), C.POINTER(SolveInfo)],
+    "lal_amg_solve": [vp, vp, vp, vp, i32, i32, i32, i32, C.POINTER(SolveOptions),
+                      C.POINTER(SolveInfo)],
     "lal_sddmm_f64": [vp, i32, i32, vp, vp, f64, vp, vp, vp],
     "lal_adjoint_solve": [vp, vp, vp, vp, i32, C.POINTER(SolveOptions), C.POINTER(SolveInfo)],
     "lal_status_string": [i32],
@@ -284,6 +286,17 @@ class Hierarchy:
         o = solve_options(**opts)
         info = SolveInfo()
         call("lal_ir_solve", self.handle.h, self.hp, ptr(b), ptr(x), C.byref(o), C.byref(info))
+        return info
+
+    CYCLES = {"v": 0, "w": 1, "f": 2}
+    SMOOTHERS = {"jacobi": 0, "gs": 1, "sgs": 2, "sor": 3, "ssor": 4}
+
+    def amg_solve(self, b, x, n1=1, n2=1, cycle="v", smoother="jacobi", **opts):
+        """AMG as a solver (x in place); max_iter caps the number of cycles."""
+        o = solve_options(**opts)
+        info = SolveInfo()
+        call("lal_amg_solve", self.handle.h, self.hp, ptr(b), ptr(x), n1, n2,
+             self.CYCLES[cycle], self.SMOOTHERS[smoother], C.byref(o), C.byref(info))
         return info
 
     def pcg_mixed(self, b, x, **opts):

@@ -788,6 +788,138 @@ void or_vcycle(const or_hierarchy* h, double* x, const double* b, int n1, int n2
     }
 }
 
+static int converges(double res, double init, double rel_tol, double abs_tol);
+
+/* Gauss-Seidel / SOR row update in CSR order (gauss_seidel.cpp:50-86,
+ * sor.cpp:43-83, ssor.cpp:43-111): sigma sums the off-diagonal products with
+ * separate multiply and add, a_jj is the (last) diagonal entry of the row. */
+static void gs_row(int j, const int* rp, const int* ci, const double* v, double* x,
+                   const double* b, int sor)
+{
+    double sigma = 0.0, ajj = 0.0;
+    for(int k = rp[j]; k < rp[j + 1]; k++)
+    {
+        int col = ci[k];
+        if(col != j)
+            sigma = sigma + v[k] * x[col];
+        else
+            ajj = v[k];
+    }
+    if(sor)
+        x[j] = x[j] + 0.5 * ((b[j] - sigma) / ajj - x[j]);
+    else
+        x[j] = (b[j] - sigma) / ajj;
+}
+
+/* apply_smoother, amg.cpp:110-142 (omega = 0.5f for SOR and SSOR). */
+void or_smooth(int m, const int* rp, const int* ci, const double* v, double* x,
+               const double* b, int smoother)
+{
+    if(smoother == 0)
+    {
+        double* xold = (double*)malloc(sizeof(double) * ((size_t)m + 1));
+        memcpy(xold, x, sizeof(double) * (size_t)m);
+        or_jacobi_iteration(m, rp, ci, v, x, xold, b);
+        free(xold);
+    }
+    else if(smoother == 1 || smoother == 2)
+    {
+        for(int j = 0; j < m; j++)
+            gs_row(j, rp, ci, v, x, b, 0);
+    }
+    else if(smoother == 3)
+    {
+        for(int j = 0; j < m; j++)
+            gs_row(j, rp, ci, v, x, b, 1);
+    }
+    else if(smoother == 4)
+    {
+        for(int j = 0; j < m; j++)
+            gs_row(j, rp, ci, v, x, b, 1);
+        for(int j = m - 1; j > -1; j--)
+            gs_row(j, rp, ci, v, x, b, 1);
+    }
+}
+
+/* vcycle / wcycle / fcycle, amg.cpp:144-513.  The V-cycle's coarsest solve is
+ * the 1x1 divide or 1000 Gauss-Seidel sweeps, the W- and F-cycles' 100 sweeps;
+ * x is used as given (callers zero the coarse correction before the first
+ * recursion, and the W/F second recursion continues from the first one's
+ * result, whose full value is added to x again, as the reference does). */
+void or_cycle(const or_hierarchy* h, double* x, const double* b, int n1, int n2, int n3,
+              int level, int cycle, int smoother)
+{
+    const or_csr* A = &h->A[level];
+    int           N = A->m;
+    if(level < h->total_levels)
+    {
+        const or_csr* R  = &h->R[level];
+        const or_csr* P  = &h->P[level];
+        int           Nc = h->A[level + 1].m;
+        for(int i = 0; i < n1; i++)
+            or_smooth(N, A->row_ptr, A->col_ind, A->val, x, b, smoother);
+        double* r    = (double*)calloc((size_t)N + 1, sizeof(double));
+        double* wres = (double*)calloc((size_t)Nc + 1, sizeof(double));
+        double* ec   = (double*)calloc((size_t)Nc + 1, sizeof(double));
+        or_residual(N, A->row_ptr, A->col_ind, A->val, x, b, r);
+        or_csrmv(R->m, R->n, R->nnz, 1.0, R->row_ptr, R->col_ind, R->val, r, 0.0, wres);
+        or_cycle(h, ec, wres, n1, n2, n3, level + 1, cycle, smoother);
+        or_csrmv(P->m, P->n, P->nnz, 1.0, P->row_ptr, P->col_ind, P->val, ec, 1.0, x);
+        for(int i = 0; i < n2; i++)
+            or_smooth(N, A->row_ptr, A->col_ind, A->val, x, b, smoother);
+        if(cycle != 0)
+        {
+            or_residual(N, A->row_ptr, A->col_ind, A->val, x, b, r);
+            or_csrmv(R->m, R->n, R->nnz, 1.0, R->row_ptr, R->col_ind, R->val, r, 0.0, wres);
+            or_cycle(h, ec, wres, n1, n2, n3, level + 1, cycle == 2 ? 0 : 1, smoother);
+            or_csrmv(P->m, P->n, P->nnz, 1.0, P->row_ptr, P->col_ind, P->val, ec, 1.0, x);
+            for(int i = 0; i < n3; i++)
+                or_smooth(N, A->row_ptr, A->col_ind, A->val, x, b, smoother);
+        }
+        free(r);
+        free(wres);
+        free(ec);
+    }
+    else if(cycle == 0 && N == 1)
+    {
+        x[0] = b[0] / A->val[0];
+    }
+    else
+    {
+        int sweeps = cycle == 0 ? 1000 : 100;
+        for(int it = 0; it < sweeps; it++)
+            for(int j = 0; j < N; j++)
+                gs_row(j, A->row_ptr, A->col_ind, A->val, x, b, 0);
+    }
+}
+
+/* amg.cpp:516-571 with iter_control.cpp:38-91; n3 = n2. */
+int or_amg_solve(const or_hierarchy* h, double* x, const double* b, int n1, int n2, int cycle,
+                 int smoother, double rel_tol, double abs_tol, int max_cycle,
+                 double* res_norm)
+{
+    const or_csr* A = &h->A[0];
+    int           n = A->m;
+    double*       r = (double*)calloc((size_t)n + 1, sizeof(double));
+    or_residual(n, A->row_ptr, A->col_ind, A->val, x, b, r);
+    double init  = or_norm_inf(n, r);
+    double rn    = init;
+    int    count = 0;
+    while(!(count > max_cycle))
+    {
+        or_cycle(h, x, b, n1, n2, n2, 0, cycle, smoother);
+        or_residual(n, A->row_ptr, A->col_ind, A->val, x, b, r);
+        rn = or_norm_inf(n, r);
+        if(converges(rn, init, rel_tol, abs_tol))
+            break;
+        count++;
+    }
+    free(r);
+    if(res_norm)
+        *res_norm = rn;
+    return count;
+}
+
 /* The reference's CPU composition of an AMG preconditioner: x.zeros() and
  * amg_solve(h, x, rhs, n1, n2, vcycle, jacobi, {max_cycle = 0}), which runs
  * exactly one V-cycle (amg.cpp:516-571, iter_control.cpp:88-91); amg_solve's

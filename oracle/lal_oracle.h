@@ -84,6 +84,20 @@ void or_jacobi_iteration(int m, const int* rp, const int* ci, const double* v, d
 void or_vcycle(const or_hierarchy* h, double* x, const double* b, int n1, int n2, int level,
                int coarse_solver);
 /* z = M^-1 r : z zeroed, one V-cycle with undamped Jacobi smoothing. */
+/* One smoothing step on A x = b (amg.cpp:110-142): smoother 0 Jacobi, 1 Gauss-Seidel,
+ * 2 "symmetric" Gauss-Seidel (a forward Gauss-Seidel sweep, as written), 3 SOR and
+ * 4 SSOR with omega = 0.5. */
+void or_smooth(int m, const int* rp, const int* ci, const double* v, double* x,
+               const double* b, int smoother);
+/* One V (0), W (1) or F (2) cycle on level `level` (amg.cpp:144-513), x in place;
+ * n3 is the W/F cycles' post-smoothing count. */
+void or_cycle(const or_hierarchy* h, double* x, const double* b, int n1, int n2, int n3,
+              int level, int cycle, int smoother);
+/* amg_solve (amg.cpp:516-571): cycles until the inf-norm residual converges or more
+ * than max_cycle cycles ran; returns the cycle count, *res_norm the last norm. */
+int or_amg_solve(const or_hierarchy* h, double* x, const double* b, int n1, int n2, int cycle,
+                 int smoother, double rel_tol, double abs_tol, int max_cycle,
+                 double* res_norm);
 void or_amg_precond(const or_hierarchy* h, const double* r, double* z, int n1, int n2,
                     int coarse_solver);
 /* Returns the iteration count as cg_solver::solve_precond does; res_norm gets the

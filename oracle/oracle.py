@@ -123,6 +123,10 @@ class Oracle:
                                    C.c_int, C.c_int, C.c_int, _dp]
         L.or_cg.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_double,
                             C.c_double, C.c_int, _dp]
+        L.or_smooth.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_int]
+        L.or_amg_solve.argtypes = [C.c_void_p, _f64p, _f64p, C.c_int, C.c_int, C.c_int,
+                                   C.c_int, C.c_double, C.c_double, C.c_int, _dp]
+        L.or_amg_solve.restype = C.c_int
 
     def set_threads(self, n):
         self.L.or_set_threads(int(n))
@@ -268,6 +272,16 @@ class OracleHierarchy:
         self.o.L.or_amg_precond(self.h, _f64(r), z, n1, n2, coarse_solver)
         return z
 
+    def amg_solve(self, b, x0=None, n1=1, n2=1, cycle=0, smoother=0, rel_tol=1e-8,
+                  abs_tol=1e-300, max_cycle=100):
+        """amg_solve: cycle 0 V / 1 W / 2 F, smoother 0 Jacobi / 1 GS / 2 SGS / 3 SOR /
+        4 SSOR; returns (x, cycles, last residual inf-norm)."""
+        x = np.zeros(len(b)) if x0 is None else _f64(x0).copy()
+        rn = C.c_double(0.0)
+        k = self.o.L.or_amg_solve(self.h, x, _f64(b), n1, n2, cycle, smoother, rel_tol,
+                                  abs_tol, max_cycle, C.byref(rn))
+        return x, k, rn.value
+
     def pcg(self, b, x0=None, rel_tol=1e-8, abs_tol=1e-300, max_iter=1000, n1=1, n2=1,
             coarse_solver=0):
         x = np.zeros(len(b)) if x0 is None else _f64(x0).copy()
@@ -314,6 +328,9 @@ class Reference:
         L.ref_csr_dims.argtypes = [C.c_void_p, _ip, _ip, _ip]
         L.ref_csr_copy.argtypes = [C.c_void_p, _i32p, _i32p, _f64p]
         L.ref_csr_free.argtypes = [C.c_void_p]
+        L.ref_amg_solve.argtypes = [C.c_void_p, _f64p, _f64p, C.c_int, C.c_int, C.c_int,
+                                    C.c_int, C.c_double, C.c_double, C.c_int]
+        L.ref_amg_solve.restype = C.c_int
         L.ref_read_mtx.argtypes = [C.c_char_p]
         L.ref_read_mtx.restype = C.c_void_p
         L.ref_write_mtx.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, _i32p, _i32p, _f64p]
@@ -419,6 +436,13 @@ class RefHierarchy:
         z = np.zeros(len(r))
         self.r.L.ref_amg_precond(self.h, _f64(r), z, n1, n2)
         return z
+
+    def amg_solve(self, b, x0=None, n1=1, n2=1, cycle=0, smoother=0, rel_tol=1e-8,
+                  abs_tol=1e-300, max_cycle=100):
+        x = np.zeros(len(b)) if x0 is None else _f64(x0).copy()
+        k = self.r.L.ref_amg_solve(self.h, x, _f64(b), n1, n2, cycle, smoother, rel_tol,
+                                   abs_tol, max_cycle)
+        return x, k
 
     def pcg(self, b, x0=None, rel_tol=1e-8, abs_tol=1e-300, max_iter=1000, n1=1, n2=1):
         x = np.zeros(len(b)) if x0 is None else _f64(x0).copy()

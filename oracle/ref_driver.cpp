@@ -193,6 +193,26 @@ int ref_pcg_saamg(void* h, double* x, const double* b, double rel_tol, double ab
     return it;
 }
 
+// amg_solve with the given cycle (0 V, 1 W, 2 F) and smoother (0 Jacobi,
+// 1 Gauss-Seidel, 2 symmetric Gauss-Seidel, 3 SOR, 4 SSOR); x in place.
+int ref_amg_solve(void* h, double* x, const double* b, int n1, int n2, int cycle, int smoother,
+                  double rel_tol, double abs_tol, int max_cycle)
+{
+    quiet q;
+    ref_hier*              rh = static_cast<ref_hier*>(h);
+    int                    n  = rh->h.A_cs[0].get_m();
+    linalg::vector<double> xv(std::vector<double>(x, x + n));
+    linalg::vector<double> bv(std::vector<double>(b, b + n));
+    linalg::iter_control   c;
+    c.rel_tol   = rel_tol;
+    c.abs_tol   = abs_tol;
+    c.max_cycle = max_cycle;
+    int k       = linalg::amg_solve(rh->h, xv, bv, n1, n2, static_cast<linalg::cycle>(cycle),
+                                    static_cast<linalg::smoother>(smoother), c);
+    std::memcpy(x, xv.get_vec(), sizeof(double) * n);
+    return k;
+}
+
 // Unpreconditioned cg_solver::solve.
 int ref_cg(int m, int nnz, const int* rp, const int* ci, const double* v, double* x,
            const double* b, double rel_tol, double abs_tol, int max_iter)

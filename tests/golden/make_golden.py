@@ -140,15 +140,40 @@ def ops_case(ref):
     print("ops written")
 
 
+def cycles_case(ref):
+    """amg_solve with every cycle (V, W, F) and smoother (Jacobi, GS, SGS, SOR,
+    SSOR) on the 2D 8x8 and 3D 8^3 Poisson grids: cycle counts and solutions."""
+    from oracle.oracle import CSR
+
+    out = {}
+    for name in ("2d8", "3d8"):
+        rp, ci, v, m = grid(name)
+        b = np.random.default_rng(1).uniform(-1, 1, m)
+        h = ref.saamg_setup(CSR(rp, ci, v, m, m))
+        out[f"{name}_b"] = b
+        for cyc in range(3):
+            for sm in range(5):
+                x, k = h.amg_solve(b, n1=1, n2=1, cycle=cyc, smoother=sm, rel_tol=1e-9,
+                                   max_cycle=40)
+                out[f"{name}_c{cyc}_s{sm}_x"] = x
+                out[f"{name}_c{cyc}_s{sm}_k"] = np.array([k])
+    np.savez_compressed(os.path.join(OUT, "golden_cycles.npz"), **out)
+    print("cycles written")
+
+
 def main():
     ref = Reference()
     ref.set_threads(1)
+    if len(sys.argv) > 1 and sys.argv[1] == "cycles":
+        cycles_case(ref)
+        return
     hierarchy_case(ref, "2d8", full=True)
     hierarchy_case(ref, "3d8", full=True)
     hierarchy_case(ref, "diff12", full=True)
     hierarchy_case(ref, "2d128", full=False)
     hierarchy_case(ref, "3d16", full=False)
     ops_case(ref)
+    cycles_case(ref)
 
 
 if __name__ == "__main__":

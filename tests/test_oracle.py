@@ -282,3 +282,67 @@ def test_oracle_equals_compiled_reference(oracle, reference, gen):
     xo, ito, _ = ho.pcg(b, rel_tol=1e-9)
     xr, itr = hr.pcg(b, rel_tol=1e-9)
     assert ito == itr and bits_equal(xo, xr)
+
+
+# ---------------------------------------------------------------------------
+# AMG as a solver: V/W/F cycles with the five smoothers (amg.cpp:110-571)
+
+def test_amg_solve_cycles_smoothers_match_reference_golden(oracle, golden):
+    """Cycle counts and solutions of amg_solve for every cycle x smoother on
+    two grids, bit for bit against the reference's outputs."""
+    g = golden("cycles")
+    for name in ("2d8", "3d8"):
+        rp, ci, v, m = grid(name)
+        h = oracle.saamg_setup(CSR(rp, ci, v, m, m))
+        b = g[f"{name}_b"]
+        for cyc in range(3):
+            for sm in range(5):
+                x, k, _ = h.amg_solve(b, n1=1, n2=1, cycle=cyc, smoother=sm, rel_tol=1e-9,
+                                      max_cycle=40)
+                assert k == int(g[f"{name}_c{cyc}_s{sm}_k"][0]), (name, cyc, sm)
+                assert bits_equal(x, g[f"{name}_c{cyc}_s{sm}_x"]), (name, cyc, sm)
+
+
+def test_amg_solve_matches_compiled_reference(oracle, reference):
+    """Same against the compiled reference on a diffusion operator, with other
+    sweep counts (n1 = 2, n2 = 1)."""
+    rp, ci, v, m = grid("diff12")
+    A = CSR(rp, ci, v, m, m)
+    b = np.random.default_rng(4).uniform(-1, 1, m)
+    ho, hr = oracle.saamg_setup(A), reference.saamg_setup(A)
+    for cyc in range(3):
+        for sm in range(5):
+            xo, ko, _ = ho.amg_solve(b, n1=2, n2=1, cycle=cyc, smoother=sm, rel_tol=1e-8,
+                                     max_cycle=25)
+            xr, kr = hr.amg_solve(b, n1=2, n2=1, cycle=cyc, smoother=sm, rel_tol=1e-8,
+                                  max_cycle=25)
+            assert ko == kr and bits_equal(xo, xr), (cyc, sm)
+
+
+def test_smoothers_closed_forms(oracle):
+    """On a diagonal matrix every smoother solves in one step (SOR/SSOR with
+    omega = 1/2 move halfway / three quarters of the way); Gauss-Seidel on a
+    lower-triangular system is forward substitution."""
+    m = 6
+    d = np.arange(1.0, m + 1)
+    rp = np.arange(m + 1, dtype=np.int32)
+    ci = np.arange(m, dtype=np.int32)
+    b = np.linspace(-1, 2, m)
+    for sm, expect in ((0, b / d), (1, b / d), (2, b / d), (3, 0.5 * b / d),
+                       (4, 0.75 * b / d)):
+        x = np.zeros(m)
+        oracle.L.or_smooth(m, rp, ci, d, x, b, sm)
+        assert np.allclose(x, expect, rtol=0, atol=1e-15), sm
+    # lower bidiagonal: a_ii = 2, a_i,i-1 = -1
+    rows, cols, vals = [], [], []
+    for i in range(m):
+        if i:
+            rows.append(i), cols.append(i - 1), vals.append(-1.0)
+        rows.append(i), cols.append(i), vals.append(2.0)
+    rp = np.searchsorted(rows, np.arange(m + 1)).astype(np.int32)
+    x = np.zeros(m)
+    oracle.L.or_smooth(m, rp, np.array(cols, np.int32), np.array(vals), x, b, 1)
+    ref = np.zeros(m)
+    for i in range(m):
+        ref[i] = (b[i] + (ref[i - 1] if i else 0.0)) / 2.0
+    assert np.allclose(x, ref, rtol=0, atol=1e-15)
