@@ -41,11 +41,12 @@ def parse():
                     help="only the standalone SpMV loop (for counter runs)")
     ap.add_argument("--mixed", choices=("vcycle", "ir"), default="vcycle",
                     help="mixed scheme reported as value")
-    ap.add_argument("--config", choices=("c2", "c1", "c3", "c4", "c5"), default="c2",
+    ap.add_argument("--config", choices=("c2", "c1", "c3", "c4", "c5", "amg"), default="c2",
                     help="c2 (default): 3D Poisson 128^3, the headline line; the others "
                          "print one JSON line each for the remaining workloads: c1 2D "
                          "Poisson 128^2 to 1e-8, c3 log-normal FV diffusion 160^3, c4 96^3 "
-                         "face-conductance gradient, c5 kNN graph Laplacian (2M points)")
+                         "face-conductance gradient, c5 kNN graph Laplacian (2M points), "
+                         "amg: AMG as a solver (V/W/F cycles, Jacobi/GS/SSOR) on 128^3")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     return ap.parse_args()
@@ -181,6 +182,8 @@ def run_config(args):
         v = problems.assemble_knn(len(ci), entry, edge, sign, dpos, w)
         b = rng.standard_normal(m)
         tol, name = 1e-10, "kNN graph Laplacian + 1e-2 I, 2M points in [0,1]^3, 12-NN, to 1e-10"
+    elif cfg == "amg":
+        return run_amg(args)
     else:
         return run_gradient(args)
     nnz = int(rp[-1])
@@ -202,6 +205,66 @@ def run_config(args):
                       "rel_residual_inf": float(np.abs(rr).max() / np.abs(b).max())}
     line["speedup_vcycle_vs_fp64"] = round(res["fp64"]["median_s"] / res["vcycle"]["median_s"], 3)
     line["speedup_ir_vs_fp64"] = round(res["fp64"]["median_s"] / res["ir"]["median_s"], 3)
+    print(json.dumps(line), flush=True)
+
+
+def run_amg(args):
+    """AMG as a stand-alone solver on 3D Poisson 128^3: seconds per cycle for
+    V/W/F cycles with Jacobi, Gauss-Seidel and SSOR smoothing (fast kernels;
+    Gauss-Seidel-type sweeps level-scheduled), cycles to 1e-8, and LinAlgLib's
+    host amg_solve per cycle on the same problem (one cycle, bounded sample)."""
+    import torch
+
+    from linalglib_amd import lal, problems
+
+    dev = torch.device("cuda:0")
+    h = lal.Handle()
+    rp, ci, v, m = problems.poisson3d(128)
+    b = problems.rhs_from_solution(rp, ci, v, problems.uniform(m, seed=1))
+    RP, CI, V, B = (torch.from_numpy(a).to(dev) for a in (rp, ci, v, b))
+    H = lal.Hierarchy(h, m, int(rp[-1]), RP, CI, V)
+    X = torch.zeros(m, dtype=torch.float64, device=dev)
+    line = {"config": {"workload": "AMG as a solver, 3D Poisson 128^3, 1 pre + 1 post sweep",
+                       "n": m, "nnz": int(rp[-1]), "levels": H.levels}, "gpu": {}}
+    for cyc in ("v", "w", "f"):
+        for sm in ("jacobi", "gs", "ssor"):
+            # first call builds the Gauss-Seidel level sets; then one timed cycle
+            X.zero_()
+            H.amg_solve(B, X, 1, 1, cyc, sm, rel_tol=1e-300, abs_tol=1e-300, max_iter=0)
+            torch.cuda.synchronize()
+            X.zero_()
+            t0 = time.perf_counter()
+            H.amg_solve(B, X, 1, 1, cyc, sm, rel_tol=1e-300, abs_tol=1e-300, max_iter=0)
+            torch.cuda.synchronize()
+            rec = {"s_per_cycle": round(time.perf_counter() - t0, 6)}
+            if cyc == "v":
+                X.zero_()
+                t0 = time.perf_counter()
+                info = H.amg_solve(B, X, 1, 1, cyc, sm, rel_tol=1e-8, abs_tol=1e-300,
+                                   max_iter=60)
+                torch.cuda.synchronize()
+                rec.update({"cycles_to_1e-8": info.iterations, "converged": bool(info.converged),
+                            "solve_s": round(time.perf_counter() - t0, 5)})
+            line["gpu"][f"{cyc}_{sm}"] = rec
+            print(f"# amg {cyc} {sm}: {rec}", file=sys.stderr, flush=True)
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+        from oracle.oracle import CSR, Oracle, Reference
+
+        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        lib = Reference() if Reference.available() else Oracle()
+        lib.set_threads(threads)
+        print("# amg: host setup and one cycle of each kind on the CPU", file=sys.stderr,
+              flush=True)
+        hc = lib.saamg_setup(CSR(rp, ci, v, m, m))
+        cpu = {}
+        for cyc, sm in ((0, 0), (0, 1)):
+            t0 = time.perf_counter()
+            hc.amg_solve(b, cycle=cyc, smoother=sm, rel_tol=1e-300, max_cycle=0)
+            cpu[f"{'vwf'[cyc]}_{['jacobi', 'gs'][sm]}"] = round(time.perf_counter() - t0, 4)
+        line["cpu_baseline"] = {"s_per_cycle": cpu, "cores": threads,
+                                "kind": "reference" if Reference.available() else "port",
+                                "sample": "one cycle of amg_solve on the same problem"}
     print(json.dumps(line), flush=True)
 
 

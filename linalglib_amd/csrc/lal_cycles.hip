@@ -28,28 +28,73 @@ namespace
     using lal::csr_dev;
     using lal::gs_sched;
 
-    // One Gauss-Seidel (sor = 0) or SOR (omega = 1/2) row update per lane for
-    // the rows perm[0 .. cnt): sigma sums the off-diagonal products with a
-    // separate multiply and add in CSR order, a_jj is the row's diagonal entry.
+    // One Gauss-Seidel (sor = 0) or SOR (omega = 1/2) update of row j: sigma
+    // sums the off-diagonal products with a separate multiply and add in CSR
+    // order, a_jj is the row's diagonal entry.
+    __device__ __forceinline__ void gs_row(int j, const int* __restrict__ rp,
+                                           const int* __restrict__ ci,
+                                           const double* __restrict__ v,
+                                           const double* __restrict__ b, double* x, int sor)
+    {
+        const int rs    = rp[j];
+        const int re    = rp[j + 1];
+        double    sigma = 0.0, ajj = 0.0;
+        // eight entries' loads in flight at a time; the sum stays in CSR order
+        for(int k0 = rs; k0 < re; k0 += 8)
+        {
+            int    col[8];
+            double vv[8], xx[8];
+#pragma unroll
+            for(int u = 0; u < 8; u++)
+            {
+                const int k = min(k0 + u, re - 1);
+                col[u]      = ci[k];
+                vv[u]       = v[k];
+            }
+#pragma unroll
+            for(int u = 0; u < 8; u++)
+                xx[u] = x[col[u]];
+#pragma unroll
+            for(int u = 0; u < 8; u++)
+            {
+                if(k0 + u >= re)
+                    break;
+                if(col[u] != j)
+                    sigma = sigma + vv[u] * xx[u];
+                else
+                    ajj = vv[u];
+            }
+        }
+        x[j] = sor ? x[j] + 0.5 * ((b[j] - sigma) / ajj - x[j]) : (b[j] - sigma) / ajj;
+    }
+
+    // The rows perm[0 .. cnt) of one level set, one lane each.
     __global__ void __launch_bounds__(256)
         gs_level_kernel(int cnt, const int* __restrict__ perm, const int* __restrict__ rp,
                         const int* __restrict__ ci, const double* __restrict__ v,
                         const double* __restrict__ b, double* x, int sor)
     {
         const int t = blockIdx.x * 256 + threadIdx.x;
-        if(t >= cnt)
-            return;
-        const int j     = perm[t];
-        double    sigma = 0.0, ajj = 0.0;
-        for(int k = rp[j]; k < rp[j + 1]; k++)
+        if(t < cnt)
+            gs_row(perm[t], rp, ci, v, b, x, sor);
+    }
+
+    // A whole sweep in one workgroup: the level sets in order, a barrier
+    // between them (the rows written in one set are read by later sets of the
+    // same workgroup), for operators whose sets are small.
+    __global__ void __launch_bounds__(1024)
+        gs_sweep_wg_kernel(int nsets, const int* __restrict__ off, const int* __restrict__ perm,
+                           const int* __restrict__ rp, const int* __restrict__ ci,
+                           const double* __restrict__ v, const double* __restrict__ b, double* x,
+                           int sor)
+    {
+        for(int s = 0; s < nsets; s++)
         {
-            const int col = ci[k];
-            if(col != j)
-                sigma = sigma + v[k] * x[col];
-            else
-                ajj = v[k];
+            const int e = off[s + 1];
+            for(int t = off[s] + (int)threadIdx.x; t < e; t += 1024)
+                gs_row(perm[t], rp, ci, v, b, x, sor);
+            __syncthreads();
         }
-        x[j] = sor ? x[j] + 0.5 * ((b[j] - sigma) / ajj - x[j]) : (b[j] - sigma) / ajj;
     }
 
     // Level sets of the forward (and backward) sweep from the host pattern:
@@ -108,6 +153,9 @@ namespace
             LAL_TRY(lal::alloc_n(&g.perm, perm.size()));
             LAL_HIP(hipMemcpyAsync(g.perm, perm.data(), sizeof(int) * perm.size(),
                                    hipMemcpyHostToDevice, c->stream));
+            LAL_TRY(lal::alloc_n(&g.off_d, g.off.size()));
+            LAL_HIP(hipMemcpyAsync(g.off_d, g.off.data(), sizeof(int) * g.off.size(),
+                                   hipMemcpyHostToDevice, c->stream));
         }
         LAL_HIP(hipStreamSynchronize(c->stream));
         return LAL_OK;
@@ -116,6 +164,14 @@ namespace
     int gs_sweep(lal_context* c, const csr_dev& A, const gs_sched& g, const double* b, double* x,
                  int sor)
     {
+        const int nsets = (int)g.off.size() - 1;
+        if(nsets > 0 && A.m / nsets < 1024)
+        {
+            gs_sweep_wg_kernel<<<1, 1024, 0, c->stream>>>(nsets, g.off_d, g.perm, A.rp, A.ci, A.v,
+                                                          b, x, sor);
+            LAL_LAUNCHED();
+            return LAL_OK;
+        }
         for(size_t k = 0; k + 1 < g.off.size(); k++)
         {
             const int cnt = g.off[k + 1] - g.off[k];

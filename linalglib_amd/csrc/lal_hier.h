@@ -50,8 +50,9 @@ namespace lal
     // backward for the second half of SSOR).
     struct gs_sched
     {
-        int*             perm = nullptr; // rows grouped by set
-        std::vector<int> off;            // set k: perm[off[k] .. off[k+1])
+        int*             perm  = nullptr; // rows grouped by set
+        std::vector<int> off;             // set k: perm[off[k] .. off[k+1])
+        int*             off_d = nullptr; // the same offsets on the device
     };
 
     struct level_t
@@ -128,8 +129,9 @@ struct lal_hierarchy_t
             for(float* ptr : {L[l].w32.x, L[l].w32.t, L[l].w32.b, L[l].w32.r, L[l].w32.diag,
                               L[l].w32.dinv})
                 (void)hipFree(ptr);
-            (void)hipFree(L[l].gs_fwd.perm);
-            (void)hipFree(L[l].gs_bwd.perm);
+            for(int* ptr : {L[l].gs_fwd.perm, L[l].gs_bwd.perm, L[l].gs_fwd.off_d,
+                            L[l].gs_bwd.off_d})
+                (void)hipFree(ptr);
         }
         for(void* ptr : {(void*)coarse_inv, (void*)res, (void*)z, (void*)p, (void*)q,
                          (void*)res32, (void*)z32, (void*)p32, (void*)q32, (void*)d32, (void*)s,
