@@ -247,6 +247,24 @@ def run_amg(args):
                             "solve_s": round(time.perf_counter() - t0, 5)})
             line["gpu"][f"{cyc}_{sm}"] = rec
             print(f"# amg {cyc} {sm}: {rec}", file=sys.stderr, flush=True)
+    # unsmoothed aggregation: setup and PCG with its V-cycle
+    del H
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    HU = lal.Hierarchy(h, m, int(rp[-1]), RP, CI, V, unsmoothed=True)
+    torch.cuda.synchronize()
+    ua_setup = time.perf_counter() - t0
+    X.zero_()
+    HU.pcg(B, X, rel_tol=1e-10, abs_tol=1e-300, max_iter=1000)
+    torch.cuda.synchronize()
+    X.zero_()
+    t0 = time.perf_counter()
+    info = HU.pcg(B, X, rel_tol=1e-10, abs_tol=1e-300, max_iter=1000)
+    torch.cuda.synchronize()
+    line["gpu"]["uaamg"] = {"setup_s": round(ua_setup, 4), "levels": HU.levels,
+                            "pcg_solve_s": round(time.perf_counter() - t0, 5),
+                            "pcg_iterations": info.iterations, "converged": bool(info.converged)}
+    print(f"# amg uaamg: {line['gpu']['uaamg']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
         from oracle.oracle import CSR, Oracle, Reference
@@ -256,13 +274,17 @@ def run_amg(args):
         lib.set_threads(threads)
         print("# amg: host setup and one cycle of each kind on the CPU", file=sys.stderr,
               flush=True)
+        t0 = time.perf_counter()
+        lib.uaamg_setup(CSR(rp, ci, v, m, m))
+        ua_cpu_setup = time.perf_counter() - t0
         hc = lib.saamg_setup(CSR(rp, ci, v, m, m))
         cpu = {}
         for cyc, sm in ((0, 0), (0, 1)):
             t0 = time.perf_counter()
             hc.amg_solve(b, cycle=cyc, smoother=sm, rel_tol=1e-300, max_cycle=0)
             cpu[f"{'vwf'[cyc]}_{['jacobi', 'gs'][sm]}"] = round(time.perf_counter() - t0, 4)
-        line["cpu_baseline"] = {"s_per_cycle": cpu, "cores": threads,
+        line["cpu_baseline"] = {"s_per_cycle": cpu, "uaamg_setup_s": round(ua_cpu_setup, 3),
+                                "cores": threads,
                                 "kind": "reference" if Reference.available() else "port",
                                 "sample": "one cycle of amg_solve on the same problem"}
     print(json.dumps(line), flush=True)

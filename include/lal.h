@@ -196,6 +196,15 @@ typedef struct
 } lal_saamg_options;
 
 void lal_saamg_default_options(lal_saamg_options* opts);
+/* uaamg_setup (iterative_solvers/amg/uaamg.cpp:125-224): the same level loop
+ * with the unsmoothed prolongator (one 1.0 per aggregated row, column =
+ * its aggregate; uaamg.cpp:48-123) and eps starting at 0.001 (the defaults
+ * of lal_uaamg_default_options; `relax` is unused).  The hierarchy serves
+ * lal_amg_solve, lal_amg_precond_f64 and the PCG entry points like a
+ * smoothed one. */
+void lal_uaamg_default_options(lal_saamg_options* opts);
+int  lal_uaamg_setup(lal_handle h, int m, int n, int nnz, const int* row_ptr, const int* col_ind,
+                     const double* val, const lal_saamg_options* opts, lal_hierarchy* hier);
 
 /* Builds the hierarchy on the device from a device-resident CSR matrix. The
  * hierarchy keeps its own copy of A. */

@@ -337,6 +337,61 @@ namespace lal
     }
 }
 
+namespace
+{
+    // Unsmoothed aggregation (uaamg.cpp:48-123): row i of P holds one entry,
+    // 1.0 in column agg[i], when the row was aggregated and nothing otherwise.
+    __global__ void __launch_bounds__(256)
+        ua_count_kernel(int m, const int64_t* __restrict__ agg, int* __restrict__ prp)
+    {
+        const int i = blockIdx.x * 256 + threadIdx.x;
+        if(i <= m)
+            prp[i] = (i < m && agg[i] >= 0) ? 1 : 0;
+    }
+
+    __global__ void __launch_bounds__(256)
+        ua_fill_kernel(int m, const int64_t* __restrict__ agg, const int* __restrict__ prp,
+                       int* __restrict__ pci, double* __restrict__ pv)
+    {
+        const int i = blockIdx.x * 256 + threadIdx.x;
+        if(i < m && agg[i] >= 0)
+        {
+            pci[prp[i]] = (int)agg[i];
+            pv[prp[i]]  = 1.0;
+        }
+    }
+}
+
+namespace lal
+{
+    int ua_prolongator_nnz(lal_context* c, int m, const int64_t* agg, int* prp, int* nnz_P,
+                           int* n_c)
+    {
+        ua_count_kernel<<<grid_for(m + 1, 256), 256, 0, c->stream>>>(m, agg, prp);
+        LAL_LAUNCHED();
+        LAL_TRY(scan_i32_dev(c, m + 1, prp));
+        int64_t mx = LLONG_MIN;
+        LAL_TRY(max_i64_dev(c, m, agg, &mx));
+        const int64_t nc = std::max<int64_t>(mx, -1) + 1;
+        if(nc > INT_MAX)
+            return LAL_ERR_UNSUPPORTED;
+        LAL_HIP(hipMemcpyAsync(nnz_P, prp + m, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        LAL_HIP(hipStreamSynchronize(c->stream));
+        *n_c = (int)nc;
+        return LAL_OK;
+    }
+
+    int ua_prolongator_fill(lal_context* c, int m, const int64_t* agg, const int* prp, int* pci,
+                            double* pv)
+    {
+        if(m == 0)
+            return LAL_OK;
+        ua_fill_kernel<<<grid_for(m, 256), 256, 0, c->stream>>>(m, agg, prp, pci, pv);
+        LAL_LAUNCHED();
+        return LAL_OK;
+    }
+}
+
 extern "C" int lal_sa_strength_f64(lal_handle c, int m, int nnz, const int* rp, const int* ci,
                                    const double* v, double eps, int* conn)
 {

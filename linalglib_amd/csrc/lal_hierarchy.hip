@@ -29,6 +29,10 @@ namespace lal
     int sa_prolongator_fill(lal_context* c, int m, const int* rp, const int* ci, const double* v,
                             const int* conn, const int64_t* agg, double relax, const int* prp,
                             int* pci, double* pv);
+    int ua_prolongator_nnz(lal_context* c, int m, const int64_t* agg, int* prp, int* nnz_P,
+                           int* n_c);
+    int ua_prolongator_fill(lal_context* c, int m, const int64_t* agg, const int* prp, int* pci,
+                            double* pv);
     int transpose_f64(lal_context* c, int m, int n, int nnz, const int* rp, const int* ci,
                       const double* v, int* trp, int* tci, double* tv);
     int csrgemm_nnz(lal_context* c, int m, int k, int n, const int* arp, const int* aci,
@@ -387,8 +391,12 @@ namespace
         return LAL_OK;
     }
 
+    // Level loop of saamg_setup (saamg.cpp:210-282) or, with unsmoothed set,
+    // uaamg_setup (uaamg.cpp:125-224): the same strength, PMIS and Galerkin
+    // steps around a different prolongator.
     int setup(lal_context* c, int m, int n, int nnz, const int* rp, const int* ci,
-              const double* v, const lal_saamg_options& o, lal_hierarchy_t* h)
+              const double* v, const lal_saamg_options& o, lal_hierarchy_t* h,
+              bool unsmoothed = false)
     {
         h->c    = c;
         h->opts = o;
@@ -425,7 +433,11 @@ namespace
             LAL_TRY(lal::pmis_aggregate(c, Af.m, Af.nnz, Af.rp, Af.ci, conn, agg));
             LAL_TRY(alloc_n(&P.rp, Af.m + 1));
             int nnzP = 0, nc = 0;
-            LAL_TRY(lal::sa_prolongator_nnz(c, Af.m, Af.rp, Af.ci, conn, agg, P.rp, &nnzP, &nc));
+            if(unsmoothed)
+                LAL_TRY(lal::ua_prolongator_nnz(c, Af.m, agg, P.rp, &nnzP, &nc));
+            else
+                LAL_TRY(lal::sa_prolongator_nnz(c, Af.m, Af.rp, Af.ci, conn, agg, P.rp, &nnzP,
+                                                &nc));
             if(nc == 0)
             {
                 (void)hipFree(conn);
@@ -438,8 +450,11 @@ namespace
             P.nnz = nnzP;
             LAL_TRY(alloc_n(&P.ci, nnzP));
             LAL_TRY(alloc_n(&P.v, nnzP));
-            LAL_TRY(lal::sa_prolongator_fill(c, Af.m, Af.rp, Af.ci, Af.v, conn, agg, o.relax,
-                                             P.rp, P.ci, P.v));
+            if(unsmoothed)
+                LAL_TRY(lal::ua_prolongator_fill(c, Af.m, agg, P.rp, P.ci, P.v));
+            else
+                LAL_TRY(lal::sa_prolongator_fill(c, Af.m, Af.rp, Af.ci, Af.v, conn, agg, o.relax,
+                                                 P.rp, P.ci, P.v));
             (void)hipStreamSynchronize(c->stream);
             (void)hipFree(conn);
             (void)hipFree(agg);
@@ -804,34 +819,60 @@ extern "C" void lal_solve_default_options(lal_solve_options* o)
     o->max_outer      = 30;
 }
 
+namespace
+{
+    int setup_entry(lal_handle c, int m, int n, int nnz, const int* rp, const int* ci,
+                    const double* v, const lal_saamg_options* opts, lal_hierarchy* out,
+                    bool unsmoothed)
+    {
+        LAL_REQUIRE(c != nullptr && out != nullptr && m >= 1 && n == m && nnz >= 0);
+        LAL_REQUIRE(rp != nullptr && (nnz == 0 || (ci && v)));
+        *out = nullptr;
+        lal_saamg_options o;
+        lal_saamg_default_options(&o);
+        if(unsmoothed)
+            o.eps = 0.001;
+        if(opts)
+            o = *opts;
+        LAL_REQUIRE(o.max_levels >= 0 && o.eps >= 0.0);
+        // validate the input pattern before any setup kernel gathers through it
+        {
+            lal_csrmv_info* d = nullptr;
+            LAL_TRY(lal::csrmv_analysis_dev(c, m, n, nnz, rp, ci, 0, &d));
+            lal_csrmv_descr_destroy(d);
+        }
+        auto* h = new lal_hierarchy_t;
+        int   s = setup(c, m, n, nnz, rp, ci, v, o, h, unsmoothed);
+        if(s != LAL_OK)
+        {
+            (void)hipStreamSynchronize(c->stream);
+            delete h;
+            return s;
+        }
+        *out = h;
+        return LAL_OK;
+    }
+}
+
+extern "C" void lal_uaamg_default_options(lal_saamg_options* o)
+{
+    lal_saamg_default_options(o);
+    if(o)
+        o->eps = 0.001;
+}
+
+extern "C" int lal_uaamg_setup(lal_handle c, int m, int n, int nnz, const int* rp,
+                               const int* ci, const double* v, const lal_saamg_options* opts,
+                               lal_hierarchy* out)
+{
+    return setup_entry(c, m, n, nnz, rp, ci, v, opts, out, true);
+}
+
 extern "C" int lal_saamg_setup(lal_handle c, int m, int n, int nnz, const int* rp,
                                const int* ci, const double* v, const lal_saamg_options* opts,
                                lal_hierarchy* out)
 {
-    LAL_REQUIRE(c != nullptr && out != nullptr && m >= 1 && n == m && nnz >= 0);
-    LAL_REQUIRE(rp != nullptr && (nnz == 0 || (ci && v)));
-    *out = nullptr;
-    lal_saamg_options o;
-    lal_saamg_default_options(&o);
-    if(opts)
-        o = *opts;
-    LAL_REQUIRE(o.max_levels >= 0 && o.eps >= 0.0);
-    // validate the input pattern before any setup kernel gathers through it
-    {
-        lal_csrmv_info* d = nullptr;
-        LAL_TRY(lal::csrmv_analysis_dev(c, m, n, nnz, rp, ci, 0, &d));
-        lal_csrmv_descr_destroy(d);
-    }
-    auto* h = new lal_hierarchy_t;
-    int   s = setup(c, m, n, nnz, rp, ci, v, o, h);
-    if(s != LAL_OK)
-    {
-        (void)hipStreamSynchronize(c->stream);
-        delete h;
-        return s;
-    }
-    *out = h;
-    return LAL_OK;
+    return setup_entry(c, m, n, nnz, rp, ci, v, opts, out, false);
 }
 
 extern "C" int lal_hierarchy_destroy(lal_hierarchy h)

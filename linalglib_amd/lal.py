@@ -95,6 +95,8 @@ _SIGS = {
     "lal_sa_prolongator_fill_f64": [vp, i32, i32, vp, vp, vp, vp, vp, f64, vp, vp, vp],
     "lal_saamg_default_options": [C.POINTER(SaamgOptions)],
     "lal_saamg_setup": [vp, i32, i32, i32, vp, vp, vp, C.POINTER(SaamgOptions), C.POINTER(vp)],
+    "lal_uaamg_setup": [vp, i32, i32, i32, vp, vp, vp, C.POINTER(SaamgOptions), C.POINTER(vp)],
+    "lal_uaamg_default_options": [C.POINTER(SaamgOptions)],
     "lal_hierarchy_destroy": [vp],
     "lal_hierarchy_levels": [vp, C.POINTER(i32)],
     "lal_hierarchy_level": [vp, i32, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
@@ -112,7 +114,8 @@ _SIGS = {
     "lal_last_error": [],
 }
 _RESTYPES = {"lal_status_string": C.c_char_p, "lal_last_error": C.c_char_p,
-             "lal_saamg_default_options": None, "lal_solve_default_options": None}
+             "lal_saamg_default_options": None,
+             "lal_uaamg_default_options": None, "lal_solve_default_options": None}
 
 
 def lib_path():
@@ -224,12 +227,17 @@ class Hierarchy:
 
     KIND = {"A": 0, "P": 1, "R": 2}
 
-    def __init__(self, handle, m, nnz, row_ptr, col_ind, val, **opts):
+    def __init__(self, handle, m, nnz, row_ptr, col_ind, val, unsmoothed=False, **opts):
+        """unsmoothed=True builds LinAlgLib's uaamg_setup hierarchy (eps from
+        0.001, one 1.0 per prolongator row) instead of saamg_setup's."""
         self.handle = handle
         self.opts = saamg_options(**opts)
+        if unsmoothed and "eps" not in opts:
+            self.opts.eps = 0.001
+        fn = "lal_uaamg_setup" if unsmoothed else "lal_saamg_setup"
         hp = vp()
-        check(load().lal_saamg_setup(handle.h, m, m, nnz, ptr(row_ptr), ptr(col_ind), ptr(val),
-                                     C.byref(self.opts), C.byref(hp)), "lal_saamg_setup")
+        check(getattr(load(), fn)(handle.h, m, m, nnz, ptr(row_ptr), ptr(col_ind), ptr(val),
+                                  C.byref(self.opts), C.byref(hp)), fn)
         self.hp = hp
         self.m = m
 

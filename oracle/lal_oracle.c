@@ -575,8 +575,52 @@ static void csr_copy(or_csr* dst, int m, int n, int nnz, const int* rp, const in
 /* saamg.cpp:210-282 (saamg_setup): eps = 0.08 halved per level, relax = 2/3,
  * stop when P has no columns or at min(max_level, 20) levels.  eps0 and relax
  * are parameters here so the same loop can run other values. */
+/* construct_prolongation_using_unsmoothed_aggregation, uaamg.cpp:48-123:
+ * n_c = max aggregate + 1; row i holds 1.0 in column agg[i] when agg[i] >= 0. */
+void or_ua_prolongator(int m, const int64_t* agg, or_csr* P)
+{
+    int64_t mx = -1;
+    for(int i = 0; i < m; i++)
+        if(mx < agg[i])
+            mx = agg[i];
+    int nnz = 0;
+    for(int i = 0; i < m; i++)
+        nnz += agg[i] >= 0;
+    csr_alloc(P, m, (int)(mx + 1), nnz);
+    int k          = 0;
+    P->row_ptr[0] = 0;
+    for(int i = 0; i < m; i++)
+    {
+        if(agg[i] >= 0)
+        {
+            P->col_ind[k] = (int)agg[i];
+            P->val[k]     = 1.0;
+            k++;
+        }
+        P->row_ptr[i + 1] = k;
+    }
+}
+
+static or_hierarchy* amg_setup(int m, int n, int nnz, const int* rp, const int* ci,
+                               const double* v, int max_level, double eps0, double relax,
+                               int unsmoothed);
+
 or_hierarchy* or_saamg_setup(int m, int n, int nnz, const int* rp, const int* ci,
                              const double* v, int max_level, double eps0, double relax)
+{
+    return amg_setup(m, n, nnz, rp, ci, v, max_level, eps0, relax, 0);
+}
+
+/* uaamg_setup, uaamg.cpp:125-224 (eps starts at 0.001 there). */
+or_hierarchy* or_uaamg_setup(int m, int n, int nnz, const int* rp, const int* ci,
+                             const double* v, int max_level, double eps0)
+{
+    return amg_setup(m, n, nnz, rp, ci, v, max_level, eps0, 0.0, 1);
+}
+
+static or_hierarchy* amg_setup(int m, int n, int nnz, const int* rp, const int* ci,
+                               const double* v, int max_level, double eps0, double relax,
+                               int unsmoothed)
 {
     or_hierarchy* h = (or_hierarchy*)calloc(1, sizeof(or_hierarchy));
     if(max_level > OR_MAX_LEVELS)
@@ -591,8 +635,11 @@ or_hierarchy* or_saamg_setup(int m, int n, int nnz, const int* rp, const int* ci
         int64_t* agg  = (int64_t*)calloc((size_t)Af->m + 1, sizeof(int64_t));
         or_strength(Af->m, Af->row_ptr, Af->col_ind, Af->val, eps, conn);
         or_aggregates_pmis(Af->m, Af->row_ptr, Af->col_ind, conn, agg, NULL);
-        or_prolongator(Af->m, Af->row_ptr, Af->col_ind, Af->val, conn, agg, relax,
-                       &h->P[level]);
+        if(unsmoothed)
+            or_ua_prolongator(Af->m, agg, &h->P[level]);
+        else
+            or_prolongator(Af->m, Af->row_ptr, Af->col_ind, Af->val, conn, agg, relax,
+                           &h->P[level]);
         free(conn);
         free(agg);
         if(h->P[level].n == 0)

@@ -71,6 +71,10 @@ typedef struct
 
 or_hierarchy* or_saamg_setup(int m, int n, int nnz, const int* rp, const int* ci,
                              const double* v, int max_level, double eps0, double relax);
+/* uaamg_setup (uaamg.cpp:125-224): unsmoothed prolongator, eps from eps0 (0.001 there). */
+or_hierarchy* or_uaamg_setup(int m, int n, int nnz, const int* rp, const int* ci,
+                             const double* v, int max_level, double eps0);
+void or_ua_prolongator(int m, const int64_t* agg, or_csr* P);
 void          or_hierarchy_free(or_hierarchy* h);
 int           or_hierarchy_levels(const or_hierarchy* h);
 /* kind: 0 = A, 1 = P, 2 = R.  Returns -1 when the level does not exist. */

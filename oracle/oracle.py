@@ -113,6 +113,9 @@ class Oracle:
         L.or_saamg_setup.argtypes = [C.c_int, C.c_int, C.c_int, _i32p, _i32p, _f64p, C.c_int,
                                      C.c_double, C.c_double]
         L.or_saamg_setup.restype = C.c_void_p
+        L.or_uaamg_setup.argtypes = [C.c_int, C.c_int, C.c_int, _i32p, _i32p, _f64p, C.c_int,
+                                     C.c_double]
+        L.or_uaamg_setup.restype = C.c_void_p
         L.or_hierarchy_free.argtypes = [C.c_void_p]
         L.or_hierarchy_levels.argtypes = [C.c_void_p]
         L.or_hierarchy_dims.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _ip, _ip]
@@ -227,6 +230,10 @@ class Oracle:
                                   eps0, relax)
         return OracleHierarchy(self, h)
 
+    def uaamg_setup(self, A, max_level=20, eps0=0.001):
+        h = self.L.or_uaamg_setup(A.m, A.n, A.nnz, A.row_ptr, A.col_ind, A.val, max_level, eps0)
+        return OracleHierarchy(self, h)
+
     def jacobi_iteration(self, A, xold, b):
         x = np.zeros(A.m)
         self.L.or_jacobi_iteration(A.m, A.row_ptr, A.col_ind, A.val, x, _f64(xold), _f64(b))
@@ -328,6 +335,8 @@ class Reference:
         L.ref_csr_dims.argtypes = [C.c_void_p, _ip, _ip, _ip]
         L.ref_csr_copy.argtypes = [C.c_void_p, _i32p, _i32p, _f64p]
         L.ref_csr_free.argtypes = [C.c_void_p]
+        L.ref_uaamg_setup.argtypes = [C.c_int, C.c_int, C.c_int, _i32p, _i32p, _f64p, C.c_int]
+        L.ref_uaamg_setup.restype = C.c_void_p
         L.ref_amg_solve.argtypes = [C.c_void_p, _f64p, _f64p, C.c_int, C.c_int, C.c_int,
                                     C.c_int, C.c_double, C.c_double, C.c_int]
         L.ref_amg_solve.restype = C.c_int
@@ -397,6 +406,10 @@ class Reference:
 
     def saamg_setup(self, A, max_level=20):
         h = self.L.ref_saamg_setup(A.m, A.n, A.nnz, A.row_ptr, A.col_ind, A.val, max_level)
+        return RefHierarchy(self, h)
+
+    def uaamg_setup(self, A, max_level=20):
+        h = self.L.ref_uaamg_setup(A.m, A.n, A.nnz, A.row_ptr, A.col_ind, A.val, max_level)
         return RefHierarchy(self, h)
 
     def cg(self, A, b, x0=None, rel_tol=1e-6, abs_tol=1e-6, max_iter=1000):

@@ -117,6 +117,17 @@ void* ref_saamg_setup(int m, int n, int nnz, const int* rp, const int* ci, const
     return r;
 }
 
+void* ref_uaamg_setup(int m, int n, int nnz, const int* rp, const int* ci, const double* v,
+                      int max_level)
+{
+    quiet q;
+    linalg::csr_matrix<double>* A = make_csr(m, n, nnz, rp, ci, v);
+    ref_hier*                   r = new ref_hier;
+    linalg::uaamg_setup(*A, max_level, r->h);
+    delete A;
+    return r;
+}
+
 void ref_hierarchy_free(void* h)
 {
     quiet q;

@@ -161,9 +161,36 @@ def cycles_case(ref):
     print("cycles written")
 
 
+def ua_case(ref):
+    """uaamg_setup hierarchies: full arrays for the 2D 8x8 and 3D 8^3 grids,
+    SHA-256 of every level for 2D 128^2 and 3D 16^3."""
+    from oracle.oracle import CSR
+
+    out = {}
+    for name, full in (("2d8", True), ("3d8", True), ("2d128", False), ("3d16", False)):
+        rp, ci, v, m = grid(name)
+        h = ref.uaamg_setup(CSR(rp, ci, v, m, m))
+        out[f"{name}_levels"] = np.array([h.levels])
+        for lvl in range(h.levels + 1):
+            for kind in ("A", "P", "R") if lvl < h.levels else ("A",):
+                M = h.get(lvl, kind)
+                key = f"{name}_{kind}{lvl}"
+                out[key + "_dims"] = np.array([M.m, M.n, M.nnz])
+                if full:
+                    out[key + "_rp"], out[key + "_ci"], out[key + "_v"] = M.row_ptr, M.col_ind, M.val
+                else:
+                    out[key + "_sha"] = np.frombuffer(
+                        (sha(M.row_ptr) + sha(M.col_ind) + sha(M.val)).encode(), dtype=np.uint8)
+    np.savez_compressed(os.path.join(OUT, "golden_ua.npz"), **out)
+    print("ua written")
+
+
 def main():
     ref = Reference()
     ref.set_threads(1)
+    if len(sys.argv) > 1 and sys.argv[1] == "ua":
+        ua_case(ref)
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "cycles":
         cycles_case(ref)
         return
@@ -174,6 +201,7 @@ def main():
     hierarchy_case(ref, "3d16", full=False)
     ops_case(ref)
     cycles_case(ref)
+    ua_case(ref)
 
 
 if __name__ == "__main__":

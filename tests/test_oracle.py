@@ -346,3 +346,54 @@ def test_smoothers_closed_forms(oracle):
     for i in range(m):
         ref[i] = (b[i] + (ref[i - 1] if i else 0.0)) / 2.0
     assert np.allclose(x, ref, rtol=0, atol=1e-15)
+
+
+# ---------------------------------------------------------------------------
+# Unsmoothed aggregation (uaamg.cpp)
+
+def _check_ua_hierarchy(g, name, full, get, levels):
+    assert levels == int(g[f"{name}_levels"][0]), name
+    for lvl in range(levels + 1):
+        for kind in ("A", "P", "R") if lvl < levels else ("A",):
+            M = get(lvl, kind)
+            key = f"{name}_{kind}{lvl}"
+            assert np.array_equal([M.m, M.n, M.nnz], g[key + "_dims"]), key
+            if full:
+                assert np.array_equal(M.row_ptr, g[key + "_rp"]), key
+                assert np.array_equal(M.col_ind, g[key + "_ci"]), key
+                assert bits_equal(M.val, g[key + "_v"]), key
+            else:
+                h = (hashlib.sha256(M.row_ptr.tobytes()).hexdigest()
+                     + hashlib.sha256(M.col_ind.tobytes()).hexdigest()
+                     + hashlib.sha256(M.val.tobytes()).hexdigest())
+                assert h.encode() == g[key + "_sha"].tobytes(), key
+
+
+@pytest.mark.parametrize("name,full", [("2d8", True), ("3d8", True), ("2d128", False),
+                                       ("3d16", False)])
+def test_uaamg_matches_reference_golden(oracle, golden, name, full):
+    g = golden("ua")
+    rp, ci, v, m = grid(name)
+    h = oracle.uaamg_setup(CSR(rp, ci, v, m, m))
+    _check_ua_hierarchy(g, name, full, h.get, h.levels)
+
+
+def test_uaamg_closed_form(oracle):
+    """Every P row holds at most one 1.0; R = P^T; A_c = P^T A P, i.e. the
+    coarse entries are sums of fine entries over aggregate pairs."""
+    rp, ci, v, m = grid("3d8")
+    A = CSR(rp, ci, v, m, m)
+    h = oracle.uaamg_setup(A)
+    P = h.get(0, "P")
+    assert np.all(np.diff(P.row_ptr) <= 1) and np.all(P.val == 1.0)
+    agg = np.full(m, -1)
+    rows = np.repeat(np.arange(m), np.diff(P.row_ptr))
+    agg[rows] = P.col_ind
+    Ac = h.get(1, "A")
+    dense = np.zeros((Ac.m, Ac.m))
+    r = np.repeat(np.arange(m), np.diff(rp))
+    keep = (agg[r] >= 0) & (agg[ci] >= 0)
+    np.add.at(dense, (agg[r][keep], agg[ci][keep]), v[keep])
+    got = np.zeros((Ac.m, Ac.m))
+    got[np.repeat(np.arange(Ac.m), np.diff(Ac.row_ptr)), Ac.col_ind] = Ac.val
+    assert np.allclose(got, dense, rtol=0, atol=1e-12)
