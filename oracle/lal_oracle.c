@@ -967,6 +967,212 @@ int or_amg_solve(const or_hierarchy* h, double* x, const double* b, int n1, int 
     return count;
 }
 
+/* Preconditioner application for the Krylov restatements: kind 0 none
+ * (z = r), 1 Jacobi (jacobi_precond: z = r / a_ii with the first diagonal
+ * match, preconditioner.cpp:41-58 and host_math.cpp:53-64), 2 one SA-AMG
+ * V-cycle of h (1 + 1 Jacobi sweeps, the reference's GS coarse solve). */
+static void precond_apply(int kind, const or_hierarchy* h, const double* diag, int n,
+                          const double* r, double* z)
+{
+    if(kind == 1)
+        for(int i = 0; i < n; i++)
+            z[i] = r[i] / diag[i];
+    else if(kind == 2)
+    {
+        memset(z, 0, sizeof(double) * (size_t)n);
+        or_vcycle(h, z, r, 1, 1, 0, 0);
+    }
+    else
+        memcpy(z, r, sizeof(double) * (size_t)n);
+}
+
+static void axpbypgz(int n, double a, const double* x, double b, const double* y, double g,
+                     double* z)
+{
+    for(int i = 0; i < n; i++)
+        z[i] = a * x[i] + b * y[i] + g * z[i];
+}
+
+/* bicgstab_solver::solve_nonprecond / solve_precond (krylov/bicgstab.cpp:64-231),
+ * including the breakdown branch's x += alpha p; returns the iteration count. */
+int or_bicgstab(int m, const int* rp, const int* ci, const double* v, double* x,
+                const double* b, int kind, const or_hierarchy* h, double rel_tol,
+                double abs_tol, int max_iter)
+{
+    double* r  = (double*)calloc((size_t)m + 1, sizeof(double));
+    double* r0 = (double*)calloc((size_t)m + 1, sizeof(double));
+    double* p  = (double*)calloc((size_t)m + 1, sizeof(double));
+    double* vv = (double*)calloc((size_t)m + 1, sizeof(double));
+    double* t  = (double*)calloc((size_t)m + 1, sizeof(double));
+    double* z  = (double*)calloc((size_t)m + 1, sizeof(double));
+    double* q  = (double*)calloc((size_t)m + 1, sizeof(double));
+    double* d  = (double*)calloc((size_t)m + 1, sizeof(double));
+    if(kind == 1)
+        or_diagonal(m, rp, ci, v, d);
+    or_residual(m, rp, ci, v, x, b, r);
+    double init = or_norm_inf(m, r);
+    memcpy(r0, r, sizeof(double) * (size_t)m);
+    double rho = or_dot(m, r0, r);
+    memcpy(p, r, sizeof(double) * (size_t)m);
+    const int pre = kind != 0;
+    if(pre)
+        precond_apply(kind, h, d, m, r, z);
+    int iter = 0;
+    while(!(iter > max_iter))
+    {
+        const double* dir = pre ? z : p;
+        or_csrmv(m, m, rp[m], 1.0, rp, ci, v, dir, 0.0, pre ? q : vv);
+        double alpha = rho / or_dot(m, r0, pre ? q : vv);
+        or_axpy(m, -1.0 * alpha, pre ? q : vv, r);
+        if(pre)
+            precond_apply(kind, h, d, m, r, vv);
+        or_csrmv(m, m, rp[m], 1.0, rp, ci, v, pre ? vv : r, 0.0, t);
+        double omega1 = or_dot(m, t, r);
+        double omega2 = or_dot(m, t, t);
+        if(omega1 == 0.0 || omega2 == 0.0)
+        {
+            or_axpy(m, alpha, p, x);
+            break;
+        }
+        double omega = omega1 / omega2;
+        axpbypgz(m, alpha, dir, omega, pre ? vv : r, 1.0, x);
+        or_axpy(m, -1.0 * omega, t, r);
+        double rn = or_norm_inf(m, r);
+        if(converges(rn, init, rel_tol, abs_tol))
+            break;
+        double rho_prev = rho;
+        rho             = or_dot(m, r0, r);
+        double beta     = (rho / rho_prev) * (alpha / omega);
+        axpbypgz(m, 1.0, r, -1.0 * beta * omega, pre ? q : vv, beta, p);
+        if(pre)
+            precond_apply(kind, h, d, m, p, z);
+        iter++;
+    }
+    free(r), free(r0), free(p), free(vv), free(t), free(z), free(q), free(d);
+    return iter;
+}
+
+/* Restarted, left-preconditioned GMRES in the form of krylov/gmres.cpp:46-530
+ * (modified Gram-Schmidt Arnoldi, Givens rotations, convergence on the
+ * preconditioned residual 2-norm, restart cycles counted as iterations), with
+ * the preconditioned Arnoldi vector q_k = M^-1 A q_(k-1) kept (gmres.cpp
+ * computes it into a temporary it then drops) and every work vector sized
+ * (gmres.cpp's build() sizes neither res nor z). */
+int or_gmres(int m, const int* rp, const int* ci, const double* v, double* x, const double* b,
+             int restart, int kind, const or_hierarchy* h, double rel_tol, double abs_tol,
+             int max_iter)
+{
+    if(restart > m - 1)
+        restart = m - 1;
+    if(restart < 1)
+        restart = 1;
+    const int ld  = restart + 1;
+    double*   Q   = (double*)calloc((size_t)m * ld + 1, sizeof(double));
+    double*   H   = (double*)calloc((size_t)ld * restart + 1, sizeof(double));
+    double*   cs  = (double*)calloc((size_t)restart + 1, sizeof(double));
+    double*   sn  = (double*)calloc((size_t)restart + 1, sizeof(double));
+    double*   res = (double*)calloc((size_t)m + ld + 1, sizeof(double));
+    double*   z   = (double*)calloc((size_t)m + 1, sizeof(double));
+    double*   w   = (double*)calloc((size_t)m + 1, sizeof(double));
+    double*   d   = (double*)calloc((size_t)m + 1, sizeof(double));
+    double*   y   = (double*)calloc((size_t)ld + 1, sizeof(double));
+    if(kind == 1)
+        or_diagonal(m, rp, ci, v, d);
+    or_residual(m, rp, ci, v, x, b, res);
+    precond_apply(kind, h, d, m, res, z);
+    double rn   = sqrt(or_dot(m, z, z));
+    double init = rn;
+    int    iter = 0;
+    if(!converges(rn, init, rel_tol, abs_tol))
+    {
+        for(int i = 0; i < m; i++)
+            Q[i] = z[i] / rn;
+        memset(y, 0, sizeof(double) * (size_t)ld);
+        y[0] = rn;
+        while(!(iter > max_iter))
+        {
+            int k = 1;
+            for(k = 1; k < restart + 1; k++)
+            {
+                const double* qkm1 = Q + (size_t)(k - 1) * m;
+                double*       qk   = Q + (size_t)k * m;
+                or_csrmv(m, m, rp[m], 1.0, rp, ci, v, qkm1, 0.0, w);
+                precond_apply(kind, h, d, m, w, qk);
+                for(int i = 0; i < k; i++)
+                {
+                    const double* qi  = Q + (size_t)i * m;
+                    double        hik = or_dot(m, qi, qk);
+                    H[i + (k - 1) * ld] = hik;
+                    for(int j = 0; j < m; j++)
+                        qk[j] = qk[j] - hik * qi[j];
+                }
+                H[k + (k - 1) * ld] = sqrt(or_dot(m, qk, qk));
+                for(int j = 0; j < m; j++)
+                    qk[j] = qk[j] / H[k + (k - 1) * ld];
+                for(int i = 0; i < k - 1; i++)
+                {
+                    double t1 = H[i + (k - 1) * ld], t2 = H[i + 1 + (k - 1) * ld];
+                    H[i + (k - 1) * ld]     = cs[i] * t1 + sn[i] * t2;
+                    H[i + 1 + (k - 1) * ld] = -sn[i] * t1 + cs[i] * t2;
+                }
+                double xi = H[k - 1 + (k - 1) * ld], xj = H[k + (k - 1) * ld];
+                if(xi == 0.0)
+                {
+                    cs[k - 1] = 0.0;
+                    sn[k - 1] = 1.0;
+                }
+                else if(xj == 0.0)
+                {
+                    cs[k - 1] = 1.0;
+                    sn[k - 1] = 0.0;
+                }
+                else if(fabs(xi) > fabs(xj))
+                {
+                    double t  = xj / xi;
+                    cs[k - 1] = 1.0 / sqrt(1.0 + t * t);
+                    sn[k - 1] = t * cs[k - 1];
+                }
+                else
+                {
+                    double t  = xi / xj;
+                    sn[k - 1] = 1.0 / sqrt(1.0 + t * t);
+                    cs[k - 1] = t * sn[k - 1];
+                }
+                {
+                    double t1 = H[k - 1 + (k - 1) * ld], t2 = H[k + (k - 1) * ld];
+                    H[k - 1 + (k - 1) * ld] = cs[k - 1] * t1 + sn[k - 1] * t2;
+                    H[k + (k - 1) * ld]     = -sn[k - 1] * t1 + cs[k - 1] * t2;
+                }
+                y[k]     = -sn[k - 1] * y[k - 1];
+                y[k - 1] = cs[k - 1] * y[k - 1];
+                if(converges(fabs(y[k]), init, rel_tol, abs_tol) || k == restart)
+                    break;
+            }
+            for(int i = k - 1; i >= 0; i--)
+            {
+                for(int j = i + 1; j < k; j++)
+                    y[i] = y[i] - H[i + ld * j] * y[j];
+                y[i] = y[i] / H[i + ld * i];
+            }
+            for(int j = 0; j < k; j++)
+                for(int i = 0; i < m; i++)
+                    x[i] = x[i] + Q[(size_t)j * m + i] * y[j];
+            or_residual(m, rp, ci, v, x, b, res);
+            precond_apply(kind, h, d, m, res, z);
+            rn = sqrt(or_dot(m, z, z));
+            if(converges(rn, init, rel_tol, abs_tol))
+                break;
+            for(int i = 0; i < m; i++)
+                Q[i] = z[i] / rn;
+            memset(y, 0, sizeof(double) * (size_t)ld);
+            y[0] = rn;
+            iter++;
+        }
+    }
+    free(Q), free(H), free(cs), free(sn), free(res), free(z), free(w), free(d), free(y);
+    return iter;
+}
+
 /* The reference's CPU composition of an AMG preconditioner: x.zeros() and
  * amg_solve(h, x, rhs, n1, n2, vcycle, jacobi, {max_cycle = 0}), which runs
  * exactly one V-cycle (amg.cpp:516-571, iter_control.cpp:88-91); amg_solve's

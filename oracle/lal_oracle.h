@@ -102,6 +102,15 @@ void or_cycle(const or_hierarchy* h, double* x, const double* b, int n1, int n2,
 int or_amg_solve(const or_hierarchy* h, double* x, const double* b, int n1, int n2, int cycle,
                  int smoother, double rel_tol, double abs_tol, int max_cycle,
                  double* res_norm);
+/* bicgstab_solver (krylov/bicgstab.cpp:64-231); kind 0 none, 1 Jacobi, 2 one SA-AMG
+ * V-cycle of h (1 + 1 sweeps).  Returns the iteration count, x in place. */
+int or_bicgstab(int m, const int* rp, const int* ci, const double* v, double* x,
+                const double* b, int kind, const or_hierarchy* h, double rel_tol,
+                double abs_tol, int max_iter);
+/* Restarted left-preconditioned GMRES in the form of krylov/gmres.cpp (see the .c). */
+int or_gmres(int m, const int* rp, const int* ci, const double* v, double* x, const double* b,
+             int restart, int kind, const or_hierarchy* h, double rel_tol, double abs_tol,
+             int max_iter);
 void or_amg_precond(const or_hierarchy* h, const double* r, double* z, int n1, int n2,
                     int coarse_solver);
 /* Returns the iteration count as cg_solver::solve_precond does; res_norm gets the

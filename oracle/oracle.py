@@ -127,6 +127,12 @@ class Oracle:
         L.or_cg.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_double,
                             C.c_double, C.c_int, _dp]
         L.or_smooth.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_int]
+        L.or_bicgstab.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_int,
+                                  C.c_void_p, C.c_double, C.c_double, C.c_int]
+        L.or_bicgstab.restype = C.c_int
+        L.or_gmres.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_int, C.c_int,
+                               C.c_void_p, C.c_double, C.c_double, C.c_int]
+        L.or_gmres.restype = C.c_int
         L.or_amg_solve.argtypes = [C.c_void_p, _f64p, _f64p, C.c_int, C.c_int, C.c_int,
                                    C.c_int, C.c_double, C.c_double, C.c_int, _dp]
         L.or_amg_solve.restype = C.c_int
@@ -229,6 +235,22 @@ class Oracle:
         h = self.L.or_saamg_setup(A.m, A.n, A.nnz, A.row_ptr, A.col_ind, A.val, max_level,
                                   eps0, relax)
         return OracleHierarchy(self, h)
+
+    def bicgstab(self, A, b, x0=None, precond=0, hier=None, rel_tol=1e-8, abs_tol=1e-300,
+                 max_iter=1000):
+        """bicgstab_solver::solve; precond 0 none, 1 Jacobi, 2 one SA-AMG V-cycle
+        of hier (an OracleHierarchy); returns (x, iterations)."""
+        x = np.zeros(A.m) if x0 is None else _f64(x0).copy()
+        it = self.L.or_bicgstab(A.m, A.row_ptr, A.col_ind, A.val, x, _f64(b), precond,
+                                hier.h if hier else None, rel_tol, abs_tol, max_iter)
+        return x, it
+
+    def gmres(self, A, b, restart=30, x0=None, precond=0, hier=None, rel_tol=1e-8,
+              abs_tol=1e-300, max_iter=1000):
+        x = np.zeros(A.m) if x0 is None else _f64(x0).copy()
+        it = self.L.or_gmres(A.m, A.row_ptr, A.col_ind, A.val, x, _f64(b), restart, precond,
+                             hier.h if hier else None, rel_tol, abs_tol, max_iter)
+        return x, it
 
     def uaamg_setup(self, A, max_level=20, eps0=0.001):
         h = self.L.or_uaamg_setup(A.m, A.n, A.nnz, A.row_ptr, A.col_ind, A.val, max_level, eps0)
@@ -340,6 +362,12 @@ class Reference:
         L.ref_amg_solve.argtypes = [C.c_void_p, _f64p, _f64p, C.c_int, C.c_int, C.c_int,
                                     C.c_int, C.c_double, C.c_double, C.c_int]
         L.ref_amg_solve.restype = C.c_int
+        L.ref_bicgstab.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_int,
+                                   C.c_void_p, C.c_double, C.c_double, C.c_int]
+        L.ref_bicgstab.restype = C.c_int
+        L.ref_gmres.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_int,
+                                C.c_int, C.c_void_p, C.c_double, C.c_double, C.c_int]
+        L.ref_gmres.restype = C.c_int
         L.ref_read_mtx.argtypes = [C.c_char_p]
         L.ref_read_mtx.restype = C.c_void_p
         L.ref_write_mtx.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, _i32p, _i32p, _f64p]
@@ -392,6 +420,22 @@ class Reference:
 
     def transpose(self, A):
         return self._take(self.L.ref_transpose(A.m, A.n, A.nnz, A.row_ptr, A.col_ind, A.val))
+
+    def bicgstab(self, A, b, x0=None, precond=0, hier=None, rel_tol=1e-8, abs_tol=1e-300,
+                 max_iter=1000):
+        """bicgstab_solver::solve; precond 0 none, 1 Jacobi, 2 one SA-AMG V-cycle of
+        hier (a RefHierarchy)."""
+        x = np.zeros(A.m) if x0 is None else _f64(x0).copy()
+        it = self.L.ref_bicgstab(A.m, A.nnz, A.row_ptr, A.col_ind, A.val, x, _f64(b), precond,
+                                 hier.h if hier else None, rel_tol, abs_tol, max_iter)
+        return x, it
+
+    def gmres(self, A, b, restart=30, x0=None, precond=0, hier=None, rel_tol=1e-8,
+              abs_tol=1e-300, max_iter=1000):
+        x = np.zeros(A.m) if x0 is None else _f64(x0).copy()
+        it = self.L.ref_gmres(A.m, A.nnz, A.row_ptr, A.col_ind, A.val, x, _f64(b), restart,
+                              precond, hier.h if hier else None, rel_tol, abs_tol, max_iter)
+        return x, it
 
     def read_mtx(self, path):
         """csr_matrix::read_mtx of the reference; None when it refuses the file."""

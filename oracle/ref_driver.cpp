@@ -224,6 +224,73 @@ int ref_amg_solve(void* h, double* x, const double* b, int n1, int n2, int cycle
     return k;
 }
 
+} // extern "C"
+
+namespace
+{
+    // precond: 0 none, 1 jacobi_precond, 2 one SA-AMG V-cycle of hierarchy h
+    template <typename Solver, typename Build>
+    int run_krylov(Build build, int m, int nnz, const int* rp, const int* ci, const double* v,
+                   double* x, const double* b, int precond, void* h, double rel_tol,
+                   double abs_tol, int max_iter)
+    {
+        linalg::csr_matrix<double>* A = make_csr(m, m, nnz, rp, ci, v);
+        linalg::vector<double>      xv(std::vector<double>(x, x + m));
+        linalg::vector<double>      bv(std::vector<double>(b, b + m));
+        Solver                      solver;
+        build(solver, *A);
+        linalg::iter_control c;
+        c.rel_tol  = rel_tol;
+        c.abs_tol  = abs_tol;
+        c.max_iter = max_iter;
+        linalg::jacobi_precond jp;
+        amg_precond            ap;
+        const linalg::preconditioner* p = nullptr;
+        if(precond == 1)
+        {
+            jp.build(*A);
+            p = &jp;
+        }
+        else if(precond == 2)
+        {
+            ap.h  = &static_cast<ref_hier*>(h)->h;
+            ap.n1 = 1;
+            ap.n2 = 1;
+            p     = &ap;
+        }
+        int it = solver.solve(*A, xv, bv, p, c);
+        std::memcpy(x, xv.get_vec(), sizeof(double) * m);
+        delete A;
+        return it;
+    }
+}
+
+extern "C" {
+
+// bicgstab_solver::solve (krylov/bicgstab.cpp:64-249); x in place.
+int ref_bicgstab(int m, int nnz, const int* rp, const int* ci, const double* v, double* x,
+                 const double* b, int precond, void* h, double rel_tol, double abs_tol,
+                 int max_iter)
+{
+    quiet q;
+    return run_krylov<linalg::bicgstab_solver>(
+        [](linalg::bicgstab_solver& s, const linalg::csr_matrix<double>& A) { s.build(A); }, m,
+        nnz, rp, ci, v, x, b, precond, h, rel_tol, abs_tol, max_iter);
+}
+
+// gmres_solver::solve (krylov/gmres.cpp:224-540) with the given restart; x in place.
+int ref_gmres(int m, int nnz, const int* rp, const int* ci, const double* v, double* x,
+              const double* b, int restart, int precond, void* h, double rel_tol,
+              double abs_tol, int max_iter)
+{
+    quiet q;
+    return run_krylov<linalg::gmres_solver>(
+        [restart](linalg::gmres_solver& s, const linalg::csr_matrix<double>& A) {
+            s.build(A, restart);
+        },
+        m, nnz, rp, ci, v, x, b, precond, h, rel_tol, abs_tol, max_iter);
+}
+
 // Unpreconditioned cg_solver::solve.
 int ref_cg(int m, int nnz, const int* rp, const int* ci, const double* v, double* x,
            const double* b, double rel_tol, double abs_tol, int max_iter)

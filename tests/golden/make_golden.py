@@ -185,9 +185,33 @@ def ua_case(ref):
     print("ua written")
 
 
+def krylov_case(ref):
+    """bicgstab_solver with no, Jacobi and SA-AMG preconditioning on two grids:
+    iteration counts and solutions (the reference's GMRES does not run: its
+    build() leaves the residual vectors unsized)."""
+    from oracle.oracle import CSR
+
+    out = {}
+    for name in ("2d8", "3d8"):
+        rp, ci, v, m = grid(name)
+        A = CSR(rp, ci, v, m, m)
+        b = np.random.default_rng(1).uniform(-1, 1, m)
+        h = ref.saamg_setup(A)
+        out[f"{name}_b"] = b
+        for pc in range(3):
+            x, it = ref.bicgstab(A, b, precond=pc, hier=h, rel_tol=1e-10, max_iter=500)
+            out[f"{name}_p{pc}_x"] = x
+            out[f"{name}_p{pc}_it"] = np.array([it])
+    np.savez_compressed(os.path.join(OUT, "golden_krylov.npz"), **out)
+    print("krylov written")
+
+
 def main():
     ref = Reference()
     ref.set_threads(1)
+    if len(sys.argv) > 1 and sys.argv[1] == "krylov":
+        krylov_case(ref)
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "ua":
         ua_case(ref)
         return
@@ -202,6 +226,7 @@ def main():
     ops_case(ref)
     cycles_case(ref)
     ua_case(ref)
+    krylov_case(ref)
 
 
 if __name__ == "__main__":

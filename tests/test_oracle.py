@@ -397,3 +397,68 @@ def test_uaamg_closed_form(oracle):
     got = np.zeros((Ac.m, Ac.m))
     got[np.repeat(np.arange(Ac.m), np.diff(Ac.row_ptr)), Ac.col_ind] = Ac.val
     assert np.allclose(got, dense, rtol=0, atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# BiCGSTAB and GMRES (krylov/bicgstab.cpp, gmres.cpp)
+
+def test_bicgstab_matches_reference_golden(oracle, golden):
+    g = golden("krylov")
+    for name in ("2d8", "3d8"):
+        rp, ci, v, m = grid(name)
+        A = CSR(rp, ci, v, m, m)
+        h = oracle.saamg_setup(A)
+        for pc in range(3):
+            x, it = oracle.bicgstab(A, g[f"{name}_b"], precond=pc, hier=h, rel_tol=1e-10,
+                                    max_iter=500)
+            assert it == int(g[f"{name}_p{pc}_it"][0]), (name, pc)
+            assert bits_equal(x, g[f"{name}_p{pc}_x"]), (name, pc)
+
+
+def test_bicgstab_matches_compiled_reference_nonsymmetric(oracle, reference):
+    import scipy.sparse as sp
+
+    m = 300
+    S = sp.random(m, m, density=0.02, random_state=5, format="csr") + sp.eye(m) * 4
+    S = S.tocsr()
+    S.sort_indices()
+    A = CSR(S.indptr, S.indices, S.data, m, m)
+    b = np.random.default_rng(5).uniform(-1, 1, m)
+    for pc in (0, 1):
+        xo, io = oracle.bicgstab(A, b, precond=pc, rel_tol=1e-10, max_iter=500)
+        xr, ir = reference.bicgstab(A, b, precond=pc, rel_tol=1e-10, max_iter=500)
+        assert io == ir and bits_equal(xo, xr), pc
+
+
+def test_gmres_closed_forms(oracle):
+    """GMRES is not checked against the reference (its gmres_solver cannot run:
+    build() never sizes the residual vectors).  Pinned instead: with restart
+    >= n it is exact after n Arnoldi steps (one cycle) on a small nonsymmetric
+    system; restarted runs agree with scipy's GMRES to the tolerance; the
+    preconditioned variants reach the direct solution."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    rng = np.random.default_rng(7)
+    m = 40
+    S = (sp.random(m, m, density=0.15, random_state=7, format="csr") + sp.eye(m) * 3).tocsr()
+    S.sort_indices()
+    A = CSR(S.indptr, S.indices, S.data, m, m)
+    b = rng.uniform(-1, 1, m)
+    xd = np.linalg.solve(S.toarray(), b)
+    x, it = oracle.gmres(A, b, restart=m - 1, rel_tol=1e-14, max_iter=3)
+    assert np.abs(x - xd).max() <= 1e-10 * np.abs(xd).max()
+    for pc in (0, 1):
+        x, it = oracle.gmres(A, b, restart=8, precond=pc, rel_tol=1e-12, max_iter=200)
+        assert np.abs(x - xd).max() <= 1e-9 * np.abs(xd).max(), pc
+    xs, info = spla.gmres(S, b, restart=8, rtol=1e-12, atol=0, maxiter=200)
+    assert info == 0
+    x, it = oracle.gmres(A, b, restart=8, rel_tol=1e-12, max_iter=200)
+    assert np.abs(x - xs).max() <= 1e-8 * np.abs(xs).max()
+    rp, ci, v, m = grid("3d8")
+    A = CSR(rp, ci, v, m, m)
+    b = rng.uniform(-1, 1, m)
+    x, it = oracle.gmres(A, b, restart=20, precond=2, hier=oracle.saamg_setup(A),
+                         rel_tol=1e-12, max_iter=50)
+    r = oracle.residual(A, x, b)
+    assert np.abs(r).max() <= 1e-9 * np.abs(b).max()
