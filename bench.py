@@ -41,12 +41,15 @@ def parse():
                     help="only the standalone SpMV loop (for counter runs)")
     ap.add_argument("--mixed", choices=("vcycle", "ir"), default="vcycle",
                     help="mixed scheme reported as value")
-    ap.add_argument("--config", choices=("c2", "c1", "c3", "c4", "c5", "amg"), default="c2",
+    ap.add_argument("--config", choices=("c2", "c1", "c3", "c4", "c5", "amg", "krylov"),
+                    default="c2",
                     help="c2 (default): 3D Poisson 128^3, the headline line; the others "
                          "print one JSON line each for the remaining workloads: c1 2D "
                          "Poisson 128^2 to 1e-8, c3 log-normal FV diffusion 160^3, c4 96^3 "
                          "face-conductance gradient, c5 kNN graph Laplacian (2M points), "
-                         "amg: AMG as a solver (V/W/F cycles, Jacobi/GS/SSOR) on 128^3")
+                         "amg: AMG as a solver (V/W/F cycles, Jacobi/GS/SSOR) on 128^3, "
+                         "krylov: BiCGSTAB / GMRES(30) with Jacobi and SA-AMG "
+                         "preconditioning on 128^3 Poisson and 1024^2 convection-diffusion")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     return ap.parse_args()
@@ -184,6 +187,8 @@ def run_config(args):
         tol, name = 1e-10, "kNN graph Laplacian + 1e-2 I, 2M points in [0,1]^3, 12-NN, to 1e-10"
     elif cfg == "amg":
         return run_amg(args)
+    elif cfg == "krylov":
+        return run_krylov(args)
     else:
         return run_gradient(args)
     nnz = int(rp[-1])
@@ -287,6 +292,92 @@ def run_amg(args):
                                 "cores": threads,
                                 "kind": "reference" if Reference.available() else "port",
                                 "sample": "one cycle of amg_solve on the same problem"}
+    print(json.dumps(line), flush=True)
+
+
+def run_krylov(args):
+    """BiCGSTAB and restarted GMRES(30) beside SA-AMG-PCG: seconds to 1e-10
+    and per iteration on 3D Poisson 128^3 (SPD) and on 2D upwind
+    convection-diffusion 1024^2 (nonsymmetric), with Jacobi and SA-AMG
+    preconditioning; the host reference's BiCGSTAB (and the oracle's GMRES:
+    the reference's cannot run) per iteration on the 128^3 problem."""
+    import torch
+
+    from linalglib_amd import lal, problems
+
+    dev = torch.device("cuda:0")
+    h = lal.Handle()
+    line = {"config": {"workload": "Krylov solvers to 1e-10: 3D Poisson 128^3 and 2D "
+                                   "convection-diffusion 1024^2 (wind (20, -10), upwind)"},
+            "gpu": {}}
+
+    def timed(tag, fn, X):
+        X.zero_()
+        fn()
+        torch.cuda.synchronize()
+        X.zero_()
+        t0 = time.perf_counter()
+        info = fn()
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        rec = {"solve_s": round(sec, 5), "iterations": info.iterations,
+               "converged": bool(info.converged),
+               "s_per_iteration": round(sec / (info.iterations + 1), 7)}
+        line["gpu"][tag] = rec
+        print(f"# krylov {tag}: {rec}", file=sys.stderr, flush=True)
+
+    for prob in ("poisson128", "cd1024"):
+        if prob == "poisson128":
+            rp, ci, v, m = problems.poisson3d(128)
+        else:
+            rp, ci, v, m = problems.convection_diffusion2d(1024)
+        b = problems.rhs_from_solution(rp, ci, v, problems.uniform(m, seed=1))
+        nnz = int(rp[-1])
+        RP, CI, V, B = (torch.from_numpy(a).to(dev) for a in (rp, ci, v, b))
+        D = lal.CsrDescr(h, m, m, nnz, RP, CI)
+        A = (RP, CI, V, m, nnz, D)
+        H = lal.Hierarchy(h, m, nnz, RP, CI, V)
+        X = torch.zeros(m, dtype=torch.float64, device=dev)
+        kw = dict(rel_tol=1e-10, abs_tol=1e-300)
+        line["config"][prob] = {"n": m, "nnz": nnz, "levels": H.levels}
+        if prob == "poisson128":
+            timed(f"{prob}_pcg_saamg", lambda: H.pcg(B, X, max_iter=1000, **kw), X)
+        timed(f"{prob}_bicgstab_saamg",
+              lambda: lal.bicgstab(h, A, B, X, "saamg", H, max_iter=1000, **kw), X)
+        timed(f"{prob}_gmres30_saamg",
+              lambda: lal.gmres(h, A, B, X, 30, "saamg", H, max_iter=200, **kw), X)
+        timed(f"{prob}_bicgstab_jacobi",
+              lambda: lal.bicgstab(h, A, B, X, "jacobi", None, max_iter=5000, **kw), X)
+        del H, D
+        torch.cuda.synchronize()
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+        from oracle.oracle import CSR, Oracle, Reference
+
+        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        rp, ci, v, m = problems.poisson3d(128)
+        b = problems.rhs_from_solution(rp, ci, v, problems.uniform(m, seed=1))
+        Ac = CSR(rp, ci, v, m, m)
+        lib = Reference() if Reference.available() else Oracle()
+        lib.set_threads(threads)
+        print("# krylov: host BiCGSTAB (3 iterations) and GMRES (one 30-step cycle)",
+              file=sys.stderr, flush=True)
+        hc = lib.saamg_setup(Ac)
+        t0 = time.perf_counter()
+        lib.bicgstab(Ac, b, precond=2, hier=hc, rel_tol=1e-300, max_iter=2)
+        bic = (time.perf_counter() - t0) / 3
+        orc = Oracle()
+        orc.set_threads(threads)
+        ho = orc.saamg_setup(Ac)
+        t0 = time.perf_counter()
+        orc.gmres(Ac, b, restart=30, precond=2, hier=ho, rel_tol=1e-300, max_iter=0)
+        gm = (time.perf_counter() - t0) / 30
+        line["cpu_baseline"] = {
+            "bicgstab_saamg_s_per_iteration": round(bic, 4), "bicgstab_cores": threads,
+            "bicgstab_kind": "reference" if Reference.available() else "port",
+            "gmres_saamg_s_per_arnoldi_step": round(gm, 4), "gmres_cores": threads,
+            "gmres_kind": "port",
+            "sample": "poisson128: 3 BiCGSTAB iterations; one GMRES(30) restart cycle"}
     print(json.dumps(line), flush=True)
 
 

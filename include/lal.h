@@ -278,6 +278,40 @@ int lal_amg_solve(lal_handle h, lal_hierarchy hier, const double* b, double* x, 
                   int cycle, int smoother, const lal_solve_options* opts, lal_solve_info* info);
 
 /* ------------------------------------------------------------------------ */
+/* Jacobi preconditioner and the nonsymmetric Krylov solvers -- replace      */
+/* linalg::jacobi_precond (iterative_solvers/preconditioner/                 */
+/* preconditioner.cpp:41-58), linalg::bicgstab_solver (krylov/bicgstab.cpp: */
+/* 64-231) and linalg::gmres_solver (krylov/gmres.cpp:46-530).               */
+/* ------------------------------------------------------------------------ */
+/* x = rhs / diag elementwise: jacobi_precond::solve (host_math.cpp:53-64). */
+int lal_jacobi_solve_f64(lal_handle h, int n, const double* rhs, const double* diag, double* x);
+
+/* Preconditioner kinds of the Krylov solvers below. */
+#define LAL_PRECOND_NONE 0
+#define LAL_PRECOND_JACOBI 1 /* first diagonal entry of each row       */
+#define LAL_PRECOND_SAAMG 2  /* one V-cycle of hier (opts->n1, n2 sweeps) */
+
+/* BiCGSTAB on the square CSR matrix A (device arrays, descr from
+ * lal_csrmv_analysis or NULL); x is the initial guess on entry.  Convergence
+ * on the inf-norm residual (iter_control); info->iterations is the
+ * reference's return value.  The breakdown branch (t.r or t.t exactly zero)
+ * adds alpha p to x and stops, as bicgstab.cpp does.  hier is needed only for
+ * LAL_PRECOND_SAAMG and must be built for A. */
+int lal_bicgstab_solve(lal_handle h, lal_csrmv_descr descr, int m, int nnz, const int* row_ptr,
+                       const int* col_ind, const double* val, const double* b, double* x,
+                       int precond, lal_hierarchy hier, const lal_solve_options* opts,
+                       lal_solve_info* info);
+/* Restarted, left-preconditioned GMRES(restart) (restart clamped to
+ * [1, m-1]); convergence on the 2-norm of the preconditioned residual;
+ * info->iterations counts restart cycles as gmres.cpp does.  The
+ * preconditioned Arnoldi vector is kept and every work vector sized, where
+ * gmres.cpp drops the one and leaves the others unsized. */
+int lal_gmres_solve(lal_handle h, lal_csrmv_descr descr, int m, int nnz, const int* row_ptr,
+                    const int* col_ind, const double* val, const double* b, double* x,
+                    int restart, int precond, lal_hierarchy hier, const lal_solve_options* opts,
+                    lal_solve_info* info);
+
+/* ------------------------------------------------------------------------ */
 /* Adjoint (no reference counterpart): for loss L(x) with x = A^-1 b and A   */
 /* SPD, dL/db = lambda with A lambda = g, and dL/dA_ij = -lambda_i x_j over  */
 /* A's pattern.                                                              */

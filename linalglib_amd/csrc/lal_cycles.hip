@@ -79,22 +79,227 @@ namespace
             gs_row(perm[t], rp, ci, v, b, x, sor);
     }
 
-    // A whole sweep in one workgroup: the level sets in order, a barrier
-    // between them (the rows written in one set are read by later sets of the
-    // same workgroup), for operators whose sets are small.
-    __global__ void __launch_bounds__(1024)
-        gs_sweep_wg_kernel(int nsets, const int* __restrict__ off, const int* __restrict__ perm,
-                           const int* __restrict__ rp, const int* __restrict__ ci,
-                           const double* __restrict__ v, const double* __restrict__ b, double* x,
-                           int sor)
+    // A whole sweep in one workgroup for operators whose level sets are thin
+    // (few rows, long rows).  The sweep is a host-built list of tiles: a
+    // tile is up to GS_CH consecutive entries of a group of at most 1024 rows
+    // of one set, in the copy of the matrix whose rows are in set order.
+    // Per tile the lanes form the products a_jk x_k of the tile's entries
+    // into LDS (+0.0 in a diagonal entry's slot); then one lane per row adds
+    // its row's slots in CSR order.  sigma starts at +0.0 and so is never
+    // -0.0, which makes adding the diagonal's +0.0 an exact no-op: the sums
+    // are those of the sequential sweep, which skips the diagonal.  a_jj (the
+    // row's last diagonal entry, 0 if none) comes from pdiag.  The rows are
+    // written after their group's last tile and a barrier ends each set.
+    // What does not depend on x (the next tile's columns and values, the next
+    // group's row bounds, b) is loaded before the LDS-only barriers, so a
+    // tile waits only for its gathers of x.  With XLDS, x lives in LDS.
+    constexpr int GS_CH       = 8192;
+    constexpr int GS_PER_LANE = GS_CH / 1024;
+    // tile flags (low 3 bits of the fourth word; the group's end row above)
+    constexpr int GS_START = 1, GS_END_GROUP = 2, GS_END_SET = 4;
+
+    // Barrier that orders LDS accesses only: global loads stay in flight.
+    __device__ __forceinline__ void lds_barrier()
     {
-        for(int s = 0; s < nsets; s++)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    }
+
+    __device__ __forceinline__ void gs_tile_loads(int c0, int c1, int tid,
+                                                  const int* __restrict__ pci,
+                                                  const double* __restrict__ pv, int* col,
+                                                  double* vv)
+    {
+#pragma unroll
+        for(int u = 0; u < GS_PER_LANE; u++)
         {
-            const int e = off[s + 1];
-            for(int t = off[s] + (int)threadIdx.x; t < e; t += 1024)
-                gs_row(perm[t], rp, ci, v, b, x, sor);
+            // clamped into the tile (entry 0 for an empty one) instead of a branch
+            const int k = max(0, min(c0 + tid + u * 1024, c1 - 1));
+            col[u]      = pci[k];
+            vv[u]       = pv[k];
+        }
+    }
+
+    template <bool XLDS>
+    __global__ void __launch_bounds__(1024)
+        gs_sweep_wg_kernel(int m, int ntiles, const int4* __restrict__ tiles,
+                           const int* __restrict__ perm, const int* __restrict__ prp,
+                           const int* __restrict__ pci, const double* __restrict__ pv,
+                           const double* __restrict__ pdiag, const double* __restrict__ b,
+                           double* x, int sor)
+    {
+        __shared__ double        s_p[GS_CH];
+        extern __shared__ double s_x[];
+        double*                  X   = XLDS ? s_x : x;
+        const int                tid = threadIdx.x;
+        if(XLDS)
+        {
+            for(int i = tid; i < m; i += 1024)
+                s_x[i] = x[i];
             __syncthreads();
         }
+        int4   T = tiles[0];
+        int    col[GS_PER_LANE];
+        double vv[GS_PER_LANE];
+        gs_tile_loads(T.x, T.y, tid, pci, pv, col, vv);
+        int    j     = T.z + tid;
+        bool   mine  = j < (T.w >> 3);
+        int    rs    = mine ? prp[j] : 0;
+        int    re    = mine ? prp[j + 1] : 0;
+        int    row   = mine ? perm[j] : 0;
+        double ajj   = mine ? pdiag[j] : 1.0;
+        double sigma = 0.0;
+        for(int t = 0; t < ntiles; t++)
+        {
+            const int  fl   = T.w & 7;
+            const bool more = t + 1 < ntiles;
+            const int4 N    = more ? tiles[t + 1] : T;
+            double     xv[GS_PER_LANE];
+#pragma unroll
+            for(int u = 0; u < GS_PER_LANE; u++)
+                xv[u] = X[col[u] < 0 ? 0 : col[u]];
+            double bj = 0.0, xo = 0.0;
+            if((fl & GS_END_GROUP) && mine)
+            {
+                bj = b[row];
+                xo = X[row]; // written by this row only
+            }
+            // the next tile's x-independent loads, in flight across the barriers
+            int    ncol[GS_PER_LANE];
+            double nvv[GS_PER_LANE];
+            gs_tile_loads(N.x, N.y, tid, pci, pv, ncol, nvv);
+            const bool ngrp  = more && (N.w & GS_START);
+            const int  nj    = N.z + tid;
+            const bool nmine = ngrp ? nj < (N.w >> 3) : mine;
+            int        nrs = rs, nre = re, nrow = row;
+            double     najj = ajj;
+            if(ngrp && nmine)
+            {
+                nrs  = prp[nj];
+                nre  = prp[nj + 1];
+                nrow = perm[nj];
+                najj = pdiag[nj];
+            }
+#pragma unroll
+            for(int u = 0; u < GS_PER_LANE; u++)
+            {
+                const int k = T.x + tid + u * 1024;
+                if(k < T.y)
+                    s_p[k - T.x] = col[u] < 0 ? 0.0 : vv[u] * xv[u];
+            }
+            lds_barrier();
+            if(mine)
+            {
+                const int k0 = max(rs, T.x) - T.x;
+                const int k1 = min(re, T.y) - T.x;
+                int       k  = k0;
+                for(; k + 8 <= k1; k += 8)
+                {
+                    double p[8];
+#pragma unroll
+                    for(int u = 0; u < 8; u++)
+                        p[u] = s_p[k + u];
+#pragma unroll
+                    for(int u = 0; u < 8; u++)
+                        sigma = sigma + p[u];
+                }
+                for(; k < k1; k++)
+                    sigma = sigma + s_p[k];
+            }
+            lds_barrier();
+            if(fl & GS_END_GROUP)
+            {
+                if(mine)
+                    X[row] = sor ? xo + 0.5 * ((bj - sigma) / ajj - xo) : (bj - sigma) / ajj;
+                sigma = 0.0;
+            }
+            if(fl & GS_END_SET)
+            {
+                if(XLDS)
+                    lds_barrier();
+                else
+                    __syncthreads();
+            }
+            T = N;
+#pragma unroll
+            for(int u = 0; u < GS_PER_LANE; u++)
+            {
+                col[u] = ncol[u];
+                vv[u]  = nvv[u];
+            }
+            if(ngrp)
+            {
+                mine = nmine;
+                rs   = nrs;
+                re   = nre;
+                row  = nrow;
+                ajj  = najj;
+            }
+        }
+        if(XLDS)
+        {
+            __syncthreads();
+            for(int i = tid; i < m; i += 1024)
+                x[i] = s_x[i];
+        }
+    }
+
+    // The tiles of one sweep from the set offsets and the set-ordered row
+    // pointer; false when a group end row does not fit beside the flags.
+    bool gs_tiles(const std::vector<int>& off, const std::vector<int>& prp,
+                  std::vector<int>& tiles)
+    {
+        tiles.clear();
+        for(size_t s = 0; s + 1 < off.size(); s++)
+        {
+            const int a = off[s], e = off[s + 1];
+            for(int g0 = a; g0 < e; g0 += 1024)
+            {
+                const int g1 = std::min(e, g0 + 1024);
+                if(g1 >= (1 << 28))
+                    return false;
+                const int e0     = prp[g0];
+                const int e1     = prp[g1];
+                const int chunks = std::max(1, (e1 - e0 + GS_CH - 1) / GS_CH);
+                for(int ch = 0; ch < chunks; ch++)
+                {
+                    const int c0 = e0 + ch * GS_CH;
+                    const int c1 = std::min(e1, c0 + GS_CH);
+                    const int fl = (ch == 0 ? GS_START : 0) |
+                                   (ch == chunks - 1 ? GS_END_GROUP : 0) |
+                                   (ch == chunks - 1 && g1 == e ? GS_END_SET : 0);
+                    tiles.insert(tiles.end(), {c0, c1, g0, (g1 << 3) | fl});
+                }
+            }
+        }
+        return !tiles.empty();
+    }
+
+    // Bytes of LDS one workgroup may hold on the current device, and the
+    // dynamic part the single-workgroup sweep may take for x.
+    int wg_sweep_lds_limit(int* dyn_max)
+    {
+        static int cached = -1;
+        if(cached < 0)
+        {
+            int dev = 0, v = 0;
+            LAL_HIP(hipGetDevice(&dev));
+            LAL_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+            const int stat = GS_CH * (int)sizeof(double);
+            int       dyn  = v - stat;
+            if(dyn > 0 && v > 65536 &&
+               hipFuncSetAttribute(reinterpret_cast<const void*>(&gs_sweep_wg_kernel<true>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   dyn) != hipSuccess)
+            {
+                (void)hipGetLastError();
+                dyn = 65536 - stat;
+            }
+            cached = dyn > 0 ? dyn : 0;
+        }
+        *dyn_max = cached;
+        return LAL_OK;
     }
 
     // Level sets of the forward (and backward) sweep from the host pattern:
@@ -136,7 +341,15 @@ namespace
             perm[fill[lev[j]]++] = j;
     }
 
-    int build_sched(lal_context* c, const csr_dev& A, gs_sched& f, gs_sched& bw)
+    // Whether a sweep over these sets runs in one workgroup.
+    bool wg_sweep(int m, size_t nsets)
+    {
+        return nsets > 0 && (size_t)m / nsets < 1024;
+    }
+
+    // Level sets of one sweep direction; for a single-workgroup sweep also the
+    // set-ordered copy of the matrix.
+    int build_sched(lal_context* c, const csr_dev& A, bool forward, gs_sched& g)
     {
         std::vector<int> rp(A.m + 1), ci(std::max(A.nnz, 1));
         LAL_HIP(hipMemcpyAsync(rp.data(), A.rp, sizeof(int) * (A.m + 1), hipMemcpyDeviceToHost,
@@ -145,16 +358,56 @@ namespace
             LAL_HIP(hipMemcpyAsync(ci.data(), A.ci, sizeof(int) * A.nnz, hipMemcpyDeviceToHost,
                                    c->stream));
         LAL_HIP(hipStreamSynchronize(c->stream));
-        for(int dir = 0; dir < 2; dir++)
+        std::vector<int> perm;
+        level_sets(rp, ci, A.m, forward, perm, g.off);
+        LAL_TRY(lal::alloc_n(&g.perm, perm.size()));
+        LAL_HIP(hipMemcpyAsync(g.perm, perm.data(), sizeof(int) * perm.size(),
+                               hipMemcpyHostToDevice, c->stream));
+        LAL_TRY(lal::alloc_n(&g.off_d, g.off.size()));
+        LAL_HIP(hipMemcpyAsync(g.off_d, g.off.data(), sizeof(int) * g.off.size(),
+                               hipMemcpyHostToDevice, c->stream));
+        if(wg_sweep(A.m, g.off.size() - 1) && A.nnz > 0)
         {
-            gs_sched&        g = dir == 0 ? f : bw;
-            std::vector<int> perm;
-            level_sets(rp, ci, A.m, dir == 0, perm, g.off);
-            LAL_TRY(lal::alloc_n(&g.perm, perm.size()));
-            LAL_HIP(hipMemcpyAsync(g.perm, perm.data(), sizeof(int) * perm.size(),
+            std::vector<double> v(A.nnz);
+            LAL_HIP(hipMemcpyAsync(v.data(), A.v, sizeof(double) * A.nnz,
+                                   hipMemcpyDeviceToHost, c->stream));
+            LAL_HIP(hipStreamSynchronize(c->stream));
+            std::vector<int>    prp(A.m + 1, 0), pci(A.nnz);
+            std::vector<double> pv(A.nnz), pdiag(A.m, 0.0);
+            for(int t = 0; t < A.m; t++)
+            {
+                const int j = perm[t];
+                int       o = prp[t];
+                for(int k = rp[j]; k < rp[j + 1]; k++, o++)
+                {
+                    pci[o] = ci[k] == j ? -1 - ci[k] : ci[k];
+                    pv[o]  = v[k];
+                    if(ci[k] == j)
+                        pdiag[t] = v[k]; // the last one, as the sweep's loop keeps
+                }
+                prp[t + 1] = o;
+            }
+            std::vector<int> tiles;
+            if(!gs_tiles(g.off, prp, tiles))
+            {
+                LAL_HIP(hipStreamSynchronize(c->stream));
+                return LAL_OK; // swept one launch per set instead
+            }
+            g.ntiles = (int)(tiles.size() / 4);
+            LAL_TRY(lal::alloc_n(&g.tiles, tiles.size()));
+            LAL_HIP(hipMemcpyAsync(g.tiles, tiles.data(), sizeof(int) * tiles.size(),
                                    hipMemcpyHostToDevice, c->stream));
-            LAL_TRY(lal::alloc_n(&g.off_d, g.off.size()));
-            LAL_HIP(hipMemcpyAsync(g.off_d, g.off.data(), sizeof(int) * g.off.size(),
+            LAL_TRY(lal::alloc_n(&g.prp, prp.size()));
+            LAL_TRY(lal::alloc_n(&g.pci, pci.size()));
+            LAL_TRY(lal::alloc_n(&g.pv, pv.size()));
+            LAL_HIP(hipMemcpyAsync(g.prp, prp.data(), sizeof(int) * prp.size(),
+                                   hipMemcpyHostToDevice, c->stream));
+            LAL_HIP(hipMemcpyAsync(g.pci, pci.data(), sizeof(int) * pci.size(),
+                                   hipMemcpyHostToDevice, c->stream));
+            LAL_HIP(hipMemcpyAsync(g.pv, pv.data(), sizeof(double) * pv.size(),
+                                   hipMemcpyHostToDevice, c->stream));
+            LAL_TRY(lal::alloc_n(&g.pdiag, pdiag.size()));
+            LAL_HIP(hipMemcpyAsync(g.pdiag, pdiag.data(), sizeof(double) * pdiag.size(),
                                    hipMemcpyHostToDevice, c->stream));
         }
         LAL_HIP(hipStreamSynchronize(c->stream));
@@ -164,11 +417,18 @@ namespace
     int gs_sweep(lal_context* c, const csr_dev& A, const gs_sched& g, const double* b, double* x,
                  int sor)
     {
-        const int nsets = (int)g.off.size() - 1;
-        if(nsets > 0 && A.m / nsets < 1024)
+        if(g.prp)
         {
-            gs_sweep_wg_kernel<<<1, 1024, 0, c->stream>>>(nsets, g.off_d, g.perm, A.rp, A.ci, A.v,
-                                                          b, x, sor);
+            int dyn_max = 0;
+            LAL_TRY(wg_sweep_lds_limit(&dyn_max));
+            const size_t xb    = sizeof(double) * (size_t)A.m;
+            const int4*  tiles = reinterpret_cast<const int4*>(g.tiles);
+            if(xb <= (size_t)dyn_max)
+                gs_sweep_wg_kernel<true><<<1, 1024, xb, c->stream>>>(
+                    A.m, g.ntiles, tiles, g.perm, g.prp, g.pci, g.pv, g.pdiag, b, x, sor);
+            else
+                gs_sweep_wg_kernel<false><<<1, 1024, 0, c->stream>>>(
+                    A.m, g.ntiles, tiles, g.perm, g.prp, g.pci, g.pv, g.pdiag, b, x, sor);
             LAL_LAUNCHED();
             return LAL_OK;
         }
@@ -199,7 +459,9 @@ namespace
                                              x);
         }
         if(!L.gs_fwd.perm)
-            LAL_TRY(build_sched(c, A, L.gs_fwd, L.gs_bwd));
+            LAL_TRY(build_sched(c, A, true, L.gs_fwd));
+        if(smoother == 4 && !L.gs_bwd.perm)
+            LAL_TRY(build_sched(c, A, false, L.gs_bwd));
         const int sor = smoother >= 3;
         LAL_TRY(gs_sweep(c, A, L.gs_fwd, b, x, sor));
         if(smoother == 4)

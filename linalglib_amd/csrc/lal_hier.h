@@ -47,12 +47,21 @@ namespace lal
     // Level sets of the Gauss-Seidel sweeps of one operator: the rows of set k
     // depend only on rows of sets < k, so a set is one launch and each row is
     // updated in the order the sequential sweep would use (forward, and
-    // backward for the second half of SSOR).
+    // backward for the second half of SSOR).  Operators whose sets are thin
+    // are swept by one workgroup from a copy of the matrix with its rows in
+    // set order (prp/pci/pv; a diagonal entry of column c is stored as
+    // -1 - c), so that a set's entries are contiguous.
     struct gs_sched
     {
         int*             perm  = nullptr; // rows grouped by set
         std::vector<int> off;             // set k: perm[off[k] .. off[k+1])
         int*             off_d = nullptr; // the same offsets on the device
+        int*             prp   = nullptr; // row pointer of the set-ordered copy
+        int*             pci   = nullptr;
+        double*          pv    = nullptr;
+        double*          pdiag = nullptr; // a_jj per row of the copy (last diagonal entry)
+        int*             tiles = nullptr; // 4 ints per tile of the one-workgroup sweep
+        int              ntiles = 0;
     };
 
     struct level_t
@@ -129,9 +138,10 @@ struct lal_hierarchy_t
             for(float* ptr : {L[l].w32.x, L[l].w32.t, L[l].w32.b, L[l].w32.r, L[l].w32.diag,
                               L[l].w32.dinv})
                 (void)hipFree(ptr);
-            for(int* ptr : {L[l].gs_fwd.perm, L[l].gs_bwd.perm, L[l].gs_fwd.off_d,
-                            L[l].gs_bwd.off_d})
-                (void)hipFree(ptr);
+            for(lal::gs_sched* g : {&L[l].gs_fwd, &L[l].gs_bwd})
+                for(void* ptr : {(void*)g->perm, (void*)g->off_d, (void*)g->prp, (void*)g->pci,
+                                 (void*)g->pv, (void*)g->pdiag, (void*)g->tiles})
+                    (void)hipFree(ptr);
         }
         for(void* ptr : {(void*)coarse_inv, (void*)res, (void*)z, (void*)p, (void*)q,
                          (void*)res32, (void*)z32, (void*)p32, (void*)q32, (void*)d32, (void*)s,

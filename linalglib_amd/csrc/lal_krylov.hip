@@ -1,0 +1,367 @@
+// BiCGSTAB and restarted GMRES on the device with no, Jacobi or SA-AMG
+// preconditioning, and the Jacobi preconditioner itself.
+//
+//   jacobi_precond      iterative_solvers/preconditioner/preconditioner.cpp:41-58,
+//                       backend/host/host_math.cpp:53-64 (z = r / a_ii)
+//   bicgstab_solver     iterative_solvers/krylov/bicgstab.cpp:64-231, including
+//                       the breakdown branch's x += alpha p
+//   gmres_solver        iterative_solvers/krylov/gmres.cpp:46-530: restarted,
+//                       left-preconditioned, modified Gram-Schmidt Arnoldi,
+//                       Givens rotations, convergence on the preconditioned
+//                       residual 2-norm, restart cycles counted as iterations.
+//                       gmres.cpp drops the preconditioned Arnoldi vector (it
+//                       lands in a temporary) and never sizes its residual
+//                       vectors; here q_k = M^-1 A q_(k-1) is kept and every
+//                       work vector is sized.
+//
+// The vectors stay on the device; the solvers' scalars (inner products,
+// norms, the small Hessenberg system) come to the host as in the reference's
+// loops.  Work vectors are stream-ordered allocations of the call.
+#include "lal_hier.h"
+
+#include <cmath>
+#include <vector>
+
+namespace
+{
+    __global__ void __launch_bounds__(256)
+        div_kernel(int n, const double* __restrict__ x, const double* __restrict__ d,
+                   double* __restrict__ y)
+    {
+        for(int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+            y[i] = x[i] / d[i];
+    }
+
+    __global__ void __launch_bounds__(256)
+        div_scalar_kernel(int n, const double* __restrict__ x, double s, double* __restrict__ y)
+    {
+        for(int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+            y[i] = x[i] / s;
+    }
+
+    int div_vec(lal_context* c, int n, const double* x, const double* d, double* y)
+    {
+        if(n == 0)
+            return LAL_OK;
+        div_kernel<<<lal::grid_for(n, 256, 16384), 256, 0, c->stream>>>(n, x, d, y);
+        LAL_LAUNCHED();
+        return LAL_OK;
+    }
+
+    int div_scalar(lal_context* c, int n, const double* x, double s, double* y)
+    {
+        if(n == 0)
+            return LAL_OK;
+        div_scalar_kernel<<<lal::grid_for(n, 256, 16384), 256, 0, c->stream>>>(n, x, s, y);
+        LAL_LAUNCHED();
+        return LAL_OK;
+    }
+
+    // Stream-ordered work vectors released when the solver returns.
+    struct workspace
+    {
+        lal_context*         c;
+        std::vector<double*> bufs;
+        explicit workspace(lal_context* c_) : c(c_) {}
+        int get(double** p, size_t n)
+        {
+            *p = nullptr;
+            LAL_HIP(hipMallocAsync((void**)p, sizeof(double) * std::max<size_t>(n, 1), c->stream));
+            bufs.push_back(*p);
+            return LAL_OK;
+        }
+        ~workspace()
+        {
+            for(double* p : bufs)
+                (void)hipFreeAsync(p, c->stream);
+        }
+    };
+
+    struct precond_t
+    {
+        int           kind = 0; // 0 none, 1 Jacobi, 2 SA-AMG V-cycle
+        lal_hierarchy h    = nullptr;
+        const double* diag = nullptr;
+        int           n1 = 1, n2 = 1;
+
+        int apply(lal_context* c, int n, const double* r, double* z) const
+        {
+            if(kind == 1)
+                return div_vec(c, n, r, diag, z);
+            if(kind == 2)
+                return lal_amg_precond_f64(c, h, r, z, n1, n2);
+            return lal_memcpy_d2d(c, z, r, sizeof(double) * n);
+        }
+    };
+
+    struct op_t
+    {
+        lal_csrmv_descr d;
+        int             m, nnz;
+        const int*      rp;
+        const int*      ci;
+        const double*   v;
+        int             spmv(lal_context* c, const double* x, double* y) const
+        {
+            return lal_csrmv_f64(c, d, m, m, nnz, 1.0, rp, ci, v, x, 0.0, y);
+        }
+        int residual(lal_context* c, const double* x, const double* b, double* r) const
+        {
+            return lal_residual_f64(c, d, m, m, nnz, rp, ci, v, x, b, r);
+        }
+    };
+
+    int setup_precond(lal_context* c, workspace& ws, const op_t& A, int kind, lal_hierarchy h,
+                      const lal_solve_options& o, precond_t* P)
+    {
+        P->kind = kind;
+        P->h    = h;
+        P->n1   = o.n1;
+        P->n2   = o.n2;
+        if(kind == 1)
+        {
+            double* d = nullptr;
+            LAL_TRY(ws.get(&d, A.m));
+            LAL_TRY(lal_fill_f64(c, d, A.m, 0.0));
+            LAL_TRY(lal_diagonal_f64(c, A.m, A.rp, A.ci, A.v, d));
+            P->diag = d;
+        }
+        return LAL_OK;
+    }
+
+    int check_args(lal_context* c, int m, int nnz, const int* rp, const double* b, double* x,
+                   int precond, lal_hierarchy h)
+    {
+        LAL_REQUIRE(c != nullptr && m >= 1 && nnz >= 0 && rp != nullptr && b && x);
+        LAL_REQUIRE(precond >= 0 && precond <= 2);
+        LAL_REQUIRE(precond != 2 || h != nullptr);
+        if(precond == 2)
+        {
+            // the V-cycle reads and writes vectors of the finest level's size
+            int           hm = 0, hn = 0, hnnz = 0;
+            const int *   hrp = nullptr, *hci = nullptr;
+            const double* hv = nullptr;
+            LAL_TRY(lal_hierarchy_level(h, 0, 0, &hm, &hn, &hnnz, &hrp, &hci, &hv));
+            LAL_REQUIRE(hm == m && hnnz == nnz);
+        }
+        return LAL_OK;
+    }
+}
+
+extern "C" int lal_jacobi_solve_f64(lal_handle c, int n, const double* rhs, const double* diag,
+                                    double* x)
+{
+    LAL_REQUIRE(c != nullptr && n >= 0 && (n == 0 || (rhs && diag && x)));
+    return div_vec(c, n, rhs, diag, x);
+}
+
+extern "C" int lal_bicgstab_solve(lal_handle c, lal_csrmv_descr d, int m, int nnz,
+                                  const int* rp, const int* ci, const double* val,
+                                  const double* b, double* x, int precond, lal_hierarchy h,
+                                  const lal_solve_options* opts, lal_solve_info* info)
+{
+    LAL_TRY(check_args(c, m, nnz, rp, b, x, precond, h));
+    lal_solve_options o;
+    lal_solve_default_options(&o);
+    if(opts)
+        o = *opts;
+    LAL_REQUIRE(o.max_iter >= 0);
+    const op_t A{d, m, nnz, rp, ci, val};
+    workspace  ws(c);
+    double *   r, *r0, *p, *vv, *t, *z, *q;
+    for(double** buf : {&r, &r0, &p, &vv, &t, &z, &q})
+        LAL_TRY(ws.get(buf, m));
+    precond_t P;
+    LAL_TRY(setup_precond(c, ws, A, precond, h, o, &P));
+    const bool pre = precond != 0;
+
+    LAL_TRY(A.residual(c, x, b, r));
+    double init = 0.0, rn = 0.0, rho = 0.0;
+    LAL_TRY(lal_norm_inf_f64(c, m, r, &init));
+    rn = init;
+    LAL_TRY(lal_memcpy_d2d(c, r0, r, sizeof(double) * m));
+    LAL_TRY(lal_dot_f64(c, m, r0, r, &rho));
+    LAL_TRY(lal_memcpy_d2d(c, p, r, sizeof(double) * m));
+    if(pre)
+        LAL_TRY(P.apply(c, m, r, z));
+    int  iter = 0;
+    bool conv = false;
+    while(!(iter > o.max_iter))
+    {
+        const double* dir = pre ? z : p;
+        double*       Ad  = pre ? q : vv;
+        LAL_TRY(A.spmv(c, dir, Ad));
+        double den = 0.0;
+        LAL_TRY(lal_dot_f64(c, m, r0, Ad, &den));
+        const double alpha = rho / den;
+        LAL_TRY(lal_axpy_f64(c, m, -1.0 * alpha, Ad, r));
+        if(pre)
+            LAL_TRY(P.apply(c, m, r, vv));
+        const double* s = pre ? vv : r;
+        LAL_TRY(A.spmv(c, s, t));
+        double omega1 = 0.0, omega2 = 0.0;
+        LAL_TRY(lal_dot_f64(c, m, t, r, &omega1));
+        LAL_TRY(lal_dot_f64(c, m, t, t, &omega2));
+        if(omega1 == 0.0 || omega2 == 0.0)
+        {
+            LAL_TRY(lal_axpy_f64(c, m, alpha, p, x));
+            break;
+        }
+        const double omega = omega1 / omega2;
+        LAL_TRY(lal_axpbypgz_f64(c, m, alpha, dir, omega, s, 1.0, x));
+        LAL_TRY(lal_axpy_f64(c, m, -1.0 * omega, t, r));
+        LAL_TRY(lal_norm_inf_f64(c, m, r, &rn));
+        if(lal::converges(rn, init, o.rel_tol, o.abs_tol))
+        {
+            conv = true;
+            break;
+        }
+        const double rho_prev = rho;
+        LAL_TRY(lal_dot_f64(c, m, r0, r, &rho));
+        const double beta = (rho / rho_prev) * (alpha / omega);
+        // p = r - beta*omega*(A d) + beta*p, in place through a copy of p
+        LAL_TRY(lal_axpbypgz_f64(c, m, 1.0, r, -1.0 * beta * omega, Ad, beta, p));
+        if(pre)
+            LAL_TRY(P.apply(c, m, p, z));
+        iter++;
+    }
+    if(info)
+    {
+        info->iterations       = iter;
+        info->outer_iterations = 0;
+        info->converged        = conv;
+        info->initial_res_norm = init;
+        info->final_res_norm   = rn;
+    }
+    return LAL_OK;
+}
+
+extern "C" int lal_gmres_solve(lal_handle c, lal_csrmv_descr d, int m, int nnz, const int* rp,
+                               const int* ci, const double* val, const double* b, double* x,
+                               int restart, int precond, lal_hierarchy h,
+                               const lal_solve_options* opts, lal_solve_info* info)
+{
+    LAL_TRY(check_args(c, m, nnz, rp, b, x, precond, h));
+    LAL_REQUIRE(restart >= 1);
+    lal_solve_options o;
+    lal_solve_default_options(&o);
+    if(opts)
+        o = *opts;
+    LAL_REQUIRE(o.max_iter >= 0);
+    restart = std::max(1, std::min(restart, m - 1));
+    const int  ld = restart + 1;
+    const op_t A{d, m, nnz, rp, ci, val};
+    workspace  ws(c);
+    double *   Q, *w, *z, *res;
+    LAL_TRY(ws.get(&Q, (size_t)m * ld));
+    for(double** buf : {&w, &z, &res})
+        LAL_TRY(ws.get(buf, m));
+    precond_t P;
+    LAL_TRY(setup_precond(c, ws, A, precond, h, o, &P));
+    std::vector<double> H((size_t)ld * restart, 0.0), cs(restart, 0.0), sn(restart, 0.0),
+        y(ld, 0.0);
+    auto col = [&](int k) { return Q + (size_t)k * m; };
+
+    LAL_TRY(A.residual(c, x, b, res));
+    LAL_TRY(P.apply(c, m, res, z));
+    double vv = 0.0;
+    LAL_TRY(lal_dot_f64(c, m, z, z, &vv));
+    double rn   = std::sqrt(vv);
+    double init = rn;
+    int    iter = 0;
+    bool   conv = lal::converges(rn, init, o.rel_tol, o.abs_tol);
+    if(!conv)
+    {
+        LAL_TRY(div_scalar(c, m, z, rn, col(0)));
+        std::fill(y.begin(), y.end(), 0.0);
+        y[0] = rn;
+        while(!(iter > o.max_iter))
+        {
+            int k = 1;
+            for(k = 1; k < restart + 1; k++)
+            {
+                double* qk = col(k);
+                LAL_TRY(A.spmv(c, col(k - 1), w));
+                LAL_TRY(P.apply(c, m, w, qk));
+                for(int i = 0; i < k; i++)
+                {
+                    double hik = 0.0;
+                    LAL_TRY(lal_dot_f64(c, m, col(i), qk, &hik));
+                    H[i + (k - 1) * ld] = hik;
+                    LAL_TRY(lal_axpy_f64(c, m, -hik, col(i), qk));
+                }
+                LAL_TRY(lal_dot_f64(c, m, qk, qk, &vv));
+                H[k + (k - 1) * ld] = std::sqrt(vv);
+                LAL_TRY(div_scalar(c, m, qk, H[k + (k - 1) * ld], qk));
+                for(int i = 0; i < k - 1; i++)
+                {
+                    const double t1 = H[i + (k - 1) * ld], t2 = H[i + 1 + (k - 1) * ld];
+                    H[i + (k - 1) * ld]     = cs[i] * t1 + sn[i] * t2;
+                    H[i + 1 + (k - 1) * ld] = -sn[i] * t1 + cs[i] * t2;
+                }
+                const double xi = H[k - 1 + (k - 1) * ld], xj = H[k + (k - 1) * ld];
+                if(xi == 0.0)
+                {
+                    cs[k - 1] = 0.0;
+                    sn[k - 1] = 1.0;
+                }
+                else if(xj == 0.0)
+                {
+                    cs[k - 1] = 1.0;
+                    sn[k - 1] = 0.0;
+                }
+                else if(std::fabs(xi) > std::fabs(xj))
+                {
+                    const double t = xj / xi;
+                    cs[k - 1]      = 1.0 / std::sqrt(1.0 + t * t);
+                    sn[k - 1]      = t * cs[k - 1];
+                }
+                else
+                {
+                    const double t = xi / xj;
+                    sn[k - 1]      = 1.0 / std::sqrt(1.0 + t * t);
+                    cs[k - 1]      = t * sn[k - 1];
+                }
+                {
+                    const double t1 = H[k - 1 + (k - 1) * ld], t2 = H[k + (k - 1) * ld];
+                    H[k - 1 + (k - 1) * ld] = cs[k - 1] * t1 + sn[k - 1] * t2;
+                    H[k + (k - 1) * ld]     = -sn[k - 1] * t1 + cs[k - 1] * t2;
+                }
+                y[k]     = -sn[k - 1] * y[k - 1];
+                y[k - 1] = cs[k - 1] * y[k - 1];
+                if(lal::converges(std::fabs(y[k]), init, o.rel_tol, o.abs_tol) || k == restart)
+                    break;
+            }
+            for(int i = k - 1; i >= 0; i--)
+            {
+                for(int j = i + 1; j < k; j++)
+                    y[i] = y[i] - H[i + ld * j] * y[j];
+                y[i] = y[i] / H[i + ld * i];
+            }
+            for(int j = 0; j < k; j++)
+                LAL_TRY(lal_axpy_f64(c, m, y[j], col(j), x));
+            LAL_TRY(A.residual(c, x, b, res));
+            LAL_TRY(P.apply(c, m, res, z));
+            LAL_TRY(lal_dot_f64(c, m, z, z, &vv));
+            rn = std::sqrt(vv);
+            if(lal::converges(rn, init, o.rel_tol, o.abs_tol))
+            {
+                conv = true;
+                break;
+            }
+            LAL_TRY(div_scalar(c, m, z, rn, col(0)));
+            std::fill(y.begin(), y.end(), 0.0);
+            y[0] = rn;
+            iter++;
+        }
+    }
+    if(info)
+    {
+        info->iterations       = iter;
+        info->outer_iterations = 0;
+        info->converged        = conv;
+        info->initial_res_norm = init;
+        info->final_res_norm   = rn;
+    }
+    return LAL_OK;
+}

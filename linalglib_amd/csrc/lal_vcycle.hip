@@ -145,7 +145,57 @@ namespace
                          const T* __restrict__ val, const T* __restrict__ b, T* __restrict__ x,
                          int sweeps, int zero)
     {
-        if(threadIdx.x != 0)
+        // The sweeps are one lane's sequential loop; when the operator, x and
+        // b fit, the workgroup first stages them in LDS so that the loop's
+        // dependent loads are LDS reads instead of L2 round trips.
+        constexpr int          CGS_LDS = 40 * 1024;
+        __shared__ __align__(16) unsigned char s_mem[CGS_LDS];
+        const int              nnz  = rp[n];
+        const size_t           need = sizeof(T) * ((size_t)nnz + 2 * (size_t)n)
+                            + sizeof(int) * ((size_t)n + 1 + (size_t)nnz);
+        const int tid = threadIdx.x;
+        if(need <= CGS_LDS)
+        {
+            T*   s_v  = reinterpret_cast<T*>(s_mem);
+            T*   s_x  = s_v + nnz;
+            T*   s_b  = s_x + n;
+            int* s_rp = reinterpret_cast<int*>(s_b + n);
+            int* s_ci = s_rp + n + 1;
+            for(int k = tid; k < nnz; k += blockDim.x)
+            {
+                s_v[k]  = val[k];
+                s_ci[k] = ci[k];
+            }
+            for(int j = tid; j < n; j += blockDim.x)
+            {
+                s_x[j] = zero ? T(0) : x[j];
+                s_b[j] = b[j];
+            }
+            for(int j = tid; j <= n; j += blockDim.x)
+                s_rp[j] = rp[j];
+            __syncthreads();
+            if(tid == 0)
+                for(int it = 0; it < sweeps; it++)
+                    for(int j = 0; j < n; j++)
+                    {
+                        T sigma = 0, ajj = 0;
+                        for(int k = s_rp[j]; k < s_rp[j + 1]; k++)
+                        {
+                            const int col = s_ci[k];
+                            const T   v   = s_v[k];
+                            if(col != j)
+                                sigma = sigma + v * s_x[col];
+                            else
+                                ajj = v;
+                        }
+                        s_x[j] = (s_b[j] - sigma) / ajj;
+                    }
+            __syncthreads();
+            for(int j = tid; j < n; j += blockDim.x)
+                x[j] = s_x[j];
+            return;
+        }
+        if(tid != 0)
             return;
         if(zero)
             for(int j = 0; j < n; j++)

@@ -109,6 +109,11 @@ _SIGS = {
     "lal_amg_solve": [vp, vp, vp, vp, i32, i32, i32, i32, C.POINTER(SolveOptions),
                       C.POINTER(SolveInfo)],
     "lal_sddmm_f64": [vp, i32, i32, vp, vp, f64, vp, vp, vp],
+    "lal_jacobi_solve_f64": [vp, i32, vp, vp, vp],
+    "lal_bicgstab_solve": [vp, vp, i32, i32, vp, vp, vp, vp, vp, i32, vp,
+                           C.POINTER(SolveOptions), C.POINTER(SolveInfo)],
+    "lal_gmres_solve": [vp, vp, i32, i32, vp, vp, vp, vp, vp, i32, i32, vp,
+                        C.POINTER(SolveOptions), C.POINTER(SolveInfo)],
     "lal_adjoint_solve": [vp, vp, vp, vp, i32, C.POINTER(SolveOptions), C.POINTER(SolveInfo)],
     "lal_status_string": [i32],
     "lal_last_error": [],
@@ -329,3 +334,43 @@ class CsrDescr:
         if getattr(self, "d", None):
             load().lal_csrmv_descr_destroy(self.d)
             self.d = None
+
+
+PRECONDS = {"none": 0, "jacobi": 1, "saamg": 2}
+
+
+def _krylov_args(A, precond, hier):
+    if precond not in PRECONDS:
+        raise ValueError(f"precond must be one of {sorted(PRECONDS)}")
+    if precond == "saamg" and hier is None:
+        raise ValueError("precond='saamg' needs the Hierarchy built for A")
+    rp, ci, val, m, nnz, descr = A
+    return (ptr(rp), ptr(ci), ptr(val), m, nnz, descr.d if descr is not None else None,
+            PRECONDS[precond], hier.hp if (hier is not None and precond == "saamg") else None)
+
+
+def jacobi_solve(handle, rhs, diag, x):
+    """x = rhs / diag elementwise (LinAlgLib's jacobi_precond::solve)."""
+    call("lal_jacobi_solve_f64", handle.h, rhs.numel(), ptr(rhs), ptr(diag), ptr(x))
+
+
+def bicgstab(handle, A, b, x, precond="none", hier=None, **opts):
+    """BiCGSTAB on A = (row_ptr, col_ind, val, m, nnz, CsrDescr or None); x is
+    the initial guess and the result.  precond: "none", "jacobi" or "saamg"
+    (one V-cycle of `hier`, opts n1/n2 sweeps)."""
+    rp, ci, val, m, nnz, d, kind, hp = _krylov_args(A, precond, hier)
+    o = solve_options(**opts)
+    info = SolveInfo()
+    call("lal_bicgstab_solve", handle.h, d, m, nnz, rp, ci, val, ptr(b), ptr(x), kind, hp,
+         C.byref(o), C.byref(info))
+    return info
+
+
+def gmres(handle, A, b, x, restart=30, precond="none", hier=None, **opts):
+    """Restarted, left-preconditioned GMRES(restart); arguments as bicgstab."""
+    rp, ci, val, m, nnz, d, kind, hp = _krylov_args(A, precond, hier)
+    o = solve_options(**opts)
+    info = SolveInfo()
+    call("lal_gmres_solve", handle.h, d, m, nnz, rp, ci, val, ptr(b), ptr(x), int(restart), kind,
+         hp, C.byref(o), C.byref(info))
+    return info

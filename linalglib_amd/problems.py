@@ -10,6 +10,10 @@ files), so these are this package's own.  Every generator returns
 * ``diffusion3d``: 7-point finite-volume diffusion with per-face
   conductances; ``cell_kappa`` gives harmonic face averaging
   ``2 k_p k_q / (k_p + k_q)`` and Dirichlet faces use ``k_p``.
+* ``convection_diffusion2d``: the nonsymmetric 5-point operator
+  ``-lap u + w . grad u`` on the unit square (mesh width 1/(n+1), first-order
+  upwinding of the convection term): an M-matrix, for the nonsymmetric
+  Krylov solvers.
 """
 import numpy as np
 
@@ -273,6 +277,39 @@ def assemble_faces(nnz, entry, face, sign, k):
     val = np.zeros(nnz)
     np.add.at(val, entry, sign * k[face])
     return val
+
+
+def convection_diffusion2d(nx, wind=(20.0, -10.0)):
+    """-lap u + wind . grad u on an nx x nx grid, Dirichlet boundaries by
+    truncation; each convection term is upwinded (backward difference for a
+    positive wind component, forward for a negative one), so off-diagonals
+    are negative and rows are weakly diagonally dominant."""
+    n = nx
+    hh = 1.0 / (n + 1)
+    m = n * n
+    wx, wy = wind
+    i = np.tile(np.arange(n), n)
+    j = np.repeat(np.arange(n), n)
+    d = 1.0 / hh**2
+    # (column offset, coefficient, present) in ascending column order
+    terms = [(-n, -d - max(wy, 0.0) / hh, j > 0),
+             (-1, -d - max(wx, 0.0) / hh, i > 0),
+             (0, 4.0 * d + abs(wx) / hh + abs(wy) / hh, np.ones(m, bool)),
+             (1, -d + min(wx, 0.0) / hh, i < n - 1),
+             (n, -d + min(wy, 0.0) / hh, j < n - 1)]
+    counts = sum(t[2].astype(np.int64) for t in terms)
+    row_ptr = np.zeros(m + 1, np.int64)
+    np.cumsum(counts, out=row_ptr[1:])
+    col_ind = np.empty(row_ptr[-1], np.int32)
+    val = np.empty(row_ptr[-1])
+    pos = row_ptr[:-1].copy()
+    rows = np.arange(m)
+    for off, coef, present in terms:
+        r = rows[present]
+        col_ind[pos[r]] = r + off
+        val[pos[r]] = coef
+        pos[r] += 1
+    return row_ptr.astype(np.int32), col_ind, val, m
 
 
 def rhs_from_solution(row_ptr, col_ind, val, x):

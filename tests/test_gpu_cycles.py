@@ -101,3 +101,26 @@ def test_amg_solve_argument_checks(h):
         with pytest.raises(lal.LalError):
             lal.call("lal_amg_solve", h.h, H.hp, lal.ptr(B), lal.ptr(x), 1, 1, cyc, sm, None,
                      None)
+
+
+def test_amg_solve_gauss_seidel_bitwise_oracle_large(h, oracle):
+    """3D Poisson 40^3 (64000 rows, sets of ~540 rows on the finest level, so
+    every level is swept by one workgroup and the finest level's x is larger
+    than a workgroup's LDS): V and W cycles with Gauss-Seidel and SSOR in
+    exact mode, bit-identical to the oracle after three cycles."""
+    rp, ci, v, m = pb.poisson3d(40)
+    A = CSR(rp, ci, v, m, m)
+    H = _hier(h, A, exact=1)
+    ho = oracle.saamg_setup(A)
+    b = pb.rhs_from_solution(rp, ci, v, pb.uniform(m, seed=6))
+    B = dev(b)
+    for cyc in (0, 1):
+        for sm in (1, 4):
+            log(f"amg_solve 40^3 exact cycle={CYC[cyc]} smoother={SM[sm]}")
+            x = torch.zeros(m, dtype=torch.float64, device="cuda")
+            info = H.amg_solve(B, x, 1, 1, CYC[cyc], SM[sm], rel_tol=1e-300, abs_tol=1e-300,
+                               max_iter=2)
+            xo, ko, _ = ho.amg_solve(b, n1=1, n2=1, cycle=cyc, smoother=sm, rel_tol=1e-300,
+                                     max_cycle=2)
+            assert info.iterations == ko == 3, (info.iterations, ko)
+            assert bits_equal(host(x), xo), (cyc, sm)
