@@ -321,8 +321,12 @@ def run_krylov(args):
         torch.cuda.synchronize()
         sec = time.perf_counter() - t0
         rec = {"solve_s": round(sec, 5), "iterations": info.iterations,
-               "converged": bool(info.converged),
-               "s_per_iteration": round(sec / (info.iterations + 1), 7)}
+               "converged": bool(info.converged)}
+        if "gmres" in tag:
+            rec["arnoldi_steps"] = info.outer_iterations
+            rec["s_per_arnoldi_step"] = round(sec / max(info.outer_iterations, 1), 7)
+        else:
+            rec["s_per_iteration"] = round(sec / (info.iterations + 1), 7)
         line["gpu"][tag] = rec
         print(f"# krylov {tag}: {rec}", file=sys.stderr, flush=True)
 

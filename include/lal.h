@@ -303,7 +303,8 @@ int lal_bicgstab_solve(lal_handle h, lal_csrmv_descr descr, int m, int nnz, cons
                        lal_solve_info* info);
 /* Restarted, left-preconditioned GMRES(restart) (restart clamped to
  * [1, m-1]); convergence on the 2-norm of the preconditioned residual;
- * info->iterations counts restart cycles as gmres.cpp does.  The
+ * info->iterations counts restart cycles as gmres.cpp does and
+ * info->outer_iterations the Arnoldi steps (preconditioned products) taken.  The
  * preconditioned Arnoldi vector is kept and every work vector sized, where
  * gmres.cpp drops the one and leaves the others unsized. */
 int lal_gmres_solve(lal_handle h, lal_csrmv_descr descr, int m, int nnz, const int* row_ptr,

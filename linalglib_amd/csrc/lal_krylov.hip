@@ -14,9 +14,11 @@
 //                       vectors; here q_k = M^-1 A q_(k-1) is kept and every
 //                       work vector is sized.
 //
-// The vectors stay on the device; the solvers' scalars (inner products,
-// norms, the small Hessenberg system) come to the host as in the reference's
-// loops.  Work vectors are stream-ordered allocations of the call.
+// The vectors stay on the device.  BiCGSTAB's scalars come to the host as in
+// the reference's loop; GMRES keeps its Gram-Schmidt inner products on the
+// device and copies one Hessenberg column per Arnoldi step to the host, where
+// the Givens rotations and the small triangular solve run.  Work vectors are
+// stream-ordered allocations of the call.
 #include "lal_hier.h"
 
 #include <cmath>
@@ -55,6 +57,71 @@ namespace
         div_scalar_kernel<<<lal::grid_for(n, 256, 16384), 256, 0, c->stream>>>(n, x, s, y);
         LAL_LAUNCHED();
         return LAL_OK;
+    }
+
+    // Modified Gram-Schmidt on the device: the inner products stay on the
+    // device as lal::PCG_BLOCKS partials that every block of the next kernel
+    // sums in the same order, so a whole Arnoldi step needs one copy of its
+    // Hessenberg column to the host.
+    constexpr int KT = 256;
+    constexpr int KB = lal::PCG_BLOCKS;
+
+    __device__ double block_sum(double v)
+    {
+        __shared__ double sh[KT / 64];
+        v = lal::wave_sum(v);
+        __syncthreads();
+        if((threadIdx.x & 63) == 0)
+            sh[threadIdx.x >> 6] = v;
+        __syncthreads();
+        double t = sh[0];
+#pragma unroll
+        for(int w = 1; w < KT / 64; w++)
+            t = t + sh[w];
+        return t;
+    }
+
+    __device__ double sum_parts(const double* __restrict__ part)
+    {
+        double v = 0.0;
+        for(int k = threadIdx.x; k < KB; k += KT)
+            v += part[k];
+        return block_sum(v);
+    }
+
+    // h = <q_i, q_k> from part_in (written to *h_out by block 0);
+    // q_k = q_k - h q_i; part_out = partials of <q_next, q_k>, or of
+    // <q_k, q_k> when q_next is null (the norm that ends the step).
+    __global__ void __launch_bounds__(KT)
+        mgs_kernel(int n, const double* __restrict__ part_in, const double* __restrict__ qi,
+                   const double* __restrict__ qnext, double* __restrict__ qk,
+                   double* __restrict__ part_out, double* __restrict__ h_out)
+    {
+        const double h = sum_parts(part_in);
+        if(blockIdx.x == 0 && threadIdx.x == 0)
+            *h_out = h;
+        double acc = 0.0;
+        for(int i = blockIdx.x * KT + threadIdx.x; i < n; i += KB * KT)
+        {
+            const double q = qk[i] - h * qi[i];
+            qk[i]          = q;
+            acc            = acc + (qnext ? qnext[i] : q) * q;
+        }
+        const double t = block_sum(acc);
+        if(threadIdx.x == 0)
+            part_out[blockIdx.x] = t;
+    }
+
+    // h = sqrt(<q_k, q_k>) from part (to *h_out); q_k = q_k / h.
+    __global__ void __launch_bounds__(KT)
+        normalize_kernel(int n, const double* __restrict__ part, double* __restrict__ qk,
+                         double* __restrict__ h_out)
+    {
+        const double h = sqrt(sum_parts(part));
+        if(blockIdx.x == 0 && threadIdx.x == 0)
+            *h_out = h;
+        for(int i = blockIdx.x * KT + threadIdx.x; i < n; i += KB * KT)
+            qk[i] = qk[i] / h;
     }
 
     // Stream-ordered work vectors released when the solver returns.
@@ -252,10 +319,12 @@ extern "C" int lal_gmres_solve(lal_handle c, lal_csrmv_descr d, int m, int nnz, 
     const int  ld = restart + 1;
     const op_t A{d, m, nnz, rp, ci, val};
     workspace  ws(c);
-    double *   Q, *w, *z, *res;
+    double *   Q, *w, *z, *res, *dH, *part;
     LAL_TRY(ws.get(&Q, (size_t)m * ld));
     for(double** buf : {&w, &z, &res})
         LAL_TRY(ws.get(buf, m));
+    LAL_TRY(ws.get(&dH, (size_t)ld * restart));
+    LAL_TRY(ws.get(&part, 2 * (size_t)KB));
     precond_t P;
     LAL_TRY(setup_precond(c, ws, A, precond, h, o, &P));
     std::vector<double> H((size_t)ld * restart, 0.0), cs(restart, 0.0), sn(restart, 0.0),
@@ -266,9 +335,10 @@ extern "C" int lal_gmres_solve(lal_handle c, lal_csrmv_descr d, int m, int nnz, 
     LAL_TRY(P.apply(c, m, res, z));
     double vv = 0.0;
     LAL_TRY(lal_dot_f64(c, m, z, z, &vv));
-    double rn   = std::sqrt(vv);
-    double init = rn;
-    int    iter = 0;
+    double rn    = std::sqrt(vv);
+    double init  = rn;
+    int    iter  = 0;
+    int    steps = 0;
     bool   conv = lal::converges(rn, init, o.rel_tol, o.abs_tol);
     if(!conv)
     {
@@ -281,18 +351,28 @@ extern "C" int lal_gmres_solve(lal_handle c, lal_csrmv_descr d, int m, int nnz, 
             for(k = 1; k < restart + 1; k++)
             {
                 double* qk = col(k);
+                steps++;
                 LAL_TRY(A.spmv(c, col(k - 1), w));
                 LAL_TRY(P.apply(c, m, w, qk));
+                // h_ik = <q_i, q_k>, q_k -= h_ik q_i for i < k, in order, then
+                // h_k,k-1 = ||q_k|| and q_k /= h_k,k-1, all on the device
+                double* hcol = dH + (size_t)(k - 1) * ld;
+                LAL_TRY(lal::dot_partials<double, double>(c, m, col(0), qk, part));
                 for(int i = 0; i < k; i++)
                 {
-                    double hik = 0.0;
-                    LAL_TRY(lal_dot_f64(c, m, col(i), qk, &hik));
-                    H[i + (k - 1) * ld] = hik;
-                    LAL_TRY(lal_axpy_f64(c, m, -hik, col(i), qk));
+                    const double* pin  = part + (size_t)(i & 1) * KB;
+                    double*       pout = part + (size_t)((i + 1) & 1) * KB;
+                    mgs_kernel<<<KB, KT, 0, c->stream>>>(m, pin, col(i),
+                                                         i + 1 < k ? col(i + 1) : nullptr, qk,
+                                                         pout, hcol + i);
+                    LAL_LAUNCHED();
                 }
-                LAL_TRY(lal_dot_f64(c, m, qk, qk, &vv));
-                H[k + (k - 1) * ld] = std::sqrt(vv);
-                LAL_TRY(div_scalar(c, m, qk, H[k + (k - 1) * ld], qk));
+                normalize_kernel<<<KB, KT, 0, c->stream>>>(m, part + (size_t)(k & 1) * KB, qk,
+                                                           hcol + k);
+                LAL_LAUNCHED();
+                LAL_HIP(hipMemcpyAsync(&H[(size_t)(k - 1) * ld], hcol, sizeof(double) * (k + 1),
+                                       hipMemcpyDeviceToHost, c->stream));
+                LAL_HIP(hipStreamSynchronize(c->stream));
                 for(int i = 0; i < k - 1; i++)
                 {
                     const double t1 = H[i + (k - 1) * ld], t2 = H[i + 1 + (k - 1) * ld];
@@ -358,7 +438,7 @@ extern "C" int lal_gmres_solve(lal_handle c, lal_csrmv_descr d, int m, int nnz, 
     if(info)
     {
         info->iterations       = iter;
-        info->outer_iterations = 0;
+        info->outer_iterations = steps;
         info->converged        = conv;
         info->initial_res_norm = init;
         info->final_res_norm   = rn;
