@@ -9,6 +9,12 @@
  * (oracle/Makefile): -O3 -fopenmp -ffp-contract=off, no -march, so that every
  * product and sum is rounded exactly as the reference's host build rounds it.
  * Only explicit std::fma in the reference becomes fma() here.
+ *
+ * Every function is checked by tests/test_oracle.py against the reference's
+ * golden vectors, the compiled reference or closed forms.  or_gmres is checked
+ * by closed forms only (exactness after m - 1 Arnoldi steps, agreement with
+ * scipy's GMRES and with direct solves): the reference's gmres_solver cannot
+ * run, as its build() leaves the residual vectors unsized.
  */
 #include "lal_oracle.h"
 
