@@ -126,7 +126,7 @@ namespace
         if(d && !d->exact)
         {
             lal::rv_args a{m,    d->nnz, d->by_nnz, d->d_blocks, d->nblocks,     rp,           ci, v, x, nullptr,
-                           b,    y,           nullptr,        (double)alpha, (double)beta};
+                           b,    y,           nullptr,        (double)alpha, (double)beta, d->win};
             return lal::rowvec_launch<T>(c, MODE == 0 ? lal::RV_SPMV : lal::RV_RESIDUAL, a);
         }
         int nb = lal::spmv_grid(d, m);
@@ -216,12 +216,29 @@ namespace lal
         d->exact   = exact;
         d->by_nnz  = m != n;
         d->nblocks = (int)blocks.size() - 1;
-        if(hipMalloc(&d->d_blocks, sizeof(int) * blocks.size()) != hipSuccess)
+        // the row offsets, then (16-byte aligned) one {r0, r1, row_ptr[r0],
+        // row_ptr[r1]} quadruple per block: a block learns its rows and its
+        // nonzero window from a single load
+        const size_t meta_at = lal::block_meta_offset(d->nblocks);
+        std::vector<int> packed(meta_at + 4 * (size_t)d->nblocks);
+        std::copy(blocks.begin(), blocks.end(), packed.begin());
+        int win = 4;
+        for(int k = 0; k < d->nblocks; k++)
+        {
+            const int span = ((h[blocks[k + 1]] + 3) & ~3) - (h[blocks[k]] & ~3);
+            win            = std::max(win, std::min(span, 2052));
+            packed[meta_at + 4 * k + 0] = blocks[k];
+            packed[meta_at + 4 * k + 1] = blocks[k + 1];
+            packed[meta_at + 4 * k + 2] = h[blocks[k]];
+            packed[meta_at + 4 * k + 3] = h[blocks[k + 1]];
+        }
+        d->win = win;
+        if(hipMalloc(&d->d_blocks, sizeof(int) * packed.size()) != hipSuccess)
         {
             delete d;
             return LAL_ERR_OOM;
         }
-        LAL_HIP(hipMemcpyAsync(d->d_blocks, blocks.data(), sizeof(int) * blocks.size(),
+        LAL_HIP(hipMemcpyAsync(d->d_blocks, packed.data(), sizeof(int) * packed.size(),
                                hipMemcpyHostToDevice, c->stream));
         LAL_HIP(hipStreamSynchronize(c->stream));
         *out = d;

@@ -60,6 +60,7 @@ namespace lal
         void*       r;
         double      alpha;
         double      beta;
+        int         win; // LDS window in entries (lal_csrmv_info::win)
     };
 
     template <typename T>

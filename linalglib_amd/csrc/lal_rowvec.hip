@@ -208,8 +208,9 @@ namespace
     //
     // Phase 1 copies the block's column indices and values into LDS with
     // 16-byte non-temporal loads, four consecutive nonzeros per lane, over a
-    // window aligned down to a multiple of four (WIN = CAP + 4 entries, so a
-    // block of at most CAP nonzeros always fits one window).  Phase 2 lets each
+    // window aligned down to a multiple of four (WIN, a multiple of four, is
+    // the widest such window of the matrix's blocks and at most CAP + 4
+    // entries, so a block of at most CAP nonzeros always fits one window).  Phase 2 lets each
     // row (one lane, or G lanes for short blocks) walk its own nonzeros in LDS
     // and gather x itself: one gather instruction then covers the same
     // stencil offset of 64 consecutive rows, a few cache lines, instead of
@@ -218,28 +219,30 @@ namespace
     // one on CDNA.
     template <typename T, int MODE>
     __global__ void __launch_bounds__(SPMV_THREADS)
-        quad_kernel(int m, int nnz, const int* __restrict__ blocks, int nb,
+        quad_kernel(int m, int nnz, const int* __restrict__ blocks, int nb, int win,
                     const int* __restrict__ rp, const int* __restrict__ ci,
                     const T* __restrict__ val, const T* __restrict__ x,
                     const T* __restrict__ d, const T* __restrict__ b, T alpha, T beta,
                     T* __restrict__ y, T* __restrict__ r)
     {
-        constexpr int WIN = 2048 + 4;
-        __shared__ __attribute__((aligned(16))) int s_c[WIN];
-        __shared__ __attribute__((aligned(16))) T   s_v[WIN];
+        // win entries of columns, then win values (win: a multiple of 4)
+        extern __shared__ __attribute__((aligned(16))) unsigned char s_raw[];
+        int* s_c = reinterpret_cast<int*>(s_raw);
+        T*   s_v = reinterpret_cast<T*>(s_raw + sizeof(int) * win);
+        const int WIN = win;
 
-        const int tid  = threadIdx.x;
-        const int tile = lal::xcd_tile(blockIdx.x, nb);
-        int       r0, r1;
-        lal::block_rows(blocks, m, tile, &r0, &r1);
+        const int   tid  = threadIdx.x;
+        const int   tile = lal::xcd_tile(blockIdx.x, nb);
+        const int4v mt   = reinterpret_cast<const int4v*>(blocks + lal::block_meta_offset(nb))[tile];
+        const int   r0 = mt.x, r1 = mt.y;
         const int  nrows  = r1 - r0;
         const int  g      = lal::spmv_group(nrows, 0);
         const int  lrow   = tid / g;
         const int  lig    = tid & (g - 1);
         const bool active = lrow < nrows;
         const int  row    = r0 + lrow;
-        const int  base   = rp[r0];
-        const int  end    = rp[r1];
+        const int  base   = mt.z;
+        const int  end    = mt.w;
         const int  rs     = active ? rp[row] : 0;
         const int  re     = active ? rp[row + 1] : 0;
         const int  nnz4   = nnz & ~3;
@@ -301,27 +304,28 @@ namespace
     // leave lanes of the per-row gather idle.
     template <typename T, int MODE>
     __global__ void __launch_bounds__(SPMV_THREADS)
-        quad_nnz_kernel(int m, int nnz, const int* __restrict__ blocks, int nb,
+        quad_nnz_kernel(int m, int nnz, const int* __restrict__ blocks, int nb, int win,
                         const int* __restrict__ rp, const int* __restrict__ ci,
                         const T* __restrict__ val, const T* __restrict__ x,
                         const T* __restrict__ d, const T* __restrict__ b, T alpha, T beta,
                         T* __restrict__ y, T* __restrict__ r)
     {
-        constexpr int WIN = 2048 + 4;
-        __shared__ __attribute__((aligned(16))) T s_p[WIN];
+        extern __shared__ __attribute__((aligned(16))) unsigned char s_raw[];
+        T*        s_p = reinterpret_cast<T*>(s_raw);
+        const int WIN = win;
 
-        const int tid  = threadIdx.x;
-        const int tile = lal::xcd_tile(blockIdx.x, nb);
-        int       r0, r1;
-        lal::block_rows(blocks, m, tile, &r0, &r1);
+        const int   tid  = threadIdx.x;
+        const int   tile = lal::xcd_tile(blockIdx.x, nb);
+        const int4v mt   = reinterpret_cast<const int4v*>(blocks + lal::block_meta_offset(nb))[tile];
+        const int   r0 = mt.x, r1 = mt.y;
         const int  nrows  = r1 - r0;
         const int  g      = lal::spmv_group(nrows, 0);
         const int  lrow   = tid / g;
         const int  lig    = tid & (g - 1);
         const bool active = lrow < nrows;
         const int  row    = r0 + lrow;
-        const int  base   = rp[r0];
-        const int  end    = rp[r1];
+        const int  base   = mt.z;
+        const int  end    = mt.w;
         const int  rs     = active ? rp[row] : 0;
         const int  re     = active ? rp[row + 1] : 0;
         const int  nnz4   = nnz & ~3;
@@ -372,13 +376,16 @@ namespace
             return LAL_OK;
         const bool aligned = ((reinterpret_cast<uintptr_t>(a.ci) |
                                reinterpret_cast<uintptr_t>(a.val)) & 15) == 0;
+        // LDS sized to the widest block window of this matrix, so that short
+        // blocks leave room for more resident workgroups
+        const int win = a.win;
         if(aligned && a.blocks && a.by_nnz)
-            quad_nnz_kernel<T, MODE><<<nb, SPMV_THREADS, 0, c->stream>>>(
-                a.m, a.nnz, a.blocks, nb, a.rp, a.ci, (const T*)a.val, (const T*)a.x,
+            quad_nnz_kernel<T, MODE><<<nb, SPMV_THREADS, sizeof(T) * win, c->stream>>>(
+                a.m, a.nnz, a.blocks, nb, win, a.rp, a.ci, (const T*)a.val, (const T*)a.x,
                 (const T*)a.d, (const T*)a.b, (T)a.alpha, (T)a.beta, (T*)a.y, (T*)a.r);
         else if(aligned && a.blocks)
-            quad_kernel<T, MODE><<<nb, SPMV_THREADS, 0, c->stream>>>(
-                a.m, a.nnz, a.blocks, nb, a.rp, a.ci, (const T*)a.val, (const T*)a.x,
+            quad_kernel<T, MODE><<<nb, SPMV_THREADS, (sizeof(int) + sizeof(T)) * win, c->stream>>>(
+                a.m, a.nnz, a.blocks, nb, win, a.rp, a.ci, (const T*)a.val, (const T*)a.x,
                 (const T*)a.d, (const T*)a.b, (T)a.alpha, (T)a.beta, (T*)a.y, (T*)a.r);
         else
             staged_kernel<T, MODE><<<nb, SPMV_THREADS, 0, c->stream>>>(

@@ -20,11 +20,21 @@ struct lal_csrmv_info
     int  exact    = 0;       // 1: sequential row sums (bit-identical to the host)
     int  by_nnz   = 0;       // fast path: lanes gather x over consecutive nonzeros
                              // (rectangular transfer operators) instead of per row
-    int* d_blocks = nullptr; // nblocks + 1 row offsets
+    int  win      = 4;       // widest 4-aligned nonzero window of a block (<= 2052)
+    int* d_blocks = nullptr; // nblocks + 1 row offsets, then per block
+                             // {r0, r1, row_ptr[r0], row_ptr[r1]} at
+                             // lal::block_meta_offset(nblocks)
 };
 
 namespace lal
 {
+    // Position (in ints, a multiple of 4) of the per-block quadruples in the
+    // descriptor's block array.
+    __host__ __device__ inline size_t block_meta_offset(int nblocks)
+    {
+        return ((size_t)nblocks + 1 + 3) & ~(size_t)3;
+    }
+
     constexpr int SPMV_THREADS = 256;
 
     template <typename T>

@@ -466,7 +466,7 @@ namespace lal
         if(d && !d->exact)
         {
             rv_args a{m, d->nnz, d->by_nnz, d->d_blocks, d->nblocks, rp, ci, v, xold, dinv, b, xnew, nullptr, 0.0,
-                      0.0};
+                      0.0, d->win};
             return rowvec_launch<T>(c, RV_JACOBI, a);
         }
         int nb = spmv_grid(d, m);
@@ -487,7 +487,7 @@ namespace lal
             return LAL_OK;
         if(d && !d->exact)
         {
-            rv_args a{m, d->nnz, d->by_nnz, d->d_blocks, d->nblocks, rp, ci, v, nullptr, dinv, b, x, r, 0.0, 0.0};
+            rv_args a{m, d->nnz, d->by_nnz, d->d_blocks, d->nblocks, rp, ci, v, nullptr, dinv, b, x, r, 0.0, 0.0, d->win};
             return rowvec_launch<T>(c, RV_JACOBI0, a);
         }
         int nb = spmv_grid(d, m);
@@ -597,7 +597,7 @@ namespace lal
     {
         if(m == 0)
             return LAL_OK;
-        rv_args a{m, d->nnz, d->by_nnz, d->d_blocks, d->nblocks, rp, ci, v, r, dinv_c, nullptr, b_c, x_c, 1.0, 0.0};
+        rv_args a{m, d->nnz, d->by_nnz, d->d_blocks, d->nblocks, rp, ci, v, r, dinv_c, nullptr, b_c, x_c, 1.0, 0.0, d->win};
         return rowvec_launch<T>(c, RV_RESTRICT, a);
     }
 
