@@ -32,6 +32,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <string>
 #include <utility>
 #include <vector>
 
@@ -355,6 +356,11 @@ namespace lal_linalg
         }
 
     public:
+        // The SpMV analysis of this pattern (built on first use).
+        lal_csrmv_descr descriptor() const
+        {
+            return analysed();
+        }
         csr_matrix() = default;
         csr_matrix(const std::vector<int>& row_ptr, const std::vector<int>& col_ind,
                    const std::vector<double>& val, int m_, int n_, int nnz_)
@@ -541,13 +547,20 @@ namespace lal_linalg
     };
 
     // --------------------------------------------------------------------- AMG
+    // include/iterative_solvers/amg/amg.h:155-243
     enum class cycle
     {
-        vcycle
+        vcycle,
+        wcycle,
+        fcycle
     };
     enum class smoother
     {
-        jacobi
+        jacobi,
+        gauss_seidel,
+        symmetric_gauss_seidel,
+        SOR,
+        SSOR
     };
 
     struct hierarchy
@@ -583,36 +596,38 @@ namespace lal_linalg
         check(lal_hierarchy_levels(h.h, &h.total_levels), "saamg_setup");
     }
 
-    // amg.cpp:516-571 as a solver: cycles x += M (b - A x) until converged or
-    // max_cycle; returns the cycle count.
-    inline int amg_solve(const hierarchy& h, vector<double>& x, const vector<double>& b, int n1,
-                         int n2, cycle, smoother, iter_control control)
+    // uaamg.cpp:125-224 on the device (eps from 0.001, one 1.0 per row of P).
+    inline void uaamg_setup(const csr_matrix<double>& A, int max_level, hierarchy& h)
     {
-        int            m = (int)b.get_size();
-        vector<double> r(m), z(m);
-        int            mm, nn, nnz;
-        const int *    arp, *aci;
-        const double*  av;
-        check(lal_hierarchy_level(h.h, 0, 0, &mm, &nn, &nnz, &arp, &aci, &av), "amg_solve");
-        auto residual = [&] {
-            check(lal_residual_f64(context(), nullptr, mm, nn, nnz, arp, aci, av, x.get_vec(),
-                                   b.get_vec(), r.get_vec()),
-                  "amg_solve");
-        };
-        residual();
-        double init = norm_inf(r);
-        int    cycle_count = 0;
-        while(!control.exceed_max_cycle(cycle_count))
-        {
-            check(lal_amg_precond_f64(context(), h.h, r.get_vec(), z.get_vec(), n1, n2),
-                  "amg_solve");
-            axpy(1.0, z, x);
-            residual();
-            if(control.residual_converges(norm_inf(r), init))
-                break;
-            cycle_count++;
-        }
-        return cycle_count;
+        lal_hierarchy_destroy(h.h);
+        h.h = nullptr;
+        lal_saamg_options o;
+        lal_uaamg_default_options(&o);
+        o.max_levels = max_level < 20 ? max_level : 20;
+        check(lal_uaamg_setup(context(), A.get_m(), A.get_n(), A.get_nnz(), A.get_row_ptr(),
+                              A.get_col_ind(), A.get_val(), &o, &h.h),
+              "uaamg_setup");
+        check(lal_hierarchy_levels(h.h, &h.total_levels), "uaamg_setup");
+    }
+
+    // amg.cpp:516-571: cycles on x in place until the inf-norm residual
+    // converges or more than control.max_cycle cycles ran; returns the cycle
+    // count as the reference does.
+    inline int amg_solve(const hierarchy& h, vector<double>& x, const vector<double>& b, int n1,
+                         int n2, cycle cycle_type, smoother smoother_type,
+                         iter_control control)
+    {
+        lal_solve_options o;
+        lal_solve_default_options(&o);
+        o.rel_tol  = control.rel_tol;
+        o.abs_tol  = control.abs_tol;
+        o.max_iter = control.max_cycle;
+        lal_solve_info info{};
+        check(lal_amg_solve(context(), h.h, b.get_vec(), x.get_vec(), n1, n2,
+                            static_cast<int>(cycle_type), static_cast<int>(smoother_type), &o,
+                            &info),
+              "amg_solve");
+        return info.iterations;
     }
 
     // ---------------------------------------------------------- preconditioner
@@ -644,6 +659,133 @@ namespace lal_linalg
         {
             check(lal_amg_precond_f64(context(), h.h, rhs.get_vec(), x.get_vec(), n1, n2),
                   "saamg_precond::solve");
+        }
+    };
+
+    // z = r / a_ii with a_ii the first diagonal entry of each row
+    // (preconditioner.cpp:41-58, host_math.cpp:53-64).
+    class jacobi_precond : public preconditioner
+    {
+    public:
+        vector<double> diag;
+        void build(const csr_matrix<double>& A) override
+        {
+            diag.resize(A.get_m());
+            diag.zeros();
+            A.extract_diagonal(diag);
+        }
+        void solve(const vector<double>& rhs, vector<double>& x) const override
+        {
+            check(lal_jacobi_solve_f64(context(), (int)rhs.get_size(), rhs.get_vec(),
+                                       diag.get_vec(), x.get_vec()),
+                  "jacobi_precond::solve");
+        }
+    };
+
+    namespace detail
+    {
+        // The device solvers' preconditioner kind for a preconditioner object.
+        inline int precond_kind(const preconditioner* p, lal_hierarchy* h, const char* where)
+        {
+            *h = nullptr;
+            if(!p)
+                return LAL_PRECOND_NONE;
+            if(dynamic_cast<const jacobi_precond*>(p))
+                return LAL_PRECOND_JACOBI;
+            if(auto* sa = dynamic_cast<const saamg_precond*>(p))
+            {
+                *h = sa->h.h;
+                return LAL_PRECOND_SAAMG;
+            }
+            throw std::runtime_error(std::string(where) +
+                                     ": this backend runs no, Jacobi or SA-AMG preconditioning");
+        }
+
+        inline lal_solve_options options(const iter_control& control, const preconditioner* p)
+        {
+            lal_solve_options o;
+            lal_solve_default_options(&o);
+            o.rel_tol  = control.rel_tol;
+            o.abs_tol  = control.abs_tol;
+            o.max_iter = control.max_iter;
+            if(auto* sa = dynamic_cast<const saamg_precond*>(p))
+            {
+                o.n1 = sa->n1;
+                o.n2 = sa->n2;
+            }
+            return o;
+        }
+    }
+
+    // ---------------------------------------------------------- bicgstab_solver
+    // krylov/bicgstab.cpp:64-231 on the device; returns the iteration count.
+    class bicgstab_solver
+    {
+    public:
+        void build(const csr_matrix<double>&) {}
+
+        int solve_nonprecond(const csr_matrix<double>& A, vector<double>& x,
+                             const vector<double>& b, iter_control control)
+        {
+            return solve_precond(A, x, b, nullptr, control);
+        }
+        int solve_precond(const csr_matrix<double>& A, vector<double>& x, const vector<double>& b,
+                          const preconditioner* precond, iter_control control)
+        {
+            lal_hierarchy     h    = nullptr;
+            const int         kind = detail::precond_kind(precond, &h, "bicgstab_solver");
+            lal_solve_options o    = detail::options(control, precond);
+            lal_solve_info    info{};
+            check(lal_bicgstab_solve(context(), A.descriptor(), A.get_m(), A.get_nnz(),
+                                     A.get_row_ptr(), A.get_col_ind(), A.get_val(), b.get_vec(),
+                                     x.get_vec(), kind, h, &o, &info),
+                  "bicgstab_solver::solve");
+            return info.iterations;
+        }
+        int solve(const csr_matrix<double>& A, vector<double>& x, const vector<double>& b,
+                  const preconditioner* precond, iter_control control)
+        {
+            return precond ? solve_precond(A, x, b, precond, control)
+                           : solve_nonprecond(A, x, b, control);
+        }
+    };
+
+    // ------------------------------------------------------------- gmres_solver
+    // Restarted, left-preconditioned GMRES (krylov/gmres.cpp:46-530 as
+    // intended); returns the number of restart cycles, as gmres.cpp does.
+    class gmres_solver
+    {
+        int restart = 30;
+
+    public:
+        void build(const csr_matrix<double>&, int restart_)
+        {
+            restart = restart_;
+        }
+
+        int solve_nonprecond(const csr_matrix<double>& A, vector<double>& x,
+                             const vector<double>& b, iter_control control)
+        {
+            return solve_precond(A, x, b, nullptr, control);
+        }
+        int solve_precond(const csr_matrix<double>& A, vector<double>& x, const vector<double>& b,
+                          const preconditioner* precond, iter_control control)
+        {
+            lal_hierarchy     h    = nullptr;
+            const int         kind = detail::precond_kind(precond, &h, "gmres_solver");
+            lal_solve_options o    = detail::options(control, precond);
+            lal_solve_info    info{};
+            check(lal_gmres_solve(context(), A.descriptor(), A.get_m(), A.get_nnz(),
+                                  A.get_row_ptr(), A.get_col_ind(), A.get_val(), b.get_vec(),
+                                  x.get_vec(), restart, kind, h, &o, &info),
+                  "gmres_solver::solve");
+            return info.iterations;
+        }
+        int solve(const csr_matrix<double>& A, vector<double>& x, const vector<double>& b,
+                  const preconditioner* precond, iter_control control)
+        {
+            return precond ? solve_precond(A, x, b, precond, control)
+                           : solve_nonprecond(A, x, b, control);
         }
     };
 

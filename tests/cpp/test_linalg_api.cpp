@@ -180,6 +180,59 @@ int main()
     expect(cycles < 200 && inf_norm_diff(x.to_host(), xs) < 1e-6, "amg_solve V-cycles converge");
     std::printf("     amg_solve cycles %d\n", cycles);
 
+    // W-cycles with SSOR and F-cycles with Gauss-Seidel converge faster than
+    // V-cycles with Jacobi
+    x.zeros();
+    int cw = amg_solve(precond.h, x, b, 1, 1, cycle::wcycle, smoother::SSOR, c2);
+    expect(cw < cycles && inf_norm_diff(x.to_host(), xs) < 1e-6, "amg_solve W-cycles, SSOR");
+    x.zeros();
+    int cf = amg_solve(precond.h, x, b, 1, 1, cycle::fcycle, smoother::gauss_seidel, c2);
+    expect(cf < cycles && inf_norm_diff(x.to_host(), xs) < 1e-6,
+           "amg_solve F-cycles, Gauss-Seidel");
+    std::printf("     amg_solve W/SSOR cycles %d, F/GS cycles %d\n", cw, cf);
+
+    // unsmoothed aggregation hierarchy as a CG preconditioner
+    {
+        saamg_precond ua;
+        uaamg_setup(A, 20, ua.h);
+        x.zeros();
+        int it = cg.solve(A, x, b, &ua, control);
+        expect(it < 500 && inf_norm_diff(x.to_host(), xs) < 1e-8, "UA-AMG-PCG solution");
+        std::printf("     UA-AMG levels %d, PCG iterations %d\n", ua.h.total_levels, it);
+    }
+
+    // BiCGSTAB and GMRES with the Jacobi and SA-AMG preconditioners
+    {
+        jacobi_precond jac;
+        jac.build(A);
+        bicgstab_solver bi;
+        bi.build(A);
+        x.zeros();
+        int ib = bi.solve(A, x, b, &jac, control);
+        expect(ib < 500 && inf_norm_diff(x.to_host(), xs) < 1e-7, "BiCGSTAB + Jacobi");
+        x.zeros();
+        int ia = bi.solve(A, x, b, &precond, control);
+        expect(ia < ib && inf_norm_diff(x.to_host(), xs) < 1e-7, "BiCGSTAB + SA-AMG");
+        gmres_solver gm;
+        gm.build(A, 20);
+        x.zeros();
+        int ig = gm.solve(A, x, b, &precond, control);
+        expect(ig < 50 && inf_norm_diff(x.to_host(), xs) < 1e-7, "GMRES(20) + SA-AMG");
+        x.zeros();
+        bool threw = false;
+        try
+        {
+            bi.solve(A, x, b, &gv, control);
+        }
+        catch(const std::runtime_error&)
+        {
+            threw = true;
+        }
+        expect(threw, "BiCGSTAB refuses a preconditioner it cannot run");
+        std::printf("     BiCGSTAB Jacobi %d, SA-AMG %d iterations; GMRES(20) %d cycles\n", ib,
+                    ia, ig);
+    }
+
     synchronize();
     std::printf("%s\n", failures ? "FAILED" : "ALL OK");
     return failures ? 1 : 0;
