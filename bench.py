@@ -39,7 +39,7 @@ def parse():
     ap.add_argument("--spmv-calls", type=int, default=200)
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the standalone SpMV loop (for counter runs)")
-    ap.add_argument("--mixed", choices=("vcycle", "ir"), default="vcycle",
+    ap.add_argument("--mixed", choices=("vcycle", "ir", "reliable"), default="reliable",
                     help="mixed scheme reported as value")
     ap.add_argument("--config", choices=("c2", "c1", "c3", "c4", "c5", "amg", "krylov"),
                     default="c2",
@@ -136,6 +136,9 @@ def timed_solves(H, B, X, kinds, steps, warmup, rel_tol):
             return H.pcg(B, X, rel_tol=rel_tol, abs_tol=1e-300, max_iter=1000)
         if kind == "vcycle":
             return H.pcg_mixed(B, X, rel_tol=rel_tol, abs_tol=1e-300, max_iter=1000)
+        if kind == "reliable":
+            return H.pcg_reliable(B, X, delta=1e-2, rel_tol=rel_tol, abs_tol=1e-300,
+                                  max_iter=1000)
         return H.ir(B, X, rel_tol=rel_tol, abs_tol=1e-300, inner_rel_tol=1e-4,
                     inner_max_iter=200, max_outer=30)
 
@@ -199,7 +202,8 @@ def run_config(args):
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
     X = torch.zeros(m, dtype=torch.float64, device=dev)
-    res = timed_solves(H, B, X, ("fp64", "vcycle", "ir"), args.steps, args.warmup, tol)
+    res = timed_solves(H, B, X, ("fp64", "vcycle", "ir", "reliable"), args.steps, args.warmup,
+                       tol)
     line = {"config": {"workload": name, "n": m, "nnz": nnz, "levels": H.levels},
             "setup_s": round(setup_s, 4)}
     for kind, r in res.items():
@@ -210,6 +214,8 @@ def run_config(args):
                       "rel_residual_inf": float(np.abs(rr).max() / np.abs(b).max())}
     line["speedup_vcycle_vs_fp64"] = round(res["fp64"]["median_s"] / res["vcycle"]["median_s"], 3)
     line["speedup_ir_vs_fp64"] = round(res["fp64"]["median_s"] / res["ir"]["median_s"], 3)
+    line["speedup_reliable_vs_fp64"] = round(res["fp64"]["median_s"] / res["reliable"]["median_s"],
+                                             3)
     print(json.dumps(line), flush=True)
 
 
@@ -549,11 +555,14 @@ def main():
             return H.pcg(B, X, rel_tol=1e-10, abs_tol=1e-300, max_iter=1000, n1=1, n2=1)
         if kind == "vcycle":
             return H.pcg_mixed(B, X, rel_tol=1e-10, abs_tol=1e-300, max_iter=1000, n1=1, n2=1)
+        if kind == "reliable":
+            return H.pcg_reliable(B, X, delta=1e-2, rel_tol=1e-10, abs_tol=1e-300,
+                                  max_iter=1000, n1=1, n2=1)
         return H.ir(B, X, rel_tol=1e-10, abs_tol=1e-300, n1=1, n2=1, inner_rel_tol=1e-4,
                     inner_max_iter=200, max_outer=30)
 
     results = {}
-    for kind in ("fp64", "vcycle", "ir"):
+    for kind in ("fp64", "vcycle", "ir", "reliable"):
         for _ in range(args.warmup):
             run(kind)
         torch.cuda.synchronize()
@@ -610,7 +619,9 @@ def main():
             "n": m, "nnz": nnz, "levels": H.levels,
             "level_sizes": [[l[0], l[1]] for l in levels],
             "mixed_scheme": {"vcycle": "FP64 PCG, FP32 V-cycle",
-                             "ir": "FP32 PCG in FP64 refinement"}[args.mixed],
+                             "ir": "FP32 PCG in FP64 refinement",
+                             "reliable": "FP32 PCG and V-cycle, FP64 reliable updates"}[
+                                 args.mixed],
             "parallelism": "single GPU",
         },
         "fp64_solve_s": round(f64["total_s"] / args.steps, 6),
@@ -624,6 +635,10 @@ def main():
         "mixed_ir_solve_s": round(results["ir"]["total_s"] / args.steps, 6),
         "mixed_ir_inner_iterations": results["ir"]["iters"],
         "mixed_ir_outer_iterations": results["ir"]["outer"],
+        "mixed_reliable_solve_s": round(results["reliable"]["total_s"] / args.steps, 6),
+        "mixed_reliable_iterations": results["reliable"]["iters"],
+        "mixed_reliable_updates": results["reliable"]["outer"],
+        "mixed_reliable_rel_residual_inf": results["reliable"]["rel_res_inf"],
         "fp64_rel_residual_inf": f64["rel_res_inf"],
         "mixed_rel_residual_inf": mx["rel_res_inf"],
         "fp64_rel_err_vs_xstar": f64["rel_err"],

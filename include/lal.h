@@ -262,6 +262,20 @@ int lal_ir_solve(lal_handle h, lal_hierarchy hier, const double* b, double* x,
  * must stay within FP32's normal range. */
 int lal_pcg_solve_mixed(lal_handle h, lal_hierarchy hier, const double* b, double* x,
                         const lal_solve_options* opts, lal_solve_info* info);
+/* Mixed precision: cg_solver::solve_precond (cg.cpp:136-223) with the Krylov
+ * recurrences in FP32 (A's FP32 values, FP32 r/p/q/z and correction y, FP64
+ * inner-product accumulation, the FP32 V-cycle) and reliable updates in FP64:
+ * whenever the recursive residual has fallen by delta = opts->inner_rel_tol
+ * (1e-2 when opts is NULL; the default options' 1e-4 suits well-conditioned
+ * operators only) since the last update, or meets the tolerance, x += y in
+ * FP64 and the residual is
+ * replaced by the true FP64 b - A x.  Convergence (rel_tol / abs_tol on the
+ * inf-norm) is tested on that true residual only; info->iterations counts CG
+ * iterations as cg_solver does, info->outer_iterations the updates.  Needs
+ * build_f32.  No reference counterpart beyond cg.cpp: the reference has no
+ * mixed-precision solver. */
+int lal_pcg_solve_reliable(lal_handle h, lal_hierarchy hier, const double* b, double* x,
+                           const lal_solve_options* opts, lal_solve_info* info);
 
 /* AMG as a stand-alone solver -- replaces linalg::amg_solve
  * (iterative_solvers/amg/amg.cpp:516-571; enums of include/iterative_solvers/
@@ -320,8 +334,9 @@ int lal_gmres_solve(lal_handle h, lal_csrmv_descr descr, int m, int nnz, const i
 /* A lambda = g with the hierarchy built for A: A is SPD, so the adjoint
  * system is the forward one and its preconditioner is reused.  lambda is
  * overwritten (zero initial guess).  mixed = 0: FP64 PCG; 1: FP32-inner /
- * FP64-outer refinement; 2: FP64 PCG with the FP32 V-cycle (1 and 2 need a
- * hierarchy built with build_f32). */
+ * FP64-outer refinement; 2: FP64 PCG with the FP32 V-cycle; 3: FP32 PCG with
+ * FP64 reliable updates, as lal_pcg_solve_reliable (1-3 need a hierarchy
+ * built with build_f32). */
 int lal_adjoint_solve(lal_handle h, lal_hierarchy hier, const double* g, double* lambda,
                       int mixed, const lal_solve_options* opts, lal_solve_info* info);
 /* vals[k] = scale * u[row(k)] * v[col(k)] over the CSR pattern; with

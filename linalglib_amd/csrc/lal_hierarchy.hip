@@ -740,6 +740,127 @@ namespace
     }
 }
 
+namespace
+{
+    // cg_solver::solve_precond (cg.cpp:136-223) with the Krylov recurrences in
+    // FP32 and reliable updates (Sleijpen and van der Vorst, 1996) in FP64.
+    // The operator product q = A p, the vectors r, p, q, z and the correction
+    // y run in FP32 (A's FP32 values, the FP32 V-cycle); inner products
+    // accumulate in FP64.  r is the FP64 residual scaled by 1/||r0||, so its
+    // entries stay in FP32's normal range whatever b's magnitude.  When the
+    // recursive residual has fallen by `delta` since the last update, or meets
+    // the tolerance, the FP64 solution takes the correction (x += y ||r0||)
+    // and the recursive residual is replaced by the true FP64 residual
+    // b - A x; convergence is declared on that true residual only.  The
+    // Krylov space is kept across updates (no restart), so the iteration
+    // count stays that of the FP64 solve while every pass but the updates
+    // moves FP32 data.
+    int pcg_reliable(lal_hierarchy_t* h, const double* b, double* x, double rel_tol,
+                     double abs_tol, int max_iter, int n1, int n2, double delta,
+                     lal_solve_info* info)
+    {
+        lal_context*   c     = h->c;
+        const csr_dev& A     = h->L[0].A;
+        const int      n     = A.m;
+        const float*   Av32  = vals<float>(A);
+        double*        s     = h->s;
+        double*        r64   = h->res;
+        float*         r32   = h->res32;
+        float*         z32   = h->z32;
+        float*         p32   = h->p32;
+        float*         q32   = h->q32;
+        float*         y32   = h->d32;
+        double*        part_pq   = h->part;
+        double*        part_g[2] = {h->part + lal::PCG_BLOCKS, h->part + 2 * lal::PCG_BLOCKS};
+        double*        part_r    = h->part + 3 * lal::PCG_BLOCKS;
+        unsigned       seq       = 0;
+        double         init      = 0.0;
+        LAL_TRY(lal::residual_f64(c, A.d, n, A.n, A.nnz, A.rp, A.ci, A.v, x, b, r64));
+        LAL_TRY(lal::norm_partials<double>(c, n, r64, part_r));
+        LAL_TRY(post_scalar(h, 1, part_r, lal::S_RNORM0, &seq));
+        LAL_TRY(wait_scalar(h, seq, lal::S_RNORM0, &init));
+        double rn   = init; // true or recursive inf-norm, unscaled
+        int    iter = 0, updates = 0;
+        bool   conv = converges(rn, init, rel_tol, abs_tol);
+        if(!conv)
+        {
+            const double sigma = 1.0 / init;
+            LAL_TRY(lal::scale_cast_f64_f32(c, n, r64, sigma, r32));
+            LAL_HIP(hipMemsetAsync(y32, 0, sizeof(float) * n, c->stream));
+            LAL_TRY(precond<float>(h, r32, z32, n1, n2));
+            LAL_HIP(hipMemcpyAsync(p32, z32, sizeof(float) * n, hipMemcpyDeviceToDevice,
+                                   c->stream));
+            int cur = 0;
+            LAL_TRY(lal::dot_partials<float, float>(c, n, z32, r32, part_g[cur]));
+            const bool   fast  = !h->opts.exact && n1 == 1 && h->levels > 0;
+            work<float>& W0    = h->w<float>(0);
+            double       r_upd = init; // true residual norm at the last update
+            double       r_prev = init;
+            while(!(iter > max_iter))
+            {
+                LAL_TRY(lal::csrmv(c, A.d, n, A.n, A.nnz, 1.0, A.rp, A.ci, Av32, p32, 0.0,
+                                   q32));
+                LAL_TRY(lal::dot_partials<float, float>(c, n, q32, p32, part_pq));
+                LAL_TRY(lal::update_xr<float, float>(c, n, p32, q32, y32, r32, part_pq,
+                                                     part_g[cur], part_r, s + lal::S_ALPHA,
+                                                     W0.dinv, fast ? W0.x : nullptr, nullptr));
+                LAL_TRY(post_scalar(h, 1, part_r, lal::S_RNORM, &seq));
+                // Far from an update the next V-cycle is queued before the
+                // host reads the norm; near one (the residual a few factors
+                // above the threshold) the host decides first, so that no
+                // V-cycle is spent on a residual about to be replaced.
+                const bool near = r_prev < 8.0 * std::max(delta * r_upd, rel_tol * init);
+                if(!near)
+                    LAL_TRY(precond<float>(h, r32, z32, n1, n2, fast));
+                double rs = 0.0;
+                LAL_TRY(wait_scalar(h, seq, lal::S_RNORM, &rs));
+                rn     = rs * init;
+                r_prev = rn;
+                if(std::isnan(rn))
+                    break;
+                const bool update = rn <= delta * r_upd || converges(rn, init, rel_tol, abs_tol);
+                if(update)
+                {
+                    // x += ||r0|| y in FP64; r <- the true residual (scaled)
+                    LAL_TRY(lal::axpy_f32_f64(c, n, init, y32, x));
+                    LAL_HIP(hipMemsetAsync(y32, 0, sizeof(float) * n, c->stream));
+                    LAL_TRY(lal::residual_f64(c, A.d, n, A.n, A.nnz, A.rp, A.ci, A.v, x, b,
+                                              r64));
+                    LAL_TRY(lal::norm_partials<double>(c, n, r64, part_r));
+                    LAL_TRY(post_scalar(h, 1, part_r, lal::S_RNORM, &seq));
+                    LAL_TRY(lal::scale_cast_f64_f32(c, n, r64, sigma, r32));
+                    LAL_TRY(wait_scalar(h, seq, lal::S_RNORM, &rn));
+                    updates++;
+                    r_upd  = rn;
+                    r_prev = rn;
+                    if(converges(rn, init, rel_tol, abs_tol))
+                    {
+                        conv = true;
+                        break;
+                    }
+                    LAL_TRY(precond<float>(h, r32, z32, n1, n2));
+                }
+                else if(near)
+                    LAL_TRY(precond<float>(h, r32, z32, n1, n2, fast));
+                LAL_TRY(lal::dot_partials<float, float>(c, n, z32, r32, part_g[cur ^ 1]));
+                LAL_TRY(lal::update_p<float, float>(c, n, z32, p32, part_g[cur ^ 1],
+                                                    part_g[cur]));
+                cur ^= 1;
+                iter++;
+            }
+        }
+        if(info)
+        {
+            info->iterations       = iter;
+            info->outer_iterations = updates;
+            info->converged        = conv;
+            info->initial_res_norm = init;
+            info->final_res_norm   = rn;
+        }
+        return LAL_OK;
+    }
+}
+
 namespace lal
 {
     bool converges(double res, double init, double rel_tol, double abs_tol)
@@ -946,6 +1067,28 @@ extern "C" int lal_pcg_solve_mixed(lal_handle c, lal_hierarchy h, const double* 
     return pcg_mixed(h, b, x, o.rel_tol, o.abs_tol, o.max_iter, o.n1, o.n2, info);
 }
 
+extern "C" int lal_pcg_solve_reliable(lal_handle c, lal_hierarchy h, const double* b,
+                                      double* x, const lal_solve_options* opts,
+                                      lal_solve_info* info)
+{
+    LAL_REQUIRE(c != nullptr && h != nullptr && b != nullptr && x != nullptr && c == h->c);
+    if(!h->has32)
+    {
+        lal::set_last_error("hierarchy was built without build_f32");
+        return LAL_ERR_INVALID;
+    }
+    lal_solve_options o;
+    lal_solve_default_options(&o);
+    if(opts)
+        o = *opts;
+    LAL_REQUIRE(o.max_iter >= 0 && o.n1 >= 0 && o.n2 >= 0);
+    // without options the update threshold is 1e-2, which also holds the
+    // iteration count on badly conditioned operators
+    const double delta = opts ? o.inner_rel_tol : 1e-2;
+    LAL_REQUIRE(delta > 0.0 && delta < 1.0);
+    return pcg_reliable(h, b, x, o.rel_tol, o.abs_tol, o.max_iter, o.n1, o.n2, delta, info);
+}
+
 extern "C" int lal_ir_solve(lal_handle c, lal_hierarchy h, const double* b, double* x,
                             const lal_solve_options* opts, lal_solve_info* info)
 {
@@ -1013,11 +1156,13 @@ extern "C" int lal_adjoint_solve(lal_handle c, lal_hierarchy h, const double* g,
                                  int mixed, const lal_solve_options* opts, lal_solve_info* info)
 {
     LAL_REQUIRE(c != nullptr && h != nullptr && g != nullptr && lambda != nullptr && c == h->c);
-    LAL_REQUIRE(mixed >= 0 && mixed <= 2);
+    LAL_REQUIRE(mixed >= 0 && mixed <= 3);
     LAL_HIP(hipMemsetAsync(lambda, 0, sizeof(double) * h->L[0].A.m, c->stream));
     if(mixed == 1)
         return lal_ir_solve(c, h, g, lambda, opts, info);
     if(mixed == 2)
         return lal_pcg_solve_mixed(c, h, g, lambda, opts, info);
+    if(mixed == 3)
+        return lal_pcg_solve_reliable(c, h, g, lambda, opts, info);
     return lal_pcg_solve_f64(c, h, g, lambda, opts, info);
 }

@@ -106,6 +106,7 @@ _SIGS = {
     "lal_pcg_solve_f64": [vp, vp, vp, vp, C.POINTER(SolveOptions), C.POINTER(SolveInfo)],
     "lal_ir_solve": [vp, vp, vp, vp, C.POINTER(SolveOptions), C.POINTER(SolveInfo)],
     "lal_pcg_solve_mixed": [vp, vp, vp, vp, C.POINTER(SolveOptions), C.POINTER(SolveInfo)],
+    "lal_pcg_solve_reliable": [vp, vp, vp, vp, C.POINTER(SolveOptions), C.POINTER(SolveInfo)],
     "lal_amg_solve": [vp, vp, vp, vp, i32, i32, i32, i32, C.POINTER(SolveOptions),
                       C.POINTER(SolveInfo)],
     "lal_sddmm_f64": [vp, i32, i32, vp, vp, f64, vp, vp, vp],
@@ -310,6 +311,17 @@ class Hierarchy:
         info = SolveInfo()
         call("lal_amg_solve", self.handle.h, self.hp, ptr(b), ptr(x), n1, n2,
              self.CYCLES[cycle], self.SMOOTHERS[smoother], C.byref(o), C.byref(info))
+        return info
+
+    def pcg_reliable(self, b, x, delta=1e-2, **opts):
+        """FP32 PCG (FP32 operator, vectors and V-cycle) with FP64 reliable
+        updates every `delta` of residual reduction (needs build_f32);
+        info.outer_iterations counts the updates."""
+        o = solve_options(**opts)
+        o.inner_rel_tol = delta
+        info = SolveInfo()
+        call("lal_pcg_solve_reliable", self.handle.h, self.hp, ptr(b), ptr(x), C.byref(o),
+             C.byref(info))
         return info
 
     def pcg_mixed(self, b, x, **opts):

@@ -13,8 +13,9 @@ Gradients with respect to parameters that assemble `val` (face
 conductances, edge weights) follow from torch's own autograd through the
 assembly, e.g. `val = torch.zeros(nnz).index_add(0, entry, sign * k[face])`.
 mixed selects the precision of both solves: False (FP64 throughout), "vcycle"
-(FP64 PCG with the V-cycle on FP32 levels; also what True means) or "ir"
-(FP32 inner SA-AMG-PCG inside FP64 iterative refinement).
+(FP64 PCG with the V-cycle on FP32 levels; also what True means), "ir"
+(FP32 inner SA-AMG-PCG inside FP64 iterative refinement) or "reliable" (FP32
+PCG and V-cycle with FP64 reliable updates every 1e-2 of residual reduction).
 """
 import ctypes as C
 
@@ -33,7 +34,7 @@ class _Solve(torch.autograd.Function):
         x = torch.zeros(m, dtype=torch.float64, device=b.device)
         b_c = b.detach().contiguous()
         opts = dict(rel_tol=tol, abs_tol=1e-300, max_iter=1000)
-        solve = {0: H.pcg, 1: H.ir, 2: H.pcg_mixed}[mixed]
+        solve = {0: H.pcg, 1: H.ir, 2: H.pcg_mixed, 3: H.pcg_reliable}[mixed]
         info = solve(b_c, x, **opts)
         if not info.converged:
             raise lal.LalError(4, "saamg_solve forward",
@@ -54,6 +55,8 @@ class _Solve(torch.autograd.Function):
         g = gx.detach().contiguous().to(torch.float64)
         lam = torch.empty_like(g)
         o = lal.solve_options(rel_tol=ctx.tol, abs_tol=1e-300, max_iter=1000)
+        if ctx.mixed == 3:
+            o.inner_rel_tol = 1e-2  # the reliable-update threshold
         info = lal.SolveInfo()
         lal.call("lal_adjoint_solve", ctx.handle.h, ctx.H.hp, lal.ptr(g), lal.ptr(lam),
                  int(ctx.mixed), C.byref(o), C.byref(info))
@@ -83,7 +86,8 @@ def saamg_solve(val, b, row_ptr, col_ind, mixed=False, tol=1e-12):
     """x = A^-1 b with A = CSR(row_ptr, col_ind, val); differentiable in val and b."""
     if val.dtype != torch.float64 or b.dtype != torch.float64:
         raise TypeError("saamg_solve takes float64 values and right-hand side")
-    modes = {False: 0, True: 2, "ir": 1, "vcycle": 2}
+    modes = {False: 0, True: 2, "ir": 1, "vcycle": 2, "reliable": 3}
     if mixed not in modes:
-        raise ValueError(f"mixed must be False, True, 'ir' or 'vcycle', not {mixed!r}")
+        raise ValueError(f"mixed must be False, True, 'ir', 'vcycle' or 'reliable', "
+                         f"not {mixed!r}")
     return _Solve.apply(val, b, row_ptr, col_ind, _handle(), modes[mixed], float(tol))

@@ -70,7 +70,7 @@ def gpu_loss_and_grad(rp, ci, entry, face, sign, k, b, xt, mixed=False):
     return L.item(), x.detach().cpu().numpy(), K.grad.cpu().numpy(), B.grad.cpu().numpy()
 
 
-@pytest.mark.parametrize("mixed", [False, "ir", "vcycle"])
+@pytest.mark.parametrize("mixed", [False, "ir", "vcycle", "reliable"])
 def test_gradient_matches_fp64_oracle(oracle, mixed):
     nx = 20
     rp, ci, m, nf, entry, face, sign, k, b, xt = setup_problem(nx, seed=3)

@@ -313,6 +313,54 @@ def test_mixed_precision_pcg_f32_vcycle(h, oracle, name):
     assert np.linalg.norm(xg - xo) <= 1e-8 * np.linalg.norm(xo)
 
 
+@pytest.mark.parametrize("name", ["3d16", "diff12", "2d128"])
+def test_mixed_precision_pcg_reliable(h, oracle, name):
+    """FP32 PCG and V-cycle with FP64 reliable updates: convergence is declared
+    on the true FP64 residual, so it meets the tolerance; the Krylov space is
+    kept across updates, so the iteration count stays within a few of the
+    FP64 PCG's, and the solution matches the FP64 oracle's to 1e-8."""
+    rp, ci, v, m = grid(name)
+    A = CSR(rp, ci, v, m, m)
+    b = pb.rhs_from_solution(rp, ci, v, pb.uniform(m, seed=1))
+    log(f"pcg_reliable {name}")
+    H = _hier(h, A, build_f32=1)
+    B = dev(b)
+    x64 = empty(m, torch.float64)
+    info64 = H.pcg(B, x64, rel_tol=1e-10, abs_tol=1e-300, max_iter=500)
+    xo, _, _ = oracle.saamg_setup(A).pcg(b, rel_tol=1e-12, coarse_solver=1)
+    for delta in (1e-2, 1e-4):
+        x = empty(m, torch.float64)
+        info = H.pcg_reliable(B, x, delta=delta, rel_tol=1e-10, abs_tol=1e-300, max_iter=500)
+        assert info.converged and info.outer_iterations >= 1, delta
+        assert abs(info.iterations - info64.iterations) <= 3, (delta, info.iterations,
+                                                              info64.iterations)
+        xg = host(x)
+        r = oracle.residual(A, xg, b)
+        assert np.abs(r).max() <= 1e-10 * np.abs(b).max() * 1.01, delta
+        assert np.linalg.norm(xg - xo) <= 1e-8 * np.linalg.norm(xo), delta
+
+
+def test_pcg_reliable_scale_and_initial_guess(h, oracle):
+    """The FP32 residual is the FP64 one scaled by 1/||r0||: right-hand sides
+    of 1e-30 and 1e+30 converge like one of order 1, and a nonzero initial
+    guess is corrected, not replaced."""
+    rp, ci, v, m = grid("3d16")
+    A = CSR(rp, ci, v, m, m)
+    H = _hier(h, A, build_f32=1)
+    b1 = pb.rhs_from_solution(rp, ci, v, pb.uniform(m, seed=3))
+    x0 = pb.uniform(m, seed=4)
+    its = []
+    for scale in (1e-30, 1.0, 1e30):
+        b = b1 * scale
+        x = dev(x0 * scale)
+        info = H.pcg_reliable(dev(b), x, rel_tol=1e-10, abs_tol=0.0, max_iter=500)
+        r0 = np.abs(oracle.residual(A, x0 * scale, b)).max()
+        r = np.abs(oracle.residual(A, host(x), b)).max()
+        assert info.converged and r <= 1e-10 * r0, scale
+        its.append(info.iterations)
+    assert max(its) - min(its) <= 1, its
+
+
 def test_mixed_requires_f32_levels(h):
     rp, ci, v, m = grid("2d8")
     H = _hier(h, CSR(rp, ci, v, m, m))
@@ -322,6 +370,8 @@ def test_mixed_requires_f32_levels(h):
         H.pcg_mixed(B, x)
     with pytest.raises(lal.LalError):
         H.ir(B, x)
+    with pytest.raises(lal.LalError):
+        H.pcg_reliable(B, x)
 
 
 def test_sddmm(h):

@@ -52,7 +52,7 @@ def test_knn_hierarchy_and_pcg(h, oracle, order):
     info = H.pcg(B, x, rel_tol=1e-10, abs_tol=1e-300, max_iter=500)
     assert info.converged and abs(info.iterations - ito) <= 1, (info.iterations, ito)
     assert np.linalg.norm(host(x) - xo) <= 1e-9 * np.linalg.norm(xo)
-    for name, solve in (("vcycle", H.pcg_mixed), ("ir", H.ir)):
+    for name, solve in (("vcycle", H.pcg_mixed), ("ir", H.ir), ("reliable", H.pcg_reliable)):
         x.zero_()
         info = solve(B, x, rel_tol=1e-10, abs_tol=1e-300)
         xg = host(x)
@@ -62,7 +62,7 @@ def test_knn_hierarchy_and_pcg(h, oracle, order):
         assert np.linalg.norm(xg - xo) <= 1e-7 * np.linalg.norm(xo), name
 
 
-@pytest.mark.parametrize("mixed", [False, "vcycle"])
+@pytest.mark.parametrize("mixed", [False, "vcycle", "reliable"])
 def test_knn_edge_weight_gradient(oracle, mixed):
     n = 4000
     rp, ci, m, E, entry, edge, sign, dpos, w, val, b = knn_problem(n, 7)
