@@ -794,8 +794,8 @@ namespace
             LAL_TRY(lal::dot_partials<float, float>(c, n, z32, r32, part_g[cur]));
             const bool   fast  = !h->opts.exact && n1 == 1 && h->levels > 0;
             work<float>& W0    = h->w<float>(0);
-            double       r_upd = init; // true residual norm at the last update
-            double       r_prev = init;
+            double       r_upd  = init; // true residual norm at the last update
+            double       r_prev = init, rate = 0.5; // last norm, observed reduction per step
             while(!(iter > max_iter))
             {
                 LAL_TRY(lal::csrmv(c, A.d, n, A.n, A.nnz, 1.0, A.rp, A.ci, Av32, p32, 0.0,
@@ -805,16 +805,19 @@ namespace
                                                      part_g[cur], part_r, s + lal::S_ALPHA,
                                                      W0.dinv, fast ? W0.x : nullptr, nullptr));
                 LAL_TRY(post_scalar(h, 1, part_r, lal::S_RNORM, &seq));
-                // Far from an update the next V-cycle is queued before the
-                // host reads the norm; near one (the residual a few factors
-                // above the threshold) the host decides first, so that no
-                // V-cycle is spent on a residual about to be replaced.
-                const bool near = r_prev < 8.0 * std::max(delta * r_upd, rel_tol * init);
+                // The next V-cycle is queued before the host reads the norm,
+                // unless the observed rate of reduction predicts that this step
+                // crosses the update threshold: then the host decides first, so
+                // that no V-cycle is spent on a residual about to be replaced.
+                const double threshold = std::max(std::max(delta * r_upd, rel_tol * init),
+                                                  abs_tol);
+                const bool   near      = r_prev * rate <= 1.25 * threshold;
                 if(!near)
                     LAL_TRY(precond<float>(h, r32, z32, n1, n2, fast));
                 double rs = 0.0;
                 LAL_TRY(wait_scalar(h, seq, lal::S_RNORM, &rs));
                 rn     = rs * init;
+                rate   = std::min(1.0, std::max(0.05, rn / r_prev));
                 r_prev = rn;
                 if(std::isnan(rn))
                     break;
