@@ -111,13 +111,18 @@ namespace
 
     // One wavefront builds row `row` of C in a table of H slots (H a power of
     // two, at least 64 and at least twice the row's distinct columns).
+    //
+    // The numeric pass sizes the table by the row's nonzero count from the
+    // symbolic pass.  Should a caller pass a row pointer that undercounts a
+    // row, the probing stops after H slots and the row is cut to its length,
+    // *err is set, and no slot or output entry outside the row is touched.
     template <bool NUMERIC, bool GLOBAL>
     __device__ void spgemm_row(int row, int* keys, double* vals, int H, int lane,
                                const int* __restrict__ arp, const int* __restrict__ aci,
                                const double* __restrict__ av, const int* __restrict__ brp,
                                const int* __restrict__ bci, const double* __restrict__ bv,
                                const int* __restrict__ crp, int* __restrict__ cout_,
-                               double* __restrict__ cval)
+                               double* __restrict__ cval, int* __restrict__ err)
     {
         const int hmask = H - 1;
         for(int s = lane; s < H; s += 64)
@@ -133,8 +138,13 @@ namespace
             {
                 const int col = bci[p];
                 int       h   = slot_of(col, hmask);
-                while(true)
+                for(int probe = 0;; probe++)
                 {
+                    if(probe == H)
+                    {
+                        *err = 1; // table full: the row pointer undercounts this row
+                        break;
+                    }
                     int old = atomicCAS(&keys[h], -1, col);
                     if(old == -1)
                     {
@@ -183,14 +193,20 @@ namespace
         }
         const int nnz = base;
         const int out = crp[row];
+        const int len = crp[row + 1] - out;
+        if(nnz != len && lane == 0)
+            *err = 1;
         for(int e = lane; e < nnz; e += 64)
         {
             const int k = keys[e];
             int       r = 0;
             for(int t = 0; t < nnz; t++)
                 r += keys[t] < k;
-            cout_[out + r] = k;
-            cval[out + r]  = vals[e];
+            if(r < len)
+            {
+                cout_[out + r] = k;
+                cval[out + r]  = vals[e];
+            }
         }
     }
 
@@ -200,7 +216,8 @@ namespace
                           const int* __restrict__ aci, const double* __restrict__ av,
                           const int* __restrict__ brp, const int* __restrict__ bci,
                           const double* __restrict__ bv, const int* __restrict__ crp,
-                          int* __restrict__ cout_, double* __restrict__ cval)
+                          int* __restrict__ cout_, double* __restrict__ cval,
+                          int* __restrict__ err)
     {
         __shared__ int    s_keys[WPB][H];
         __shared__ double s_vals[NUMERIC ? WPB : 1][NUMERIC ? H : 1];
@@ -210,7 +227,7 @@ namespace
         if(r >= nrows)
             return;
         spgemm_row<NUMERIC, false>(rows[r], s_keys[wid], NUMERIC ? s_vals[wid] : nullptr, H, lane, arp,
-                            aci, av, brp, bci, bv, crp, cout_, cval);
+                            aci, av, brp, bci, bv, crp, cout_, cval, err);
     }
 
     template <bool NUMERIC>
@@ -221,7 +238,8 @@ namespace
                              const int* __restrict__ aci, const double* __restrict__ av,
                              const int* __restrict__ brp, const int* __restrict__ bci,
                              const double* __restrict__ bv, const int* __restrict__ crp,
-                             int* __restrict__ cout_, double* __restrict__ cval)
+                             int* __restrict__ cout_, double* __restrict__ cval,
+                             int* __restrict__ err)
     {
         const int r = blockIdx.x;
         if(r >= nrows)
@@ -229,10 +247,13 @@ namespace
         const long long o = toff[r];
         const int       H = (int)(toff[r + 1] - o);
         spgemm_row<NUMERIC, true>(rows[r], tkeys + o, NUMERIC ? tvals + o : nullptr, H,
-                            threadIdx.x & 63, arp, aci, av, brp, bci, bv, crp, cout_, cval);
+                            threadIdx.x & 63, arp, aci, av, brp, bci, bv, crp, cout_, cval, err);
     }
 
-    constexpr int kLdsMaxH = 4096;
+    // LDS tables: up to 4096 slots with values (numeric pass, 48 KB), up to
+    // 32768 keys alone (symbolic pass, 128 KB of the 160 KB of LDS).
+    constexpr int kLdsMaxH    = 4096;
+    constexpr int kLdsMaxHSym = 32768;
 
     int table_size(int prod, int ncols)
     {
@@ -274,31 +295,51 @@ namespace
         }
     };
 
-    constexpr int kBins = 7; // 64 .. 4096
+    constexpr int kBins = 10; // 64 .. 32768 (above 4096 only for the symbolic pass)
 
+    // Rows of C binned by table size: the symbolic pass sizes a row's table by
+    // its product count (an upper bound of its distinct columns), the numeric
+    // pass by its exact nonzero count from the symbolic pass (crp), which keeps
+    // almost every row in LDS.
     int make_plan(lal_context* c, int m, int n, const int* arp, const int* aci, const int* brp,
-                  bool numeric, spgemm_plan& P)
+                  const int* crp, bool numeric, spgemm_plan& P)
     {
-        int* d_prod = nullptr;
-        LAL_HIP(hipMallocAsync((void**)&d_prod, sizeof(int) * std::max(m, 1), c->stream));
-        products_kernel<<<lal::grid_for(m, 256), 256, 0, c->stream>>>(m, arp, aci, brp, d_prod);
-        LAL_LAUNCHED();
         std::vector<int> prod(m);
-        LAL_HIP(hipMemcpyAsync(prod.data(), d_prod, sizeof(int) * m, hipMemcpyDeviceToHost,
-                               c->stream));
-        LAL_HIP(hipFreeAsync(d_prod, c->stream));
-        LAL_HIP(hipStreamSynchronize(c->stream));
+        if(numeric)
+        {
+            std::vector<int> h(m + 1);
+            LAL_HIP(hipMemcpyAsync(h.data(), crp, sizeof(int) * (m + 1), hipMemcpyDeviceToHost,
+                                   c->stream));
+            LAL_HIP(hipStreamSynchronize(c->stream));
+            for(int i = 0; i < m; i++)
+                prod[i] = h[i + 1] - h[i];
+        }
+        else
+        {
+            int* d_prod = nullptr;
+            LAL_HIP(hipMallocAsync((void**)&d_prod, sizeof(int) * std::max(m, 1), c->stream));
+            products_kernel<<<lal::grid_for(m, 256), 256, 0, c->stream>>>(m, arp, aci, brp,
+                                                                          d_prod);
+            LAL_LAUNCHED();
+            LAL_HIP(hipMemcpyAsync(prod.data(), d_prod, sizeof(int) * m,
+                                   hipMemcpyDeviceToHost, c->stream));
+            LAL_HIP(hipFreeAsync(d_prod, c->stream));
+            LAL_HIP(hipStreamSynchronize(c->stream));
+        }
 
+        const int                     max_h = numeric ? kLdsMaxH : kLdsMaxHSym;
         std::vector<std::vector<int>> bins(kBins + 1);
         for(int i = 0; i < m; i++)
         {
-            if(prod[i] == 0)
+            // the numeric pass also visits rows the row pointer says are empty,
+            // so that a row pointer which undercounts them is detected
+            if(prod[i] <= 0 && !numeric)
                 continue;
-            int H = table_size(prod[i], n);
+            int H = table_size(std::max(prod[i], 0), n);
             int b = 0;
             while((64 << b) < H && b < kBins)
                 b++;
-            bins[H > kLdsMaxH ? kBins : b].push_back(i);
+            bins[H > max_h ? kBins : b].push_back(i);
         }
         P.bin_start.assign(kBins + 1, 0);
         P.rows.clear();
@@ -348,18 +389,22 @@ namespace
             if(nr == 0)
                 continue;
             const int* rows = P.d_rows + n0;
-#define LAL_SPGEMM_BIN(HH, WPB)                                                              \
-    spgemm_lds_kernel<HH, WPB, NUMERIC><<<(nr + WPB - 1) / WPB, WPB * 64, 0, c->stream>>>(   \
-        nr, rows, arp, aci, av, brp, bci, bv, crp, cout_, cval)
+#define LAL_SPGEMM_BIN(HH, WPB, NUM)                                                         \
+    spgemm_lds_kernel<HH, WPB, NUM><<<(nr + WPB - 1) / WPB, WPB * 64, 0, c->stream>>>(       \
+        nr, rows, arp, aci, av, brp, bci, bv, crp, cout_, cval, c->d_flag)
             switch(b)
             {
-            case 0: LAL_SPGEMM_BIN(64, 4); break;
-            case 1: LAL_SPGEMM_BIN(128, 4); break;
-            case 2: LAL_SPGEMM_BIN(256, 4); break;
-            case 3: LAL_SPGEMM_BIN(512, 4); break;
-            case 4: LAL_SPGEMM_BIN(1024, 4); break;
-            case 5: LAL_SPGEMM_BIN(2048, 1); break;
-            case 6: LAL_SPGEMM_BIN(4096, 1); break;
+            case 0: LAL_SPGEMM_BIN(64, 4, NUMERIC); break;
+            case 1: LAL_SPGEMM_BIN(128, 4, NUMERIC); break;
+            case 2: LAL_SPGEMM_BIN(256, 4, NUMERIC); break;
+            case 3: LAL_SPGEMM_BIN(512, 4, NUMERIC); break;
+            case 4: LAL_SPGEMM_BIN(1024, 4, NUMERIC); break;
+            case 5: LAL_SPGEMM_BIN(2048, 1, NUMERIC); break;
+            case 6: LAL_SPGEMM_BIN(4096, 1, NUMERIC); break;
+            // key-only tables of the symbolic pass (the numeric pass never bins here)
+            case 7: LAL_SPGEMM_BIN(8192, 1, false); break;
+            case 8: LAL_SPGEMM_BIN(16384, 1, false); break;
+            case 9: LAL_SPGEMM_BIN(32768, 1, false); break;
             }
 #undef LAL_SPGEMM_BIN
             LAL_LAUNCHED();
@@ -368,7 +413,7 @@ namespace
         {
             spgemm_global_kernel<NUMERIC><<<P.n_global, 64, 0, c->stream>>>(
                 P.n_global, P.d_rows + P.n_lds, P.d_toff, P.d_keys, P.d_vals, arp, aci, av, brp,
-                bci, bv, crp, cout_, cval);
+                bci, bv, crp, cout_, cval, c->d_flag);
             LAL_LAUNCHED();
         }
         return LAL_OK;
@@ -407,7 +452,7 @@ namespace lal
         (void)k;
         LAL_HIP(hipMemsetAsync(crp, 0, sizeof(int) * (m + 1), c->stream));
         spgemm_plan P;
-        LAL_TRY(make_plan(c, m, n, arp, aci, brp, false, P));
+        LAL_TRY(make_plan(c, m, n, arp, aci, brp, nullptr, false, P));
         int st = run_plan<false>(c, P, arp, aci, nullptr, brp, bci, nullptr, nullptr, crp,
                                  nullptr);
         P.release(c->stream);
@@ -426,10 +471,20 @@ namespace lal
     {
         (void)k;
         spgemm_plan P;
-        LAL_TRY(make_plan(c, m, n, arp, aci, brp, true, P));
+        LAL_TRY(make_plan(c, m, n, arp, aci, brp, crp, true, P));
+        LAL_HIP(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
         int st = run_plan<true>(c, P, arp, aci, av, brp, bci, bv, crp, cci, cv);
         P.release(c->stream);
-        return st;
+        LAL_TRY(st);
+        LAL_HIP(hipMemcpyAsync(c->h_flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost,
+                               c->stream));
+        LAL_HIP(hipStreamSynchronize(c->stream));
+        if(c->h_flag[0])
+        {
+            set_last_error("csrgemm_solve: row_ptr of C does not match the product's pattern");
+            return LAL_ERR_INVALID;
+        }
+        return LAL_OK;
     }
 }
 

@@ -131,6 +131,54 @@ def test_spgemm_bins_and_global_tables(h, oracle):
     assert csr_equal(_gpu_spgemm(h, A, Bn), oracle.csrgemm(A, Bn))
     T = random_csr(rng, 500, 3000, 40)
     assert csr_equal(_gpu_transpose(h, T), oracle.transpose(T))
+    # symbolic tables beyond the largest LDS table (> 16384 products over more
+    # than 16384 columns) take global memory; the numeric pass sizes the same
+    # rows by their nonzero count
+    k2, n2 = 600, 60000
+    lens2 = np.full(30, 3)
+    lens2[:6] = 250
+    rows2 = [np.sort(rng.choice(k2, size=x, replace=False)) for x in lens2]
+    rp2 = np.zeros(31, np.int32)
+    rp2[1:] = np.cumsum(lens2)
+    A2 = CSR(rp2, np.concatenate(rows2).astype(np.int32), rng.uniform(-1, 1, rp2[-1]), 30, k2)
+    B2 = random_csr(rng, k2, n2, 100)
+    assert csr_equal(_gpu_spgemm(h, A2, B2), oracle.csrgemm(A2, B2))
+
+
+def test_spgemm_refuses_inconsistent_row_pointer(h, oracle):
+    """The numeric pass sizes its tables by the row pointer of C; one that
+    does not match the product's pattern (an entry moved between two rows, or
+    an empty row that is not) is refused, and nothing outside C's rows is
+    written."""
+    rng = np.random.default_rng(22)
+    A = random_csr(rng, 200, 150, 8)
+    B = random_csr(rng, 150, 180, 9)
+    da, db = DevCSR(A), DevCSR(B)
+    crp = empty(A.m + 1, torch.int32)
+    nnz = C.c_int()
+    lal.call("lal_csrgemm_nnz", h.h, A.m, A.n, B.n, A.nnz, lal.ptr(da.row_ptr),
+             lal.ptr(da.col_ind), B.nnz, lal.ptr(db.row_ptr), lal.ptr(db.col_ind), lal.ptr(crp),
+             C.byref(nnz))
+    good = host(crp).copy()
+    lens = np.diff(good)
+    r = int(np.argmax(lens > 2))
+    for shift in ((r, -1, r + 1, +1), (r, +1, r + 1, -1)):
+        bad = lens.copy()
+        bad[shift[0]] += shift[1]
+        bad[shift[2]] += shift[3]
+        brp = np.zeros(A.m + 1, np.int32)
+        brp[1:] = np.cumsum(bad)
+        log(f"spgemm inconsistent row pointer {shift}")
+        guard = empty(nnz.value + 64, torch.float64)
+        gci = empty(nnz.value + 64, torch.int32)
+        guard.fill_(7.0)
+        gci.fill_(-7)
+        with pytest.raises(lal.LalError):
+            lal.call("lal_csrgemm_solve_f64", h.h, A.m, A.n, B.n, A.nnz, lal.ptr(da.row_ptr),
+                     lal.ptr(da.col_ind), lal.ptr(da.val), B.nnz, lal.ptr(db.row_ptr),
+                     lal.ptr(db.col_ind), lal.ptr(db.val), lal.ptr(dev(brp, np.int32)),
+                     lal.ptr(gci), lal.ptr(guard))
+        assert np.all(host(guard)[nnz.value:] == 7.0) and np.all(host(gci)[nnz.value:] == -7)
 
 
 # ------------------------------------------------------------------ hierarchy
