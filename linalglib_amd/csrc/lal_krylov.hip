@@ -14,10 +14,12 @@
 //                       vectors; here q_k = M^-1 A q_(k-1) is kept and every
 //                       work vector is sized.
 //
-// The vectors stay on the device.  BiCGSTAB's scalars come to the host as in
-// the reference's loop; GMRES keeps its Gram-Schmidt inner products on the
-// device and copies one Hessenberg column per Arnoldi step to the host, where
-// the Givens rotations and the small triangular solve run.  Work vectors are
+// The vectors stay on the device.  BiCGSTAB keeps its scalars on the device
+// (inner products as block partials summed by the kernel that uses them) and
+// reads one pair of numbers per iteration, the residual norm and the
+// breakdown flag; GMRES keeps its Gram-Schmidt inner products on the device
+// and copies one Hessenberg column per Arnoldi step to the host, where the
+// Givens rotations and the small triangular solve run.  Work vectors are
 // stream-ordered allocations of the call.
 #include "lal_hier.h"
 
@@ -122,6 +124,103 @@ namespace
             *h_out = h;
         for(int i = blockIdx.x * KT + threadIdx.x; i < n; i += KB * KT)
             qk[i] = qk[i] / h;
+    }
+
+    // BiCGSTAB with its scalars on the device.  sc[] holds alpha, omega, the
+    // breakdown flag and the residual norm; the inner products are KB partials
+    // that every block of the consuming kernel sums in the same order.
+    enum
+    {
+        SC_ALPHA = 0,
+        SC_OMEGA = 1,
+        SC_FLAG  = 2,
+        SC_RNORM = 3,
+        SC_COUNT = 4
+    };
+
+    // partials of <t, r> and <t, t> in one pass
+    __global__ void __launch_bounds__(KT)
+        dot2_kernel(int n, const double* __restrict__ t, const double* __restrict__ r,
+                    double* __restrict__ part1, double* __restrict__ part2)
+    {
+        double a = 0.0, c = 0.0;
+        for(int i = blockIdx.x * KT + threadIdx.x; i < n; i += KB * KT)
+        {
+            const double ti = t[i];
+            a               = a + ti * r[i];
+            c               = c + ti * ti;
+        }
+        const double sa = block_sum(a);
+        const double sc = block_sum(c);
+        if(threadIdx.x == 0)
+        {
+            part1[blockIdx.x] = sa;
+            part2[blockIdx.x] = sc;
+        }
+    }
+
+    // alpha = rho / <r0, A d>; r = -alpha (A d) + r
+    __global__ void __launch_bounds__(KT)
+        bicg_alpha_kernel(int n, const double* __restrict__ part_rho,
+                          const double* __restrict__ part_den, const double* __restrict__ ad,
+                          double* __restrict__ r, double* __restrict__ sc)
+    {
+        const double rho   = sum_parts(part_rho);
+        const double den   = sum_parts(part_den);
+        const double alpha = rho / den;
+        if(blockIdx.x == 0 && threadIdx.x == 0)
+            sc[SC_ALPHA] = alpha;
+        for(int i = blockIdx.x * KT + threadIdx.x; i < n; i += KB * KT)
+            r[i] = -1.0 * alpha * ad[i] + r[i];
+    }
+
+    // omega = <t, r> / <t, t>; unless either is exactly zero (then x += alpha p
+    // and the breakdown flag is raised): x = alpha d + omega s + x and
+    // r = -omega t + r
+    __global__ void __launch_bounds__(KT)
+        bicg_omega_kernel(int n, const double* __restrict__ part1,
+                          const double* __restrict__ part2, const double* __restrict__ d,
+                          const double* __restrict__ s, const double* __restrict__ t,
+                          const double* __restrict__ p, double* __restrict__ x,
+                          double* __restrict__ r, double* __restrict__ sc)
+    {
+        const double w1    = sum_parts(part1);
+        const double w2    = sum_parts(part2);
+        const double alpha = sc[SC_ALPHA];
+        const bool   brk   = w1 == 0.0 || w2 == 0.0;
+        const double omega = brk ? 0.0 : w1 / w2;
+        if(blockIdx.x == 0 && threadIdx.x == 0)
+        {
+            sc[SC_OMEGA] = omega;
+            sc[SC_FLAG]  = brk ? 1.0 : 0.0;
+        }
+        for(int i = blockIdx.x * KT + threadIdx.x; i < n; i += KB * KT)
+        {
+            if(brk)
+                x[i] = alpha * p[i] + x[i];
+            else
+            {
+                x[i] = alpha * d[i] + omega * s[i] + 1.0 * x[i];
+                r[i] = -1.0 * omega * t[i] + r[i];
+            }
+        }
+    }
+
+    // beta = (rho_new / rho_old) (alpha / omega); p = 1 r - beta omega (A d) + beta p
+    __global__ void __launch_bounds__(KT)
+        bicg_p_kernel(int n, const double* __restrict__ part_new,
+                      const double* __restrict__ part_old, const double* __restrict__ r,
+                      const double* __restrict__ ad, double* __restrict__ p,
+                      const double* __restrict__ sc)
+    {
+        const double rho_new = sum_parts(part_new);
+        const double rho_old = sum_parts(part_old);
+        const double alpha   = sc[SC_ALPHA];
+        const double omega   = sc[SC_OMEGA];
+        const double beta    = (rho_new / rho_old) * (alpha / omega);
+        const double g       = -1.0 * beta * omega;
+        for(int i = blockIdx.x * KT + threadIdx.x; i < n; i += KB * KT)
+            p[i] = 1.0 * r[i] + g * ad[i] + beta * p[i];
     }
 
     // Stream-ordered work vectors released when the solver returns.
@@ -241,55 +340,68 @@ extern "C" int lal_bicgstab_solve(lal_handle c, lal_csrmv_descr d, int m, int nn
     precond_t P;
     LAL_TRY(setup_precond(c, ws, A, precond, h, o, &P));
     const bool pre = precond != 0;
+    double *   sc, *part;
+    LAL_TRY(ws.get(&sc, SC_COUNT));
+    LAL_TRY(ws.get(&part, 5 * (size_t)KB));
+    double* part_rho[2] = {part, part + KB};
+    double* part_den    = part + 2 * KB;
+    double* part_w1     = part + 3 * KB;
+    double* part_w2     = part + 4 * KB;
 
     LAL_TRY(A.residual(c, x, b, r));
-    double init = 0.0, rn = 0.0, rho = 0.0;
+    double init = 0.0, rn = 0.0;
     LAL_TRY(lal_norm_inf_f64(c, m, r, &init));
     rn = init;
     LAL_TRY(lal_memcpy_d2d(c, r0, r, sizeof(double) * m));
-    LAL_TRY(lal_dot_f64(c, m, r0, r, &rho));
+    int cur = 0;
+    LAL_TRY(lal::dot_partials<double, double>(c, m, r0, r, part_rho[cur]));
     LAL_TRY(lal_memcpy_d2d(c, p, r, sizeof(double) * m));
     if(pre)
         LAL_TRY(P.apply(c, m, r, z));
     int  iter = 0;
     bool conv = false;
+    // pinned read-back of {flag, norm}: slots 8 and 9 of the context's mirror
+    // (slot 0 belongs to the blocking reductions), so the copy is asynchronous
+    // and the next direction's kernels queue behind it
+    double* hs = c->h_slots + 8;
     while(!(iter > o.max_iter))
     {
         const double* dir = pre ? z : p;
         double*       Ad  = pre ? q : vv;
         LAL_TRY(A.spmv(c, dir, Ad));
-        double den = 0.0;
-        LAL_TRY(lal_dot_f64(c, m, r0, Ad, &den));
-        const double alpha = rho / den;
-        LAL_TRY(lal_axpy_f64(c, m, -1.0 * alpha, Ad, r));
+        LAL_TRY(lal::dot_partials<double, double>(c, m, r0, Ad, part_den));
+        bicg_alpha_kernel<<<KB, KT, 0, c->stream>>>(m, part_rho[cur], part_den, Ad, r, sc);
+        LAL_LAUNCHED();
         if(pre)
             LAL_TRY(P.apply(c, m, r, vv));
-        const double* s = pre ? vv : r;
-        LAL_TRY(A.spmv(c, s, t));
-        double omega1 = 0.0, omega2 = 0.0;
-        LAL_TRY(lal_dot_f64(c, m, t, r, &omega1));
-        LAL_TRY(lal_dot_f64(c, m, t, t, &omega2));
-        if(omega1 == 0.0 || omega2 == 0.0)
-        {
-            LAL_TRY(lal_axpy_f64(c, m, alpha, p, x));
-            break;
-        }
-        const double omega = omega1 / omega2;
-        LAL_TRY(lal_axpbypgz_f64(c, m, alpha, dir, omega, s, 1.0, x));
-        LAL_TRY(lal_axpy_f64(c, m, -1.0 * omega, t, r));
-        LAL_TRY(lal_norm_inf_f64(c, m, r, &rn));
+        const double* sv = pre ? vv : r;
+        LAL_TRY(A.spmv(c, sv, t));
+        dot2_kernel<<<KB, KT, 0, c->stream>>>(m, t, r, part_w1, part_w2);
+        LAL_LAUNCHED();
+        bicg_omega_kernel<<<KB, KT, 0, c->stream>>>(m, part_w1, part_w2, dir, sv, t, p, x, r,
+                                                    sc);
+        LAL_LAUNCHED();
+        LAL_TRY(lal::norm_inf_f64_dev(c, m, r, sc + SC_RNORM, 0));
+        LAL_HIP(hipMemcpyAsync(hs, sc + SC_FLAG, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                               c->stream));
+        // the next direction depends on r only: it is queued before the host
+        // reads the norm (and is simply unused after the last iteration)
+        LAL_TRY(lal::dot_partials<double, double>(c, m, r0, r, part_rho[cur ^ 1]));
+        bicg_p_kernel<<<KB, KT, 0, c->stream>>>(m, part_rho[cur ^ 1], part_rho[cur], r, Ad, p,
+                                                sc);
+        LAL_LAUNCHED();
+        if(pre)
+            LAL_TRY(P.apply(c, m, p, z));
+        LAL_HIP(hipStreamSynchronize(c->stream));
+        if(hs[0] != 0.0)
+            break; // breakdown: x += alpha p was applied, as bicgstab.cpp does
+        rn = hs[1];
         if(lal::converges(rn, init, o.rel_tol, o.abs_tol))
         {
             conv = true;
             break;
         }
-        const double rho_prev = rho;
-        LAL_TRY(lal_dot_f64(c, m, r0, r, &rho));
-        const double beta = (rho / rho_prev) * (alpha / omega);
-        // p = r - beta*omega*(A d) + beta*p, in place through a copy of p
-        LAL_TRY(lal_axpbypgz_f64(c, m, 1.0, r, -1.0 * beta * omega, Ad, beta, p));
-        if(pre)
-            LAL_TRY(P.apply(c, m, p, z));
+        cur ^= 1;
         iter++;
     }
     if(info)
