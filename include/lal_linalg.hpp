@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <fstream>
 #include <iomanip>
 #include <iostream>
@@ -124,6 +125,17 @@ namespace lal_linalg
             size = n;
             check(lal_alloc((void**)&d, sizeof(T) * n), "vector::resize");
         }
+        void resize(size_t n, T val)
+        {
+            resize(n);
+            fill(val);
+        }
+        void clear()
+        {
+            lal_free(d);
+            d    = nullptr;
+            size = 0;
+        }
         void fill(T val);
         void zeros()
         {
@@ -133,6 +145,19 @@ namespace lal_linalg
         {
             fill(T(1));
         }
+        // vector<T>::rand (vector.cpp:182-199, host_memory.cpp:74-85): the same
+        // formula over the C library's rand(), filled on the host and uploaded
+        void rand(T min, T max)
+        {
+            std::vector<T> h(size);
+            for(size_t i = 0; i < size; i++)
+                h[i] = min + static_cast<T>(std::rand()) / (static_cast<T>(RAND_MAX / (max - min)));
+            check(lal_memcpy_h2d(context(), d, h.data(), sizeof(T) * size), "vector::rand");
+        }
+        // The data lives on the device for its whole life: residency moves are
+        // no-ops (vector.cpp's move_to_device / move_to_host).
+        void move_to_device() {}
+        void move_to_host() {}
         void copy_from(const vector& x)
         {
             resize(x.get_size());
@@ -413,6 +438,17 @@ namespace lal_linalg
         double* get_val()
         {
             return v.get_vec();
+        }
+
+        void move_to_device() {}
+        void move_to_host() {}
+        // csr_matrix<T>::copy_from (csr_matrix.cpp): pattern and values
+        void copy_from(const csr_matrix& A)
+        {
+            resize(A.m, A.n, A.nnz);
+            rp.copy_from(A.rp);
+            ci.copy_from(A.ci);
+            v.copy_from(A.v);
         }
 
         void resize(int m_, int n_, int nnz_)

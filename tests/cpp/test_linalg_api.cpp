@@ -118,6 +118,28 @@ int main()
         vector<double> ones(m, 1.0);
         expect(std::fabs(dot_product(ones, ones) - m) == 0.0, "dot_product closed form");
     }
+    // vector helpers and csr_matrix::copy_from
+    {
+        vector<double> r(1000);
+        r.rand(-2.0, 3.0);
+        auto h = r.to_host();
+        double lo = 1e300, hi = -1e300;
+        for(double t : h)
+        {
+            lo = std::fmin(lo, t);
+            hi = std::fmax(hi, t);
+        }
+        expect(lo >= -2.0 && hi <= 3.0 && hi - lo > 4.0, "vector::rand range");
+        r.resize(10, 2.5);
+        expect(r.get_size() == 10 && r.to_host()[9] == 2.5, "vector::resize(n, val)");
+        r.clear();
+        expect(r.get_size() == 0, "vector::clear");
+        csr_matrix<double> Ac;
+        Ac.copy_from(A);
+        vector<double> xd(xs), y(m);
+        Ac.multiply_by_vector(y, xd);
+        expect(inf_norm_diff(y.to_host(), hb) < 1e-12, "csr_matrix::copy_from");
+    }
     // Matrix Market round trip: write_mtx, read_mtx into a device matrix, SpMV
     {
         const char* path = "/tmp/lal_api_test_poisson.mtx";
