@@ -829,6 +829,7 @@ namespace lal_linalg
     class cg_solver
     {
         vector<double> z, p, res;
+        int            restart_iter = -1; // cg.cpp:45-49
 
     public:
         void build(const csr_matrix<double>& A)
@@ -883,11 +884,48 @@ namespace lal_linalg
             return iter;
         }
 
+        // cg.cpp:60-134, every step a device call.  As in the reference, the
+        // restart block there is not guarded by restart_iter != -1, and
+        // iter % -1 == 0, so with the default restart_iter = -1 every
+        // iteration after the first restarts from the true residual (the
+        // method is then steepest descent); kept so that iteration counts match.
+        int solve_nonprecond(const csr_matrix<double>& A, vector<double>& x,
+                             const vector<double>& b, iter_control control)
+        {
+            compute_residual(A, x, b, res);
+            const double initial_res_norm = norm_inf(res);
+            p.copy_from(res);
+            double gamma = dot_product(res, res);
+            int    iter  = 0;
+            while(!control.exceed_max_iter(iter))
+            {
+                if(iter > 0 && iter % restart_iter == 0)
+                {
+                    compute_residual(A, x, b, res);
+                    p.copy_from(res);
+                    gamma = dot_product(res, res);
+                }
+                A.multiply_by_vector(z, p);
+                double alpha = gamma / dot_product(z, p);
+                axpy(alpha, p, x);
+                axpy(-1.0 * alpha, z, res);
+                double res_norm = norm_inf(res);
+                if(control.residual_converges(res_norm, initial_res_norm))
+                    break;
+                double old_gamma = gamma;
+                gamma            = dot_product(res, res);
+                double beta      = gamma / old_gamma;
+                axpby(1.0, res, beta, p);
+                iter++;
+            }
+            return iter;
+        }
+
         int solve(const csr_matrix<double>& A, vector<double>& x, const vector<double>& b,
                   const preconditioner* precond, iter_control control)
         {
             if(!precond)
-                throw std::runtime_error("cg_solver: this backend runs the preconditioned path");
+                return solve_nonprecond(A, x, b, control);
             return solve_precond(A, x, b, precond, control);
         }
     };
