@@ -24,6 +24,7 @@
 #include "lal_hier.h"
 
 #include <cmath>
+#include <new>
 #include <vector>
 
 namespace
@@ -439,8 +440,21 @@ extern "C" int lal_gmres_solve(lal_handle c, lal_csrmv_descr d, int m, int nnz, 
     LAL_TRY(ws.get(&part, 2 * (size_t)KB));
     precond_t P;
     LAL_TRY(setup_precond(c, ws, A, precond, h, o, &P));
-    std::vector<double> H((size_t)ld * restart, 0.0), cs(restart, 0.0), sn(restart, 0.0),
-        y(ld, 0.0);
+    // the host Hessenberg matrix is (restart + 1) x restart: refuse what the
+    // host cannot hold instead of letting an allocation failure escape the C ABI
+    std::vector<double> H, cs, sn, y;
+    try
+    {
+        H.assign((size_t)ld * restart, 0.0);
+        cs.assign(restart, 0.0);
+        sn.assign(restart, 0.0);
+        y.assign(ld, 0.0);
+    }
+    catch(const std::bad_alloc&)
+    {
+        lal::set_last_error("gmres: restart too large for the host Hessenberg matrix");
+        return LAL_ERR_OOM;
+    }
     auto col = [&](int k) { return Q + (size_t)k * m; };
 
     LAL_TRY(A.residual(c, x, b, res));
