@@ -462,3 +462,13 @@ def test_gmres_closed_forms(oracle):
                          rel_tol=1e-12, max_iter=50)
     r = oracle.residual(A, x, b)
     assert np.abs(r).max() <= 1e-9 * np.abs(b).max()
+
+
+def test_bicgstab_breakdown_branch_closed_form(oracle):
+    """A = 2I: one step makes r exactly zero, t.r = 0 takes the breakdown
+    branch of bicgstab.cpp and x = alpha p = b / 2 after zero iterations."""
+    m = 50
+    A = CSR(np.arange(m + 1, dtype=np.int32), np.arange(m, dtype=np.int32), np.full(m, 2.0), m, m)
+    b = np.random.default_rng(2).uniform(-1, 1, m)
+    x, it = oracle.bicgstab(A, b, rel_tol=1e-12, max_iter=10)
+    assert it == 0 and bits_equal(x, b / 2)
