@@ -599,6 +599,26 @@ namespace
         return vcycle<T>(h, 0, r, z, n1, n2, xinit, &out);
     }
 
+    // True when the norm r_prev, reduced once more at the observed rate, is
+    // predicted to meet the threshold (or to come within 25 % of it).
+    bool near_threshold(double r_prev, double rate, double threshold)
+    {
+        return r_prev * rate <= 1.25 * threshold;
+    }
+
+    bool near_tolerance(double r_prev, double rate, double init, double rel_tol, double abs_tol)
+    {
+        return near_threshold(r_prev, rate, std::max(rel_tol * init, abs_tol));
+    }
+
+    // Records the reduction of the norm over the last step, clamped to
+    // [0.05, 1] (a NaN norm gives 0.05).
+    void observe_rate(double rn, double* r_prev, double* rate)
+    {
+        *rate   = std::min(1.0, std::max(0.05, rn / *r_prev));
+        *r_prev = rn;
+    }
+
     // cg_solver::solve_precond on the device.  T = double uses the FP64
     // levels and workspaces, T = float the FP32 ones.
     template <typename T>
@@ -632,6 +652,7 @@ namespace
         double rn   = init;
         int    iter = 0;
         bool   conv = false;
+        double r_prev = init, rate = 0.5; // last norm, observed reduction per step
         while(!(iter > max_iter))
         {
             LAL_TRY(lal::csrmv(c, A.d, n, A.n, A.nnz, 1.0, A.rp, A.ci, Av, p, 0.0, q));
@@ -645,16 +666,22 @@ namespace
                                          nullptr));
             LAL_TRY(post_scalar(h, 1, part_r, lal::S_RNORM, &seq));
             // The next preconditioner application depends on r only: it is
-            // queued before the host looks at the norm, so the GPU never
-            // waits for the convergence test (its result is simply unused
-            // after the last iteration).
-            LAL_TRY(precond<T>(h, res, z, n1, n2, fast));
+            // queued before the host looks at the norm, so the GPU does not
+            // wait for the convergence test -- unless the observed rate of
+            // reduction predicts that this step converges, when the host
+            // decides first so that no V-cycle is spent after the last step.
+            const bool near = near_tolerance(r_prev, rate, init, rel_tol, abs_tol);
+            if(!near)
+                LAL_TRY(precond<T>(h, res, z, n1, n2, fast));
             LAL_TRY(wait_scalar(h, seq, lal::S_RNORM, &rn));
+            observe_rate(rn, &r_prev, &rate);
             if(converges(rn, init, rel_tol, abs_tol))
             {
                 conv = true;
                 break;
             }
+            if(near)
+                LAL_TRY(precond<T>(h, res, z, n1, n2, fast));
             LAL_TRY(lal::dot_partials<T, T>(c, n, z, res, part_g[cur ^ 1]));
             LAL_TRY(lal::update_p<T, T>(c, n, z, p, part_g[cur ^ 1], part_g[cur]));
             cur ^= 1;
@@ -708,6 +735,7 @@ namespace
         bool         conv = false;
         const bool   fast = !h->opts.exact && n1 == 1 && h->levels > 0;
         work<float>& W0   = h->w<float>(0);
+        double       r_prev = init, rate = 0.5;
         while(!(iter > max_iter))
         {
             LAL_TRY(lal::csrmv(c, A.d, n, A.n, A.nnz, 1.0, A.rp, A.ci, A.v, p, 0.0, q));
@@ -716,13 +744,18 @@ namespace
                                                   part_r, s + lal::S_ALPHA, W0.dinv,
                                                   fast ? W0.x : nullptr, r32));
             LAL_TRY(post_scalar(h, 1, part_r, lal::S_RNORM, &seq));
-            LAL_TRY(precond<float>(h, r32, z32, n1, n2, fast));
+            const bool near = near_tolerance(r_prev, rate, init, rel_tol, abs_tol);
+            if(!near)
+                LAL_TRY(precond<float>(h, r32, z32, n1, n2, fast));
             LAL_TRY(wait_scalar(h, seq, lal::S_RNORM, &rn));
+            observe_rate(rn, &r_prev, &rate);
             if(converges(rn, init, rel_tol, abs_tol))
             {
                 conv = true;
                 break;
             }
+            if(near)
+                LAL_TRY(precond<float>(h, r32, z32, n1, n2, fast));
             LAL_TRY(lal::dot_partials<float, double>(c, n, z32, res, part_g[cur ^ 1]));
             LAL_TRY(lal::update_p<float, double>(c, n, z32, p, part_g[cur ^ 1], part_g[cur]));
             cur ^= 1;
@@ -811,14 +844,13 @@ namespace
                 // that no V-cycle is spent on a residual about to be replaced.
                 const double threshold = std::max(std::max(delta * r_upd, rel_tol * init),
                                                   abs_tol);
-                const bool   near      = r_prev * rate <= 1.25 * threshold;
+                const bool   near      = near_threshold(r_prev, rate, threshold);
                 if(!near)
                     LAL_TRY(precond<float>(h, r32, z32, n1, n2, fast));
                 double rs = 0.0;
                 LAL_TRY(wait_scalar(h, seq, lal::S_RNORM, &rs));
-                rn     = rs * init;
-                rate   = std::min(1.0, std::max(0.05, rn / r_prev));
-                r_prev = rn;
+                rn = rs * init;
+                observe_rate(rn, &r_prev, &rate);
                 if(std::isnan(rn))
                     break;
                 const bool update = rn <= delta * r_upd || converges(rn, init, rel_tol, abs_tol);
