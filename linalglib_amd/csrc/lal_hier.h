@@ -171,6 +171,16 @@ namespace lal
 {
     // iter_control::residual_converges (iter_control.cpp:38-78)
     bool converges(double res, double init, double rel_tol, double abs_tol);
+    // Convergence prediction for the solver loops, which queue the next
+    // preconditioner application before the host reads a residual norm unless
+    // the step is predicted to converge (or to cross a threshold): true when
+    // r_prev, reduced once more at the observed rate, meets the threshold or
+    // comes within 25 % of it.
+    bool near_threshold(double r_prev, double rate, double threshold);
+    bool near_tolerance(double r_prev, double rate, double init, double rel_tol, double abs_tol);
+    // Records the reduction over the last step, clamped to [0.05, 1] (a NaN
+    // norm gives 0.05), and the new norm.
+    void observe_rate(double rn, double* r_prev, double* rate);
     // Reduces PCG_BLOCKS partials (op 0 sum, 1 max) into the solver slot and
     // hands the value to the host through the hierarchy's mapped note;
     // returns the sequence number to wait for.

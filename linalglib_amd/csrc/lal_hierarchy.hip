@@ -599,26 +599,6 @@ namespace
         return vcycle<T>(h, 0, r, z, n1, n2, xinit, &out);
     }
 
-    // True when the norm r_prev, reduced once more at the observed rate, is
-    // predicted to meet the threshold (or to come within 25 % of it).
-    bool near_threshold(double r_prev, double rate, double threshold)
-    {
-        return r_prev * rate <= 1.25 * threshold;
-    }
-
-    bool near_tolerance(double r_prev, double rate, double init, double rel_tol, double abs_tol)
-    {
-        return near_threshold(r_prev, rate, std::max(rel_tol * init, abs_tol));
-    }
-
-    // Records the reduction of the norm over the last step, clamped to
-    // [0.05, 1] (a NaN norm gives 0.05).
-    void observe_rate(double rn, double* r_prev, double* rate)
-    {
-        *rate   = std::min(1.0, std::max(0.05, rn / *r_prev));
-        *r_prev = rn;
-    }
-
     // cg_solver::solve_precond on the device.  T = double uses the FP64
     // levels and workspaces, T = float the FP32 ones.
     template <typename T>
@@ -670,11 +650,11 @@ namespace
             // wait for the convergence test -- unless the observed rate of
             // reduction predicts that this step converges, when the host
             // decides first so that no V-cycle is spent after the last step.
-            const bool near = near_tolerance(r_prev, rate, init, rel_tol, abs_tol);
+            const bool near = lal::near_tolerance(r_prev, rate, init, rel_tol, abs_tol);
             if(!near)
                 LAL_TRY(precond<T>(h, res, z, n1, n2, fast));
             LAL_TRY(wait_scalar(h, seq, lal::S_RNORM, &rn));
-            observe_rate(rn, &r_prev, &rate);
+            lal::observe_rate(rn, &r_prev, &rate);
             if(converges(rn, init, rel_tol, abs_tol))
             {
                 conv = true;
@@ -744,11 +724,11 @@ namespace
                                                   part_r, s + lal::S_ALPHA, W0.dinv,
                                                   fast ? W0.x : nullptr, r32));
             LAL_TRY(post_scalar(h, 1, part_r, lal::S_RNORM, &seq));
-            const bool near = near_tolerance(r_prev, rate, init, rel_tol, abs_tol);
+            const bool near = lal::near_tolerance(r_prev, rate, init, rel_tol, abs_tol);
             if(!near)
                 LAL_TRY(precond<float>(h, r32, z32, n1, n2, fast));
             LAL_TRY(wait_scalar(h, seq, lal::S_RNORM, &rn));
-            observe_rate(rn, &r_prev, &rate);
+            lal::observe_rate(rn, &r_prev, &rate);
             if(converges(rn, init, rel_tol, abs_tol))
             {
                 conv = true;
@@ -844,13 +824,13 @@ namespace
                 // that no V-cycle is spent on a residual about to be replaced.
                 const double threshold = std::max(std::max(delta * r_upd, rel_tol * init),
                                                   abs_tol);
-                const bool   near      = near_threshold(r_prev, rate, threshold);
+                const bool   near      = lal::near_threshold(r_prev, rate, threshold);
                 if(!near)
                     LAL_TRY(precond<float>(h, r32, z32, n1, n2, fast));
                 double rs = 0.0;
                 LAL_TRY(wait_scalar(h, seq, lal::S_RNORM, &rs));
                 rn = rs * init;
-                observe_rate(rn, &r_prev, &rate);
+                lal::observe_rate(rn, &r_prev, &rate);
                 if(std::isnan(rn))
                     break;
                 const bool update = rn <= delta * r_upd || converges(rn, init, rel_tol, abs_tol);
@@ -898,6 +878,26 @@ namespace
 
 namespace lal
 {
+    // True when the norm r_prev, reduced once more at the observed rate, is
+    // predicted to meet the threshold (or to come within 25 % of it).
+    bool near_threshold(double r_prev, double rate, double threshold)
+    {
+        return r_prev * rate <= 1.25 * threshold;
+    }
+
+    bool near_tolerance(double r_prev, double rate, double init, double rel_tol, double abs_tol)
+    {
+        return near_threshold(r_prev, rate, std::max(rel_tol * init, abs_tol));
+    }
+
+    // Records the reduction of the norm over the last step, clamped to
+    // [0.05, 1] (a NaN norm gives 0.05).
+    void observe_rate(double rn, double* r_prev, double* rate)
+    {
+        *rate   = std::min(1.0, std::max(0.05, rn / *r_prev));
+        *r_prev = rn;
+    }
+
     bool converges(double res, double init, double rel_tol, double abs_tol)
     {
         if(std::isnan(res) || std::isnan(init))

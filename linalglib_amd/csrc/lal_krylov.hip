@@ -224,12 +224,19 @@ namespace
             p[i] = 1.0 * r[i] + g * ad[i] + beta * p[i];
     }
 
-    // Stream-ordered work vectors released when the solver returns.
+    // Stream-ordered work vectors released when the solver returns, and an
+    // event that marks a scalar read-back on the stream.
     struct workspace
     {
         lal_context*         c;
         std::vector<double*> bufs;
+        hipEvent_t           ev = nullptr;
         explicit workspace(lal_context* c_) : c(c_) {}
+        int make_event()
+        {
+            LAL_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            return LAL_OK;
+        }
         int get(double** p, size_t n)
         {
             *p = nullptr;
@@ -241,6 +248,8 @@ namespace
         {
             for(double* p : bufs)
                 (void)hipFreeAsync(p, c->stream);
+            if(ev)
+                (void)hipEventDestroy(ev);
         }
     };
 
@@ -365,6 +374,18 @@ extern "C" int lal_bicgstab_solve(lal_handle c, lal_csrmv_descr d, int m, int nn
     // (slot 0 belongs to the blocking reductions), so the copy is asynchronous
     // and the next direction's kernels queue behind it
     double* hs = c->h_slots + 8;
+    LAL_TRY(ws.make_event());
+    double r_prev = init, rate = 0.5; // last norm, observed reduction per step
+    // rho_new = <r0, r>; p = r - beta omega (A d) + beta p; z = M^-1 p
+    auto next_direction = [&](const double* Ad) -> int {
+        LAL_TRY(lal::dot_partials<double, double>(c, m, r0, r, part_rho[cur ^ 1]));
+        bicg_p_kernel<<<KB, KT, 0, c->stream>>>(m, part_rho[cur ^ 1], part_rho[cur], r, Ad, p,
+                                                sc);
+        LAL_LAUNCHED();
+        if(pre)
+            LAL_TRY(P.apply(c, m, p, z));
+        return LAL_OK;
+    };
     while(!(iter > o.max_iter))
     {
         const double* dir = pre ? z : p;
@@ -385,23 +406,26 @@ extern "C" int lal_bicgstab_solve(lal_handle c, lal_csrmv_descr d, int m, int nn
         LAL_TRY(lal::norm_inf_f64_dev(c, m, r, sc + SC_RNORM, 0));
         LAL_HIP(hipMemcpyAsync(hs, sc + SC_FLAG, 2 * sizeof(double), hipMemcpyDeviceToHost,
                                c->stream));
-        // the next direction depends on r only: it is queued before the host
-        // reads the norm (and is simply unused after the last iteration)
-        LAL_TRY(lal::dot_partials<double, double>(c, m, r0, r, part_rho[cur ^ 1]));
-        bicg_p_kernel<<<KB, KT, 0, c->stream>>>(m, part_rho[cur ^ 1], part_rho[cur], r, Ad, p,
-                                                sc);
-        LAL_LAUNCHED();
-        if(pre)
-            LAL_TRY(P.apply(c, m, p, z));
-        LAL_HIP(hipStreamSynchronize(c->stream));
+        LAL_HIP(hipEventRecord(ws.ev, c->stream));
+        // The next direction depends on r only: it is queued before the host
+        // waits for the read-back, so the GPU keeps working while the host
+        // tests the norm -- unless the observed rate of reduction predicts
+        // that this step converges, when the host decides first.
+        const bool near = lal::near_tolerance(r_prev, rate, init, o.rel_tol, o.abs_tol);
+        if(!near)
+            LAL_TRY(next_direction(Ad));
+        LAL_HIP(hipEventSynchronize(ws.ev));
         if(hs[0] != 0.0)
             break; // breakdown: x += alpha p was applied, as bicgstab.cpp does
         rn = hs[1];
+        lal::observe_rate(rn, &r_prev, &rate);
         if(lal::converges(rn, init, o.rel_tol, o.abs_tol))
         {
             conv = true;
             break;
         }
+        if(near)
+            LAL_TRY(next_direction(Ad));
         cur ^= 1;
         iter++;
     }
@@ -456,6 +480,12 @@ extern "C" int lal_gmres_solve(lal_handle c, lal_csrmv_descr d, int m, int nnz, 
         return LAL_ERR_OOM;
     }
     auto col = [&](int k) { return Q + (size_t)k * m; };
+    // q_(k+1) = M^-1 A q_k (before its orthogonalisation)
+    auto arnoldi_product = [&](int k) -> int {
+        LAL_TRY(A.spmv(c, col(k), w));
+        return P.apply(c, m, w, col(k + 1));
+    };
+    LAL_TRY(ws.make_event());
 
     LAL_TRY(A.residual(c, x, b, res));
     LAL_TRY(P.apply(c, m, res, z));
@@ -473,13 +503,13 @@ extern "C" int lal_gmres_solve(lal_handle c, lal_csrmv_descr d, int m, int nnz, 
         y[0] = rn;
         while(!(iter > o.max_iter))
         {
-            int k = 1;
+            LAL_TRY(arnoldi_product(0));
+            double r_prev = y[0], rate = 0.5; // residual estimate, reduction per step
+            int    k      = 1;
             for(k = 1; k < restart + 1; k++)
             {
                 double* qk = col(k);
                 steps++;
-                LAL_TRY(A.spmv(c, col(k - 1), w));
-                LAL_TRY(P.apply(c, m, w, qk));
                 // h_ik = <q_i, q_k>, q_k -= h_ik q_i for i < k, in order, then
                 // h_k,k-1 = ||q_k|| and q_k /= h_k,k-1, all on the device
                 double* hcol = dH + (size_t)(k - 1) * ld;
@@ -498,7 +528,16 @@ extern "C" int lal_gmres_solve(lal_handle c, lal_csrmv_descr d, int m, int nnz, 
                 LAL_LAUNCHED();
                 LAL_HIP(hipMemcpyAsync(&H[(size_t)(k - 1) * ld], hcol, sizeof(double) * (k + 1),
                                        hipMemcpyDeviceToHost, c->stream));
-                LAL_HIP(hipStreamSynchronize(c->stream));
+                LAL_HIP(hipEventRecord(ws.ev, c->stream));
+                // the next Arnoldi product needs only the normalised q_k: it
+                // is queued before the host waits for the column, unless this
+                // is the cycle's last step or the residual estimate is
+                // predicted to converge here
+                const bool ahead =
+                    k < restart && !lal::near_tolerance(r_prev, rate, init, o.rel_tol, o.abs_tol);
+                if(ahead)
+                    LAL_TRY(arnoldi_product(k));
+                LAL_HIP(hipEventSynchronize(ws.ev));
                 for(int i = 0; i < k - 1; i++)
                 {
                     const double t1 = H[i + (k - 1) * ld], t2 = H[i + 1 + (k - 1) * ld];
@@ -535,8 +574,11 @@ extern "C" int lal_gmres_solve(lal_handle c, lal_csrmv_descr d, int m, int nnz, 
                 }
                 y[k]     = -sn[k - 1] * y[k - 1];
                 y[k - 1] = cs[k - 1] * y[k - 1];
+                lal::observe_rate(std::fabs(y[k]), &r_prev, &rate);
                 if(lal::converges(std::fabs(y[k]), init, o.rel_tol, o.abs_tol) || k == restart)
                     break;
+                if(!ahead)
+                    LAL_TRY(arnoldi_product(k));
             }
             for(int i = k - 1; i >= 0; i--)
             {
