@@ -1,0 +1,112 @@
+"""Parity on the two remaining solve workloads at their full sizes, built exactly
+as bench.py builds them (numpy generator seeded 2024):
+
+* 3D finite-volume diffusion 160^3 (n = 4,096,000, nnz = 28,518,400), cell
+  conductivity kappa = exp(N(0, 1.5^2)) with harmonic-mean faces, b ~ N(0, 1);
+* kNN graph Laplacian of 2M uniform points in [0,1]^3 (symmetrised 12-NN, edge
+  weights ~ U[0.5, 2], + 1e-2 I), b ~ N(0, 1).
+
+For each: the exact-mode device hierarchy equals the oracle's level for level
+(SHA-256 of every A/P/R array, values included); the fast FP64 SA-AMG-PCG solve
+to 1e-10 takes the oracle's iteration count within one and lands within 1e-10
+relative of the oracle's solution; the reliable-update mixed solve meets the
+tolerance on the true FP64 residual and lands within 1e-8 of the oracle's
+solution.
+"""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from linalglib_amd import lal, problems as pb
+from oracle.oracle import CSR, Oracle
+from tests.gpu_util import DevCSR, dev, host, log
+
+pytestmark = pytest.mark.gpu
+
+
+def sha(*arrays):
+    s = hashlib.sha256()
+    for a in arrays:
+        s.update(np.ascontiguousarray(a).tobytes())
+    return s.hexdigest()
+
+
+def diffusion160():
+    rng = np.random.default_rng(2024)
+    nx = 160
+    kappa = np.exp(1.5 * rng.standard_normal((nx, nx, nx)))
+    rp, ci, v, m = pb.diffusion3d(nx, cell_kappa=kappa)
+    b = rng.standard_normal(m)
+    return CSR(rp, ci, v, m, m), b
+
+
+def knn2m():
+    rng = np.random.default_rng(2024)
+    rp, ci, m, E, entry, edge, sign, dpos = pb.knn_laplacian(2_000_000, seed=5)
+    w = rng.uniform(0.5, 2.0, len(E))
+    v = pb.assemble_knn(len(ci), entry, edge, sign, dpos, w)
+    b = rng.standard_normal(m)
+    return CSR(rp, ci, v, m, m), b
+
+
+@pytest.fixture(scope="module")
+def oracle_mt():
+    o = Oracle()
+    o.set_threads(min(16, len(os.sched_getaffinity(0))))
+    yield o
+    o.set_threads(1)
+
+
+@pytest.fixture(scope="module")
+def h():
+    return lal.Handle()
+
+
+@pytest.mark.parametrize("workload", ["diffusion160", "knn2m"])
+def test_workload_hierarchy_and_solves(h, oracle_mt, workload):
+    log(f"{workload}: build")
+    A, b = {"diffusion160": diffusion160, "knn2m": knn2m}[workload]()
+    d = DevCSR(A)
+    log(f"{workload}: oracle SA setup (n={A.m}, nnz={A.nnz})")
+    ho = oracle_mt.saamg_setup(A)
+
+    log(f"{workload}: device SA setup (exact)")
+    He = lal.Hierarchy(h, A.m, A.nnz, d.row_ptr, d.col_ind, d.val, exact=1)
+    assert He.levels == ho.levels, (He.levels, ho.levels)
+    for lvl in range(He.levels + 1):
+        for kind in ("A", "P", "R") if lvl < He.levels else ("A",):
+            grp, gci, gv, gm, gn = He.level(lvl, kind)
+            ref = ho.get(lvl, kind)
+            assert (gm, gn) == (ref.m, ref.n), (lvl, kind)
+            assert sha(grp, gci, gv) == sha(ref.row_ptr, ref.col_ind, ref.val), (lvl, kind)
+    He.close()
+
+    log(f"{workload}: oracle SA-AMG-PCG")
+    xo, ito, _ = ho.pcg(b, rel_tol=1e-10, coarse_solver=1)
+    del ho
+    log(f"{workload}: device SA-AMG-PCG (fast, FP64 and reliable updates)")
+    H = lal.Hierarchy(h, A.m, A.nnz, d.row_ptr, d.col_ind, d.val, build_f32=1)
+    B = dev(b)
+    x = torch.zeros(A.m, dtype=torch.float64, device="cuda")
+    info = H.pcg(B, x, rel_tol=1e-10, abs_tol=1e-300, max_iter=500)
+    xg = host(x)
+    log(f"{workload}: FP64 {info.iterations} it (oracle {ito}), "
+        f"rel. diff {np.linalg.norm(xg - xo) / np.linalg.norm(xo):.3e}")
+    assert info.converged and abs(info.iterations - ito) <= 1, (info.iterations, ito)
+    assert np.linalg.norm(xg - xo) <= 1e-10 * np.linalg.norm(xo)
+    r = oracle_mt.residual(A, xg, b)
+    assert np.abs(r).max() <= 1e-10 * np.abs(b).max() * 1.01
+    x.zero_()
+    info = H.pcg_reliable(B, x, rel_tol=1e-10, abs_tol=1e-300, max_iter=500)
+    xm = host(x)
+    rm = oracle_mt.residual(A, xm, b)
+    log(f"{workload}: reliable {info.iterations} it, {info.outer_iterations} updates, "
+        f"rel. residual {np.abs(rm).max() / np.abs(b).max():.3e}, "
+        f"rel. diff {np.linalg.norm(xm - xo) / np.linalg.norm(xo):.3e}")
+    assert info.converged, info.iterations
+    assert np.abs(rm).max() <= 1e-10 * np.abs(b).max() * 1.01
+    assert np.linalg.norm(xm - xo) <= 1e-8 * np.linalg.norm(xo)
+    H.close()
