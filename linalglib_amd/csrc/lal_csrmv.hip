@@ -127,6 +127,7 @@ namespace
         {
             lal::rv_args a{m,    d->nnz, d->by_nnz, d->d_blocks, d->nblocks,     rp,           ci, v, x, nullptr,
                            b,    y,           nullptr,        (double)alpha, (double)beta, d->win};
+            a.keep = d->keep;
             return lal::rowvec_launch<T>(c, MODE == 0 ? lal::RV_SPMV : lal::RV_RESIDUAL, a);
         }
         int nb = lal::spmv_grid(d, m);

@@ -268,6 +268,12 @@ namespace
             level_t& L = h->L[l];
             LAL_TRY(lal::csrmv_analysis_dev(c, L.A.m, L.A.n, L.A.nnz, L.A.rp, L.A.ci, exact,
                                             &L.A.d));
+            // The finest operator is read three times per PCG iteration
+            // (residual, post-sweep, the Krylov product), the last two with
+            // only vector passes between them: it may stay in the Infinity
+            // Cache when it fits (decided per precision at launch).  Keeping
+            // a coarser level as well overflows the cache and slows the solve.
+            L.A.d->keep = l == 0;
             LAL_TRY(alloc_n(&L.w64.x, L.A.m));
             LAL_TRY(alloc_n(&L.w64.t, L.A.m));
             LAL_TRY(alloc_n(&L.w64.b, L.A.m));

@@ -13,8 +13,11 @@
 // loaded and stage the products, which wastes no lanes on short rows.
 // Unaligned arrays take the product-staging kernel: lanes gather x for
 // consecutive nonzeros and stage a_ij * x_j in LDS before the row sums.
-// Column indices and values are read once, so they are streamed
-// (non-temporal) and x keeps the caches.  The sum order differs from the
+// Column indices and values are read once per call, so they are streamed
+// (non-temporal) and x keeps the caches -- except for an operator marked
+// `keep` that fits lal::CACHE_KEEP_BYTES (a hierarchy's finest level, read
+// several times per solver iteration), whose ordinary loads let it stay in
+// the Infinity Cache between calls.  The sum order differs from the
 // host's sequential fma, so results agree to rounding, deterministically; the
 // bit-exact path is the row-block kernel of lal_csrmv.hip.
 //
@@ -61,6 +64,7 @@ namespace lal
         double      alpha;
         double      beta;
         int         win; // LDS window in entries (lal_csrmv_info::win)
+        int         keep = 0; // the matrix may stay in the cache (lal_csrmv_info::keep)
     };
 
     template <typename T>

@@ -22,9 +22,13 @@ namespace
     struct quad<float>
     {
         flt4v v;
+        template <bool KEEP = false>
         __device__ __forceinline__ void load(const float* p)
         {
-            v = __builtin_nontemporal_load(reinterpret_cast<const flt4v*>(p));
+            if(KEEP)
+                v = *reinterpret_cast<const flt4v*>(p);
+            else
+                v = __builtin_nontemporal_load(reinterpret_cast<const flt4v*>(p));
         }
         __device__ __forceinline__ void store(float* s) const
         {
@@ -35,10 +39,19 @@ namespace
     struct quad<double>
     {
         dbl2v a, b;
+        template <bool KEEP = false>
         __device__ __forceinline__ void load(const double* p)
         {
-            a = __builtin_nontemporal_load(reinterpret_cast<const dbl2v*>(p));
-            b = __builtin_nontemporal_load(reinterpret_cast<const dbl2v*>(p) + 1);
+            if(KEEP)
+            {
+                a = reinterpret_cast<const dbl2v*>(p)[0];
+                b = reinterpret_cast<const dbl2v*>(p)[1];
+            }
+            else
+            {
+                a = __builtin_nontemporal_load(reinterpret_cast<const dbl2v*>(p));
+                b = __builtin_nontemporal_load(reinterpret_cast<const dbl2v*>(p) + 1);
+            }
         }
         __device__ __forceinline__ void store(double* s) const
         {
@@ -217,7 +230,7 @@ namespace
     // every offset of a handful of rows.  Fewer and wider memory instructions
     // and compact gathers are what lift this kernel over the product-staging
     // one on CDNA.
-    template <typename T, int MODE>
+    template <typename T, int MODE, bool KEEP>
     __global__ void __launch_bounds__(SPMV_THREADS)
         quad_kernel(int m, int nnz, const int* __restrict__ blocks, int nb, int win,
                     const int* __restrict__ rp, const int* __restrict__ ci,
@@ -260,8 +273,9 @@ namespace
                 for(int u = 0; u < 2; u++)
                 {
                     const int q = min(q0 + 4 * SPMV_THREADS * u, vend - 4);
-                    cq[u]       = __builtin_nontemporal_load(reinterpret_cast<const int4v*>(ci + q));
-                    vq[u].load(val + q);
+                    const int4v* cp = reinterpret_cast<const int4v*>(ci + q);
+                    cq[u]           = KEEP ? *cp : __builtin_nontemporal_load(cp);
+                    vq[u].template load<KEEP>(val + q);
                 }
 #pragma unroll
                 for(int u = 0; u < 2; u++)
@@ -383,8 +397,15 @@ namespace
             quad_nnz_kernel<T, MODE><<<nb, SPMV_THREADS, sizeof(T) * win, c->stream>>>(
                 a.m, a.nnz, a.blocks, nb, win, a.rp, a.ci, (const T*)a.val, (const T*)a.x,
                 (const T*)a.d, (const T*)a.b, (T)a.alpha, (T)a.beta, (T*)a.y, (T*)a.r);
+        else if(aligned && a.blocks && a.keep &&
+                (sizeof(T) + sizeof(int)) * (size_t)a.nnz <= lal::CACHE_KEEP_BYTES)
+            quad_kernel<T, MODE, true><<<nb, SPMV_THREADS, (sizeof(int) + sizeof(T)) * win,
+                                         c->stream>>>(
+                a.m, a.nnz, a.blocks, nb, win, a.rp, a.ci, (const T*)a.val, (const T*)a.x,
+                (const T*)a.d, (const T*)a.b, (T)a.alpha, (T)a.beta, (T*)a.y, (T*)a.r);
         else if(aligned && a.blocks)
-            quad_kernel<T, MODE><<<nb, SPMV_THREADS, (sizeof(int) + sizeof(T)) * win, c->stream>>>(
+            quad_kernel<T, MODE, false><<<nb, SPMV_THREADS, (sizeof(int) + sizeof(T)) * win,
+                                          c->stream>>>(
                 a.m, a.nnz, a.blocks, nb, win, a.rp, a.ci, (const T*)a.val, (const T*)a.x,
                 (const T*)a.d, (const T*)a.b, (T)a.alpha, (T)a.beta, (T*)a.y, (T*)a.r);
         else

@@ -21,6 +21,9 @@ struct lal_csrmv_info
     int  by_nnz   = 0;       // fast path: lanes gather x over consecutive nonzeros
                              // (rectangular transfer operators) instead of per row
     int  win      = 4;       // widest 4-aligned nonzero window of a block (<= 2052)
+    int  keep     = 0;       // fast path: the matrix may be read with
+                             // cache-allocating loads instead of streaming
+                             // ones when it fits lal::CACHE_KEEP_BYTES
     int* d_blocks = nullptr; // nblocks + 1 row offsets, then per block
                              // {r0, r1, row_ptr[r0], row_ptr[r1]} at
                              // lal::block_meta_offset(nblocks)
@@ -36,6 +39,11 @@ namespace lal
     }
 
     constexpr int SPMV_THREADS = 256;
+
+    // Largest matrix (values and column indices at the precision of the call)
+    // that a descriptor marked `keep` reads with cache-allocating loads: three
+    // quarters of MI355X's 256 MB Infinity Cache, leaving room for the vectors.
+    constexpr size_t CACHE_KEEP_BYTES = (size_t)192 << 20;
 
     template <typename T>
     struct spmv_cap
