@@ -656,7 +656,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": "quad_kernel<double, 0> (FP64 CSR SpMV, 3 rotating copies)",
+            "kernel": "quad_kernel<double, 0, false> (FP64 CSR SpMV, 3 rotating copies)",
             "bytes_per_launch": b64,
         },
         "cpu_baseline": cpu,
