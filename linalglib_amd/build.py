@@ -3,7 +3,8 @@
     python -m linalglib_amd.build          # compile every csrc/*.hip and link
 
 The library holds the HIP kernels, the C ABI declared in include/lal.h and
-the C++ host layer (linalglib_amd/csrc/*.cpp) above it.  Flags:
+the C++ host code above it (both in linalglib_amd/csrc/*.hip; plain *.cpp
+sources there, if any, are compiled as host code).  Flags:
 
 * ``--offload-arch=gfx950`` only (no other targets);
 * ``-ffp-contract=off`` so that no multiply-add is fused unless the source
