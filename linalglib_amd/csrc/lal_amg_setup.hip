@@ -242,6 +242,165 @@ namespace
     }
 }
 
+namespace
+{
+    // The same two kernels with one wavefront per row, for long rows (the
+    // coarse Galerkin levels): lanes read the row in coalesced chunks of 64,
+    // the next aggregate in ascending order is a wave-wide minimum, and every
+    // CSR-order fold (the weak-lumped diagonal, each column's sum of
+    // contributions) runs over the lanes in entry order, so the results are
+    // bit for bit those of the thread-per-row kernels.
+    __device__ __forceinline__ int64_t wave_min_i64(int64_t v)
+    {
+#pragma unroll
+        for(int o = 32; o > 0; o >>= 1)
+        {
+            const int64_t t = __shfl_xor(v, o, 64);
+            v               = t < v ? t : v;
+        }
+        return v;
+    }
+
+    // Smallest aggregate id above `last` among the row's candidates (LLONG_MAX
+    // when there is none); wave-uniform.
+    __device__ __forceinline__ int64_t next_aggregate(int i, int rs, int re, int lane,
+                                                      const int* __restrict__ ci,
+                                                      const int* __restrict__ conn,
+                                                      const int64_t* __restrict__ agg,
+                                                      int64_t last)
+    {
+        int64_t best = LLONG_MAX;
+        for(int j = rs + lane; j < re; j += 64)
+        {
+            const int c = ci[j];
+            if(p_candidate(i, c, conn[j]))
+            {
+                const int64_t a = agg[c];
+                if(a > last && a < best)
+                    best = a;
+            }
+        }
+        return wave_min_i64(best);
+    }
+
+    __global__ void __launch_bounds__(256)
+        prolong_count_wave_kernel(int m, const int* __restrict__ rp, const int* __restrict__ ci,
+                                  const int* __restrict__ conn, const int64_t* __restrict__ agg,
+                                  int* __restrict__ count)
+    {
+        const int lane = threadIdx.x & 63;
+        const int i    = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+        if(i > m)
+            return;
+        if(i == m)
+        {
+            if(lane == 0)
+                count[m] = 0;
+            return;
+        }
+        const int rs = rp[i], re = rp[i + 1];
+        int       k    = 0;
+        int64_t   last = -1; // aggregates >= 0 only
+        for(;;)
+        {
+            const int64_t best = next_aggregate(i, rs, re, lane, ci, conn, agg, last);
+            if(best == LLONG_MAX)
+                break;
+            k++;
+            last = best;
+        }
+        if(lane == 0)
+            count[i] = k;
+    }
+
+    __global__ void __launch_bounds__(256)
+        prolong_fill_wave_kernel(int m, const int* __restrict__ rp, const int* __restrict__ ci,
+                                 const double* __restrict__ val, const int* __restrict__ conn,
+                                 const int64_t* __restrict__ agg, double relax,
+                                 const int* __restrict__ prp, int* __restrict__ pci,
+                                 double* __restrict__ pv)
+    {
+        const int lane = threadIdx.x & 63;
+        const int i    = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+        if(i >= m)
+            return;
+        const int rs = rp[i], re = rp[i + 1];
+        // a_ii minus the weak couplings, summed in CSR order
+        double diagonal = 0.0;
+        for(int j0 = rs; j0 < re; j0 += 64)
+        {
+            const int j    = j0 + lane;
+            int       kind = 0;
+            double    v    = 0.0;
+            if(j < re)
+            {
+                const int c = ci[j];
+                if(c == i)
+                    kind = 1;
+                else if(conn[j] == 0)
+                    kind = 2;
+                v = val[j];
+            }
+            const int n = min(64, re - j0);
+            for(int t = 0; t < n; t++)
+            {
+                const int    kt = __shfl(kind, t, 64);
+                const double vt = __shfl(v, t, 64);
+                if(kt == 1)
+                    diagonal += vt;
+                else if(kt == 2)
+                    diagonal -= vt;
+            }
+        }
+        const double invd = 1.0 / diagonal;
+        int64_t      last = -1;
+        for(int out = prp[i]; out < prp[i + 1]; out++)
+        {
+            const int64_t best  = next_aggregate(i, rs, re, lane, ci, conn, agg, last);
+            bool          first = true;
+            double        acc   = 0.0;
+            for(int j0 = rs; j0 < re; j0 += 64)
+            {
+                const int j     = j0 + lane;
+                bool      match = false;
+                double    value = 0.0;
+                if(j < re)
+                {
+                    const int c = ci[j];
+                    if(p_candidate(i, c, conn[j]) && agg[c] == best)
+                    {
+                        match = true;
+                        value = (c == i) ? 1.0 - relax : -relax * invd * val[j];
+                    }
+                }
+                unsigned long long mask = __ballot(match);
+                while(mask)
+                {
+                    const int    t  = __ffsll((long long)mask) - 1;
+                    const double vt = __shfl(value, t, 64);
+                    mask &= mask - 1;
+                    if(first)
+                    {
+                        acc   = vt;
+                        first = false;
+                    }
+                    else
+                        acc += vt;
+                }
+            }
+            if(lane == 0)
+            {
+                pci[out] = (int)best;
+                pv[out]  = acc;
+            }
+            last = best;
+        }
+    }
+
+    // Rows this long on average take the wavefront-per-row kernels.
+    constexpr long PROLONG_WAVE_MIN_ROW = 24;
+}
+
 namespace lal
 {
     int diagonal_f64(lal_context* c, int m, const int* rp, const int* ci, const double* v,
@@ -305,12 +464,16 @@ namespace lal
         return LAL_OK;
     }
 
-    int sa_prolongator_nnz(lal_context* c, int m, const int* rp, const int* ci,
+    int sa_prolongator_nnz(lal_context* c, int m, int nnz, const int* rp, const int* ci,
                            const int* conn, const int64_t* agg, int* prp, int* nnz_P,
                            int* n_c)
     {
-        prolong_count_kernel<<<grid_for(m + 1, 256), 256, 0, c->stream>>>(m, rp, ci, conn, agg,
-                                                                          prp);
+        if(m > 0 && nnz >= PROLONG_WAVE_MIN_ROW * (long)m)
+            prolong_count_wave_kernel<<<(m + 1 + 3) / 4, 256, 0, c->stream>>>(m, rp, ci, conn,
+                                                                              agg, prp);
+        else
+            prolong_count_kernel<<<grid_for(m + 1, 256), 256, 0, c->stream>>>(m, rp, ci, conn,
+                                                                              agg, prp);
         LAL_LAUNCHED();
         LAL_TRY(scan_i32_dev(c, m + 1, prp));
         int64_t mx = LLONG_MIN;
@@ -324,14 +487,18 @@ namespace lal
         return LAL_OK;
     }
 
-    int sa_prolongator_fill(lal_context* c, int m, const int* rp, const int* ci, const double* v,
-                            const int* conn, const int64_t* agg, double relax, const int* prp,
-                            int* pci, double* pv)
+    int sa_prolongator_fill(lal_context* c, int m, int nnz, const int* rp, const int* ci,
+                            const double* v, const int* conn, const int64_t* agg, double relax,
+                            const int* prp, int* pci, double* pv)
     {
         if(m == 0)
             return LAL_OK;
-        prolong_fill_kernel<<<grid_for(m, 256), 256, 0, c->stream>>>(m, rp, ci, v, conn, agg,
-                                                                     relax, prp, pci, pv);
+        if(nnz >= PROLONG_WAVE_MIN_ROW * (long)m)
+            prolong_fill_wave_kernel<<<(m + 3) / 4, 256, 0, c->stream>>>(m, rp, ci, v, conn, agg,
+                                                                         relax, prp, pci, pv);
+        else
+            prolong_fill_kernel<<<grid_for(m, 256), 256, 0, c->stream>>>(m, rp, ci, v, conn, agg,
+                                                                         relax, prp, pci, pv);
         LAL_LAUNCHED();
         return LAL_OK;
     }
@@ -414,7 +581,7 @@ extern "C" int lal_sa_prolongator_nnz(lal_handle c, int m, int nnz, const int* r
 {
     LAL_REQUIRE(c != nullptr && m >= 0 && nnz >= 0 && prp && nnz_P && n_c);
     LAL_REQUIRE(m == 0 || (rp && agg && (nnz == 0 || (ci && conn))));
-    return lal::sa_prolongator_nnz(c, m, rp, ci, conn, agg, prp, nnz_P, n_c);
+    return lal::sa_prolongator_nnz(c, m, nnz, rp, ci, conn, agg, prp, nnz_P, n_c);
 }
 
 extern "C" int lal_sa_prolongator_fill_f64(lal_handle c, int m, int nnz, const int* rp,
@@ -424,5 +591,5 @@ extern "C" int lal_sa_prolongator_fill_f64(lal_handle c, int m, int nnz, const i
 {
     LAL_REQUIRE(c != nullptr && m >= 0 && nnz >= 0);
     LAL_REQUIRE(m == 0 || (rp && agg && prp && (nnz == 0 || (ci && v && conn))));
-    return lal::sa_prolongator_fill(c, m, rp, ci, v, conn, agg, relax, prp, pci, pv);
+    return lal::sa_prolongator_fill(c, m, nnz, rp, ci, v, conn, agg, relax, prp, pci, pv);
 }

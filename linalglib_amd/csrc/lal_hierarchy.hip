@@ -24,11 +24,11 @@ namespace lal
                     const double* v, double eps, int* conn);
     int pmis_aggregate(lal_context* c, int m, int nnz, const int* rp, const int* ci,
                        const int* conn, int64_t* agg);
-    int sa_prolongator_nnz(lal_context* c, int m, const int* rp, const int* ci, const int* conn,
-                           const int64_t* agg, int* prp, int* nnz_P, int* n_c);
-    int sa_prolongator_fill(lal_context* c, int m, const int* rp, const int* ci, const double* v,
-                            const int* conn, const int64_t* agg, double relax, const int* prp,
-                            int* pci, double* pv);
+    int sa_prolongator_nnz(lal_context* c, int m, int nnz, const int* rp, const int* ci,
+                           const int* conn, const int64_t* agg, int* prp, int* nnz_P, int* n_c);
+    int sa_prolongator_fill(lal_context* c, int m, int nnz, const int* rp, const int* ci,
+                            const double* v, const int* conn, const int64_t* agg, double relax,
+                            const int* prp, int* pci, double* pv);
     int ua_prolongator_nnz(lal_context* c, int m, const int64_t* agg, int* prp, int* nnz_P,
                            int* n_c);
     int ua_prolongator_fill(lal_context* c, int m, const int64_t* agg, const int* prp, int* pci,
@@ -442,8 +442,8 @@ namespace
             if(unsmoothed)
                 LAL_TRY(lal::ua_prolongator_nnz(c, Af.m, agg, P.rp, &nnzP, &nc));
             else
-                LAL_TRY(lal::sa_prolongator_nnz(c, Af.m, Af.rp, Af.ci, conn, agg, P.rp, &nnzP,
-                                                &nc));
+                LAL_TRY(lal::sa_prolongator_nnz(c, Af.m, Af.nnz, Af.rp, Af.ci, conn, agg, P.rp,
+                                                &nnzP, &nc));
             if(nc == 0)
             {
                 (void)hipFree(conn);
@@ -459,8 +459,8 @@ namespace
             if(unsmoothed)
                 LAL_TRY(lal::ua_prolongator_fill(c, Af.m, agg, P.rp, P.ci, P.v));
             else
-                LAL_TRY(lal::sa_prolongator_fill(c, Af.m, Af.rp, Af.ci, Af.v, conn, agg, o.relax,
-                                                 P.rp, P.ci, P.v));
+                LAL_TRY(lal::sa_prolongator_fill(c, Af.m, Af.nnz, Af.rp, Af.ci, Af.v, conn, agg,
+                                                 o.relax, P.rp, P.ci, P.v));
             (void)hipStreamSynchronize(c->stream);
             (void)hipFree(conn);
             (void)hipFree(agg);
