@@ -45,29 +45,69 @@ namespace
         }
     }
 
-    // Insertion sort of each row by column (columns are distinct in a row).
+    // Sorts each row by column, stably, with one wavefront per row: each lane
+    // ranks its entries (up to four) by counting the row's smaller columns,
+    // ties broken by position; every load of the row completes before the
+    // first store, so the row is permuted in place.  Rows longer than 256
+    // entries take lane 0's insertion sort.
     __global__ void __launch_bounds__(256)
-        row_sort_kernel(int m, const int* __restrict__ rp, int* __restrict__ ci,
-                        double* __restrict__ v)
+        row_sort_wave_kernel(int m, const int* __restrict__ rp, int* __restrict__ ci,
+                             double* __restrict__ v)
     {
-        int i = blockIdx.x * 256 + threadIdx.x;
+        const int lane = threadIdx.x & 63;
+        const int i    = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
         if(i >= m)
             return;
         const int s = rp[i], e = rp[i + 1];
-        for(int j = s + 1; j < e; j++)
+        if(e - s <= 1)
+            return;
+        if(e - s > 256)
         {
-            int    kc = ci[j];
-            double kv = v[j];
-            int    t  = j - 1;
-            while(t >= s && ci[t] > kc)
+            if(lane != 0)
+                return;
+            for(int j = s + 1; j < e; j++)
             {
-                ci[t + 1] = ci[t];
-                v[t + 1]  = v[t];
-                t--;
+                int    kc = ci[j];
+                double kv = v[j];
+                int    t  = j - 1;
+                while(t >= s && ci[t] > kc)
+                {
+                    ci[t + 1] = ci[t];
+                    v[t + 1]  = v[t];
+                    t--;
+                }
+                ci[t + 1] = kc;
+                v[t + 1]  = kv;
             }
-            ci[t + 1] = kc;
-            v[t + 1]  = kv;
+            return;
         }
+        int    key[4], rank[4];
+        double val[4];
+#pragma unroll
+        for(int u = 0; u < 4; u++)
+        {
+            const int j = s + lane + 64 * u;
+            key[u]      = j < e ? ci[j] : INT_MAX;
+            val[u]      = j < e ? v[j] : 0.0;
+            rank[u]     = 0;
+        }
+        for(int t = s; t < e; t++)
+        {
+            const int kt = ci[t];
+#pragma unroll
+            for(int u = 0; u < 4; u++)
+                rank[u] += kt < key[u] || (kt == key[u] && t < s + lane + 64 * u);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for(int u = 0; u < 4; u++)
+            if(s + lane + 64 * u < e)
+            {
+                ci[s + rank[u]] = key[u];
+                v[s + rank[u]]  = val[u];
+            }
     }
 
     // -------------------------------------------------------------- SpGEMM
@@ -440,7 +480,7 @@ namespace lal
         transpose_scatter_kernel<<<grid_for(m, 256), 256, 0, c->stream>>>(m, rp, ci, v, cursor,
                                                                           tci, tv);
         LAL_LAUNCHED();
-        row_sort_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(n, trp, tci, tv);
+        row_sort_wave_kernel<<<(n + 3) / 4, 256, 0, c->stream>>>(n, trp, tci, tv);
         LAL_LAUNCHED();
         LAL_HIP(hipFreeAsync(cursor, c->stream));
         return LAL_OK;

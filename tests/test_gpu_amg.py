@@ -111,6 +111,25 @@ def test_transpose_and_spgemm_golden(h, oracle, golden):
     assert csr_equal(got, mk("t", 70, 80))
 
 
+def test_transpose_row_length_classes(h, oracle):
+    """Transposed rows of every length the sort distinguishes: up to 64
+    entries (one per lane), up to 256 (several per lane) and beyond 256 (the
+    insertion-sort path), against the oracle's stable transpose."""
+    rng = np.random.default_rng(5)
+    m, n = 3000, 40
+    p = (np.arange(n) + 1.0) ** 2
+    p /= p.sum()
+    rows = [np.sort(rng.choice(n, size=int(rng.integers(0, 6)), replace=False, p=p))
+            for _ in range(m)]
+    rp = np.zeros(m + 1, np.int32)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    T = CSR(rp, np.concatenate(rows).astype(np.int32), rng.uniform(-1, 1, rp[-1]), m, n)
+    counts = np.bincount(T.col_ind, minlength=n)
+    assert counts.min() <= 64 and ((counts > 64) & (counts <= 256)).any() and counts.max() > 256
+    log("transpose row-length classes")
+    assert csr_equal(_gpu_transpose(h, T), oracle.transpose(T))
+
+
 def test_spgemm_bins_and_global_tables(h, oracle):
     """Rows spanning every hash-table size, including rows beyond the LDS
     tables (products > 2048) and empty rows."""
