@@ -80,6 +80,13 @@ def solve_bytes(levels, iters, vbytes):
     return iters * per_iter
 
 
+def reliable_update_bytes(m, nnz):
+    """Algorithmic bytes of one FP64 update of the reliable-update scheme:
+    x += ||r0|| y (y FP32 read, x FP64 read and written), y = 0, r = b - A x in
+    FP64, its inf-norm, and the scaled FP32 copy of r."""
+    return 20 * m + 4 * m + (nnz * 12 + 4 * (m + 1) + 24 * m) + 8 * m + 12 * m
+
+
 def measured_traffic():
     """HBM bytes per FP64 SpMV launch at 128^3 from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE passes (profiles/spmv_poisson128_hbm_traffic.md)."""
@@ -589,6 +596,10 @@ def main():
 
     f64, mx = results["fp64"], results[args.mixed]
     solve64_gbs = solve_bytes(levels, f64["iters"], 8) / f64["median_s"] / 1e9
+    rel = results["reliable"]
+    rel_bytes = (solve_bytes(levels, rel["iters"], 4)
+                 + rel["outer"] * reliable_update_bytes(levels[0][0], levels[0][1]))
+    solve_rel_gbs = rel_bytes / rel["median_s"] / 1e9
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -649,6 +660,9 @@ def main():
         "spmv_fp64_cache_resident_gbs": round(b64 / t64_hot / 1e9, 1),
         "spmv_fp64_us": round(t64 * 1e6, 3),
         "solve_fp64_algorithmic_gbs": round(solve64_gbs, 1),
+        "solve_fp64_roofline_frac": round(solve64_gbs / HBM_PEAK_GBS, 4),
+        "solve_mixed_reliable_algorithmic_gbs": round(solve_rel_gbs, 1),
+        "solve_mixed_reliable_roofline_frac": round(solve_rel_gbs / HBM_PEAK_GBS, 4),
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 1),
