@@ -99,10 +99,10 @@ def measured_traffic():
         return None
 
 
-def cpu_baseline(rp, ci, v, m, b, threads):
+def cpu_baseline(rp, ci, v, m, b, threads, tol=1e-10, name=None):
     """LinAlgLib's own host path (oracle/_ref, compiled from its sources) on
-    the same 128^3 problem when it was built, else the C oracle; SA setup and
-    one SA-AMG-PCG solve to 1e-10, timed on this machine's host cores."""
+    the same problem when it was built, else the C oracle; SA setup and one
+    SA-AMG-PCG solve to `tol`, timed on this machine's host cores."""
     here = os.path.dirname(os.path.abspath(__file__))
     sys.path.insert(0, here)
     from oracle.oracle import CSR, Oracle, Reference
@@ -116,15 +116,16 @@ def cpu_baseline(rp, ci, v, m, b, threads):
     t0 = time.perf_counter()
     h = lib.saamg_setup(A)
     t1 = time.perf_counter()
-    out = h.pcg(b, rel_tol=1e-10, abs_tol=1e-300, max_iter=1000)
+    out = h.pcg(b, rel_tol=tol, abs_tol=1e-300, max_iter=1000)
     t2 = time.perf_counter()
     iters = out[1]
     return {
         "value": round(t2 - t1, 4),
-        "unit": "s per SA-AMG-PCG solve to 1e-10 (FP64)",
+        "unit": f"s per SA-AMG-PCG solve to {tol:g} (FP64)",
         "cores": threads,
         "kind": kind,
-        "sample": (f"one full solve of the same 3D Poisson {round(m ** (1 / 3))}^3 problem "
+        "sample": (f"one full solve of the same "
+                   f"{name or f'3D Poisson {round(m ** (1 / 3))}^3'} problem "
                    f"(setup {t1 - t0:.2f} s excluded, {iters} PCG iterations, "
                    f"{h.levels} levels), OMP_NUM_THREADS={threads}"),
         "setup_s": round(t1 - t0, 3),
@@ -223,6 +224,12 @@ def run_config(args):
     line["speedup_ir_vs_fp64"] = round(res["fp64"]["median_s"] / res["ir"]["median_s"], 3)
     line["speedup_reliable_vs_fp64"] = round(res["fp64"]["median_s"] / res["reliable"]["median_s"],
                                              3)
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        try:
+            line["cpu_baseline"] = cpu_baseline(rp, ci, v, m, b, threads, tol, name)
+        except Exception as e:  # the baseline is context only
+            line["cpu_baseline"] = {"value": None, "cores": threads, "sample": f"not run: {e}"}
     print(json.dumps(line), flush=True)
 
 
