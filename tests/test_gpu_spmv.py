@@ -4,6 +4,7 @@ import ctypes as C
 
 import numpy as np
 import pytest
+import torch
 
 from linalglib_amd import lal, problems as pb
 from oracle.oracle import CSR
@@ -73,6 +74,37 @@ def test_csrmv_f64(h, oracle, exact):
         lal.call("lal_csrmv_f64", h.h, None, A.m, A.n, A.nnz, 1.0, lal.ptr(d.row_ptr),
                  lal.ptr(d.col_ind), lal.ptr(d.val), lal.ptr(X), 0.0, lal.ptr(y))
         assert bits_equal(host(y)[: A.m], oracle.csrmv(A, x))
+
+
+@pytest.mark.parametrize("exact", [1, 0, None])
+def test_csrmv_empty_inputs(h, oracle, exact):
+    """Degenerate sizes: no rows (a no-op), and rows without nonzeros
+    (y = beta y, the residual returns b), with an exact or fast descriptor or
+    none."""
+    log(f"csrmv empty inputs exact={exact}")
+    none = CSR(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0), 0, 5)
+    d0 = DevCSR(none)
+    desc0 = None if exact is None else lal.CsrDescr(h, 0, 5, 0, d0.row_ptr, d0.col_ind,
+                                                    exact=exact)
+    lal.call("lal_csrmv_f64", h.h, desc0.d if desc0 else None, 0, 5, 0, 1.0,
+             lal.ptr(d0.row_ptr), None, None, None, 0.0, lal.ptr(empty(1, torch.float64)))
+    m, n = 300, 40
+    A = CSR(np.zeros(m + 1, np.int32), np.zeros(0, np.int32), np.zeros(0), m, n)
+    d = DevCSR(A)
+    desc = None if exact is None else lal.CsrDescr(h, m, n, 0, d.row_ptr, d.col_ind, exact=exact)
+    x = np.random.default_rng(2).uniform(-1, 1, n)
+    y0 = np.random.default_rng(3).uniform(-1, 1, m)
+    X = dev(x)
+    for alpha, beta in ((1.0, 0.0), (2.0, -0.5)):
+        y = dev(y0)
+        lal.call("lal_csrmv_f64", h.h, desc.d if desc else None, m, n, 0, alpha,
+                 lal.ptr(d.row_ptr), None, None, lal.ptr(X), beta, lal.ptr(y))
+        assert bits_equal(host(y), oracle.csrmv(A, x, alpha, beta, y0)), (alpha, beta)
+    r = empty(m, torch.float64)
+    B = dev(y0)
+    lal.call("lal_residual_f64", h.h, desc.d if desc else None, m, n, 0, lal.ptr(d.row_ptr),
+             None, None, lal.ptr(X), lal.ptr(B), lal.ptr(r))
+    assert bits_equal(host(r), y0)
 
 
 def test_csrmv_f32(h, oracle):
