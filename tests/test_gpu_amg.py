@@ -210,6 +210,34 @@ def _hier(h, A, **kw):
     return H
 
 
+@pytest.mark.parametrize("m", [1, 100])
+def test_hierarchy_without_coarse_levels(h, oracle, m):
+    """A diagonal operator has no strong couplings: PMIS aggregates nothing,
+    P would have no columns and the setup stops at the finest level, as the
+    oracle's does; the V-cycle is then the coarsest solve alone (1x1 divide
+    or dense inverse) and PCG stops at its first convergence test."""
+    rng = np.random.default_rng(0)
+    d = rng.uniform(1, 5, m)
+    A = CSR(np.arange(m + 1, dtype=np.int32), np.arange(m, dtype=np.int32), d, m, m)
+    b = rng.standard_normal(m)
+    ho = oracle.saamg_setup(A)
+    xo, ito, _ = ho.pcg(b, rel_tol=1e-10, coarse_solver=1)
+    log(f"hierarchy without coarse levels m={m}")
+    for kw in ({"exact": 1}, {"exact": 0, "build_f32": 1}):
+        H = _hier(h, A, **kw)
+        assert H.levels == ho.levels == 0
+        x = empty(m, torch.float64)
+        info = H.pcg(dev(b), x, rel_tol=1e-10, abs_tol=1e-300, max_iter=50)
+        assert info.converged and info.iterations == ito, (kw, info.iterations, ito)
+        assert np.abs(host(x)[:m] - xo).max() <= 1e-14 * np.abs(xo).max(), kw
+        if kw.get("build_f32"):
+            x.zero_()
+            info = H.pcg_reliable(dev(b), x, rel_tol=1e-10, abs_tol=1e-300, max_iter=50)
+            assert info.converged, info.iterations
+            assert np.abs(host(x)[:m] - xo).max() <= 1e-10 * np.abs(xo).max()
+        H.close()
+
+
 @pytest.mark.parametrize("name,full", [("2d8", True), ("3d8", True), ("diff12", True),
                                        ("2d128", False), ("3d16", False)])
 def test_hierarchy_matches_reference_golden(h, golden, name, full):
