@@ -9,6 +9,8 @@ nnz = 14,581,760), x* ~ U[-1,1] (seed 1), b = A x*, x0 = 0, to 1e-10.
 * the reported mixed scheme (FP32 PCG with FP64 reliable updates) meets the
   tolerance on the true FP64 residual and lands within 1e-8 of the oracle's
   solution;
+* the unsmoothed-aggregation hierarchy equals the oracle's bit for bit and
+  UA-AMG-PCG takes its iteration count within one, solution within 1e-10;
 * one fast SpMV is within the recursive-summation bound of the oracle's;
 * on the differentiable-solve workload at its size (96^3 face-conductance
   diffusion), dL/dk and dL/db agree with the FP64 oracle's to 1e-6 relative,
@@ -105,6 +107,38 @@ def test_pcg_fp64_and_mixed_128(h, problem, oracle_hier, oracle_big):
     assert np.abs(rm).max() <= 1e-10 * np.abs(b).max() * 1.01
     assert np.linalg.norm(xm - xo) <= 1e-8 * np.linalg.norm(xo)
     assert np.linalg.norm(xm - xs) <= 1e-8 * np.linalg.norm(xs)
+    H.close()
+
+
+def test_uaamg_hierarchy_and_pcg_128(h, problem, oracle_big):
+    """Unsmoothed aggregation at the headline size: the exact-mode device
+    hierarchy equals the oracle's level for level bit for bit, and
+    UA-AMG-PCG (fast mode) takes the oracle's iteration count within one with
+    the solution within 1e-10 relative of the oracle's."""
+    A, b, _ = problem
+    d = DevCSR(A)
+    log("oracle UA setup 128^3")
+    ho = oracle_big.uaamg_setup(A)
+    log("device UA setup 128^3 (exact)")
+    He = lal.Hierarchy(h, A.m, A.nnz, d.row_ptr, d.col_ind, d.val, unsmoothed=True, exact=1)
+    assert He.levels == ho.levels, (He.levels, ho.levels)
+    for lvl in range(He.levels + 1):
+        for kind in ("A", "P", "R") if lvl < He.levels else ("A",):
+            grp, gci, gv, gm, gn = He.level(lvl, kind)
+            ref = ho.get(lvl, kind)
+            assert (gm, gn) == (ref.m, ref.n), (lvl, kind)
+            assert sha(grp, gci, gv) == sha(ref.row_ptr, ref.col_ind, ref.val), (lvl, kind)
+    He.close()
+    log("oracle / device UA-AMG-PCG 128^3")
+    xo, ito, _ = ho.pcg(b, rel_tol=1e-10, coarse_solver=1)
+    H = lal.Hierarchy(h, A.m, A.nnz, d.row_ptr, d.col_ind, d.val, unsmoothed=True)
+    x = torch.zeros(A.m, dtype=torch.float64, device="cuda")
+    info = H.pcg(dev(b), x, rel_tol=1e-10, abs_tol=1e-300, max_iter=500)
+    xg = host(x)
+    log(f"UA-AMG-PCG {info.iterations} it (oracle {ito}), "
+        f"rel. diff {np.linalg.norm(xg - xo) / np.linalg.norm(xo):.3e}")
+    assert info.converged and abs(info.iterations - ito) <= 1, (info.iterations, ito)
+    assert np.linalg.norm(xg - xo) <= 1e-10 * np.linalg.norm(xo)
     H.close()
 
 
