@@ -11,6 +11,8 @@ nnz = 14,581,760), x* ~ U[-1,1] (seed 1), b = A x*, x0 = 0, to 1e-10.
   solution;
 * the unsmoothed-aggregation hierarchy equals the oracle's bit for bit and
   UA-AMG-PCG takes its iteration count within one, solution within 1e-10;
+* SA-AMG-preconditioned BiCGSTAB and GMRES(30) take the oracle's iteration
+  counts (within 2 / 1 restart cycle) with true residuals as small;
 * one fast SpMV is within the recursive-summation bound of the oracle's;
 * on the differentiable-solve workload at its size (96^3 face-conductance
   diffusion), dL/dk and dL/db agree with the FP64 oracle's to 1e-6 relative,
@@ -139,6 +141,40 @@ def test_uaamg_hierarchy_and_pcg_128(h, problem, oracle_big):
         f"rel. diff {np.linalg.norm(xg - xo) / np.linalg.norm(xo):.3e}")
     assert info.converged and abs(info.iterations - ito) <= 1, (info.iterations, ito)
     assert np.linalg.norm(xg - xo) <= 1e-10 * np.linalg.norm(xo)
+    H.close()
+
+
+@pytest.mark.parametrize("solver", ["bicgstab", "gmres"])
+def test_krylov_saamg_128(h, problem, oracle_hier, oracle_big, solver):
+    """SA-AMG-preconditioned BiCGSTAB and GMRES(30) at the headline size
+    (fast V-cycle): the oracle's iteration count within 2 (restart cycles
+    within 1 for GMRES) and a true residual within 10x of the oracle's."""
+    A, b, _ = problem
+    d = DevCSR(A)
+    descr = lal.CsrDescr(h, A.m, A.m, A.nnz, d.row_ptr, d.col_ind)
+    Ad = (d.row_ptr, d.col_ind, d.val, A.m, A.nnz, descr)
+    H = lal.Hierarchy(h, A.m, A.nnz, d.row_ptr, d.col_ind, d.val)
+    x = torch.zeros(A.m, dtype=torch.float64, device="cuda")
+    log(f"oracle / device {solver} + SA-AMG 128^3")
+    if solver == "bicgstab":
+        xo, ko = oracle_big.bicgstab(A, b, precond=2, hier=oracle_hier, rel_tol=1e-10,
+                                     max_iter=300)
+        info = lal.bicgstab(h, Ad, dev(b), x, "saamg", H, rel_tol=1e-10, abs_tol=1e-300,
+                            max_iter=300)
+        slack = 2
+    else:
+        xo, ko = oracle_big.gmres(A, b, restart=30, precond=2, hier=oracle_hier, rel_tol=1e-10,
+                                  max_iter=50)
+        info = lal.gmres(h, Ad, dev(b), x, 30, "saamg", H, rel_tol=1e-10, abs_tol=1e-300,
+                         max_iter=50)
+        slack = 1
+    xg = host(x)
+    rr = np.abs(oracle_big.residual(A, xg, b)).max() / np.abs(b).max()
+    ro = np.abs(oracle_big.residual(A, xo, b)).max() / np.abs(b).max()
+    log(f"{solver}: {info.iterations} it (oracle {ko}), true rel. residual {rr:.2e} "
+        f"(oracle {ro:.2e}), rel. diff {np.linalg.norm(xg - xo) / np.linalg.norm(xo):.2e}")
+    assert info.converged and abs(info.iterations - ko) <= slack, (info.iterations, ko)
+    assert rr <= 10 * ro + 1e-14
     H.close()
 
 
