@@ -13,6 +13,8 @@ nnz = 14,581,760), x* ~ U[-1,1] (seed 1), b = A x*, x0 = 0, to 1e-10.
   UA-AMG-PCG takes its iteration count within one, solution within 1e-10;
 * SA-AMG-preconditioned BiCGSTAB and GMRES(30) take the oracle's iteration
   counts (within 2 / 1 restart cycle) with true residuals as small;
+* AMG as a solver in exact mode (Gauss-Seidel V-cycles, Jacobi W-cycles)
+  equals the oracle's bit for bit;
 * one fast SpMV is within the recursive-summation bound of the oracle's;
 * on the differentiable-solve workload at its size (96^3 face-conductance
   diffusion), dL/dk and dL/db agree with the FP64 oracle's to 1e-6 relative,
@@ -27,7 +29,7 @@ import torch
 
 from linalglib_amd import lal, problems as pb
 from oracle.oracle import CSR, Oracle
-from tests.gpu_util import DevCSR, dev, host, log
+from tests.gpu_util import DevCSR, bits_equal, dev, host, log
 
 pytestmark = pytest.mark.gpu
 
@@ -176,6 +178,26 @@ def test_krylov_saamg_128(h, problem, oracle_hier, oracle_big, solver):
     assert info.converged and abs(info.iterations - ko) <= slack, (info.iterations, ko)
     assert rr <= 10 * ro + 1e-14
     H.close()
+
+
+@pytest.mark.parametrize("cycle,smoother", [("v", "gs"), ("w", "jacobi")])
+def test_amg_solve_exact_bitwise_128(h, problem, oracle_hier, cycle, smoother):
+    """AMG as a solver in exact mode at the headline size: three cycles
+    (Gauss-Seidel V-cycles with their level-scheduled sweeps, Jacobi
+    W-cycles) bit for bit equal to the oracle's."""
+    A, b, _ = problem
+    d = DevCSR(A)
+    He = lal.Hierarchy(h, A.m, A.nnz, d.row_ptr, d.col_ind, d.val, exact=1)
+    x = torch.zeros(A.m, dtype=torch.float64, device="cuda")
+    log(f"amg_solve exact {cycle}/{smoother} 128^3, 3 cycles")
+    info = He.amg_solve(dev(b), x, 1, 1, cycle, smoother, rel_tol=1e-30, abs_tol=1e-300,
+                        max_iter=3)
+    xo, ko, _ = oracle_hier.amg_solve(b, cycle={"v": 0, "w": 1}[cycle],
+                                      smoother={"jacobi": 0, "gs": 1}[smoother], rel_tol=1e-30,
+                                      max_cycle=3)
+    assert info.iterations == ko, (info.iterations, ko)
+    assert bits_equal(host(x), xo)
+    He.close()
 
 
 def test_spmv_fast_128(h, problem, oracle_big):
