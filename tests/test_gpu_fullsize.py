@@ -6,9 +6,9 @@ nnz = 14,581,760), x* ~ U[-1,1] (seed 1), b = A x*, x0 = 0, to 1e-10.
 * the fast-mode FP64 SA-AMG-PCG solve takes the oracle's iteration count within
   one, its solution is within 1e-10 relative of the oracle's and its true
   residual meets the tolerance;
-* the reported mixed scheme (FP32 PCG with FP64 reliable updates) meets the
-  tolerance on the true FP64 residual and lands within 1e-8 of the oracle's
-  solution;
+* the reported mixed scheme (FP32 PCG with FP64 reliable updates), and the
+  FP32-V-cycle and iterative-refinement schemes, meet the tolerance on the
+  true FP64 residual and land within 1e-8 of the oracle's solution;
 * the unsmoothed-aggregation hierarchy equals the oracle's bit for bit and
   UA-AMG-PCG takes its iteration count within one, solution within 1e-10;
 * SA-AMG-preconditioned BiCGSTAB and GMRES(30) take the oracle's iteration
@@ -111,6 +111,18 @@ def test_pcg_fp64_and_mixed_128(h, problem, oracle_hier, oracle_big):
     assert np.abs(rm).max() <= 1e-10 * np.abs(b).max() * 1.01
     assert np.linalg.norm(xm - xo) <= 1e-8 * np.linalg.norm(xo)
     assert np.linalg.norm(xm - xs) <= 1e-8 * np.linalg.norm(xs)
+    # the other two mixed schemes: FP64 PCG with the FP32 V-cycle (same
+    # iteration count as FP64) and FP32 PCG inside FP64 refinement
+    for name, solve in (("vcycle", H.pcg_mixed), ("ir", H.ir)):
+        x.zero_()
+        info = solve(B, x, rel_tol=1e-10, abs_tol=1e-300)
+        xk = host(x)
+        rk = oracle_big.residual(A, xk, b)
+        assert info.converged, name
+        if name == "vcycle":
+            assert abs(info.iterations - ito) <= 3, (info.iterations, ito)
+        assert np.abs(rk).max() <= 1e-10 * np.abs(b).max() * 1.01, name
+        assert np.linalg.norm(xk - xo) <= 1e-8 * np.linalg.norm(xo), name
     H.close()
 
 
