@@ -18,7 +18,7 @@ nnz = 14,581,760), x* ~ U[-1,1] (seed 1), b = A x*, x0 = 0, to 1e-10.
 * one fast SpMV is within the recursive-summation bound of the oracle's;
 * on the differentiable-solve workload at its size (96^3 face-conductance
   diffusion), dL/dk and dL/db agree with the FP64 oracle's to 1e-6 relative,
-  in FP64 and with the reliable-update mixed solves.
+  in FP64 and in each mixed mode.
 """
 import hashlib
 import os
@@ -241,7 +241,7 @@ def _face_cells(rp, ci, m, entry, face, sign, nf):
     return p, q
 
 
-@pytest.mark.parametrize("mixed", [False, "reliable"])
+@pytest.mark.parametrize("mixed", [False, "reliable", "vcycle", "ir"])
 def test_adjoint_gradient_96(oracle_big, mixed):
     """The differentiable-solve workload at its size: 96^3 finite-volume
     diffusion with per-face conductances k = exp(z), L = 0.5 ||x - x_t||^2;
