@@ -80,6 +80,11 @@ def test_bench_byte_formulas():
     assert bench.spmv_bytes(m, m, nnz, 4, 4) == 141_819_908
     # one-level "hierarchy": only the PCG terms remain
     assert bench.solve_bytes([(10, 30, 0)], 1, 8) == bench.spmv_bytes(10, 10, 30, 8, 8) + 14 * 8 * 10
+    # one reliable update: an FP64 residual (SpMV bytes plus b read and r
+    # written), the FP32 correction read with x read and written, y cleared,
+    # the norm of r and its scaled FP32 copy
+    upd = bench.reliable_update_bytes(m, nnz)
+    assert upd == bench.spmv_bytes(m, m, nnz, 8, 8) + 8 * m + (4 + 16) * m + 4 * m + 8 * m + 12 * m
 
 
 def test_poisson_generators_symmetric_and_sorted():
