@@ -10,11 +10,19 @@ preconditioner application, PCG to ||r||_inf / ||r0||_inf <= 1e-10.
 
 A step is one complete solve from x = 0 with the hierarchy resident in HBM
 (setup is timed once and reported separately).  `value` is the mixed
-precision solve time; two mixed schemes are timed:
-  vcycle  FP64 PCG (x, r, p, A p, inner products) with the V-cycle applied on
-          FP32 copies of the hierarchy (the default, `value`)
-  ir      FP32 SA-AMG-PCG to 1e-4 inside FP64 iterative refinement to 1e-10
+precision solve time of the scheme --mixed names; three mixed schemes are
+timed:
+  reliable  FP32 PCG and V-cycle (FP64 inner-product accumulation) with FP64
+            reliable updates of x and the true residual every 1e-2 of
+            residual reduction (the default, `value`)
+  vcycle    FP64 PCG (x, r, p, A p, inner products) with the V-cycle applied
+            on FP32 copies of the hierarchy
+  ir        FP32 SA-AMG-PCG to 1e-4 inside FP64 iterative refinement to 1e-10
 and the plain FP64 solve time is reported beside them.
+
+The CPU baseline runs LinAlgLib's own host library on the same problem with
+every host core the job was given (OMP_NUM_THREADS, else the affinity mask)
+and again with one thread.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1]
 """
@@ -99,10 +107,20 @@ def measured_traffic():
         return None
 
 
-def cpu_baseline(rp, ci, v, m, b, threads, tol=1e-10, name=None):
+def host_threads():
+    """Host cores this job may use: OMP_NUM_THREADS when the launcher set it
+    (the share of the machine given to one GPU), else the affinity mask."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(rp, ci, v, m, b, threads, tol=1e-10, name=None, one_thread=True):
     """LinAlgLib's own host path (oracle/_ref, compiled from its sources) on
     the same problem when it was built, else the C oracle; SA setup and one
-    SA-AMG-PCG solve to `tol`, timed on this machine's host cores."""
+    SA-AMG-PCG solve to `tol`, timed on this machine's host cores (`threads`,
+    then one thread on the same hierarchy when one_thread is set)."""
     here = os.path.dirname(os.path.abspath(__file__))
     sys.path.insert(0, here)
     from oracle.oracle import CSR, Oracle, Reference
@@ -119,6 +137,13 @@ def cpu_baseline(rp, ci, v, m, b, threads, tol=1e-10, name=None):
     out = h.pcg(b, rel_tol=tol, abs_tol=1e-300, max_iter=1000)
     t2 = time.perf_counter()
     iters = out[1]
+    solve1 = None
+    if one_thread and threads > 1:
+        lib.set_threads(1)
+        t3 = time.perf_counter()
+        h.pcg(b, rel_tol=tol, abs_tol=1e-300, max_iter=1000)
+        solve1 = round(time.perf_counter() - t3, 4)
+        lib.set_threads(threads)
     return {
         "value": round(t2 - t1, 4),
         "unit": f"s per SA-AMG-PCG solve to {tol:g} (FP64)",
@@ -130,6 +155,7 @@ def cpu_baseline(rp, ci, v, m, b, threads, tol=1e-10, name=None):
                    f"{h.levels} levels), OMP_NUM_THREADS={threads}"),
         "setup_s": round(t1 - t0, 3),
         "iterations": iters,
+        "value_1_thread": solve1 if solve1 is not None else round(t2 - t1, 4),
     }
 
 
@@ -225,9 +251,10 @@ def run_config(args):
     line["speedup_reliable_vs_fp64"] = round(res["fp64"]["median_s"] / res["reliable"]["median_s"],
                                              3)
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        threads = args.cpu_threads or host_threads()
         try:
-            line["cpu_baseline"] = cpu_baseline(rp, ci, v, m, b, threads, tol, name)
+            line["cpu_baseline"] = cpu_baseline(rp, ci, v, m, b, threads, tol, name,
+                                                one_thread=False)
         except Exception as e:  # the baseline is context only
             line["cpu_baseline"] = {"value": None, "cores": threads, "sample": f"not run: {e}"}
     print(json.dumps(line), flush=True)
@@ -294,7 +321,7 @@ def run_amg(args):
         sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
         from oracle.oracle import CSR, Oracle, Reference
 
-        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        threads = args.cpu_threads or host_threads()
         lib = Reference() if Reference.available() else Oracle()
         lib.set_threads(threads)
         print("# amg: host setup and one cycle of each kind on the CPU", file=sys.stderr,
@@ -378,7 +405,7 @@ def run_krylov(args):
         sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
         from oracle.oracle import CSR, Oracle, Reference
 
-        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        threads = args.cpu_threads or host_threads()
         rp, ci, v, m = problems.poisson3d(128)
         b = problems.rhs_from_solution(rp, ci, v, problems.uniform(m, seed=1))
         Ac = CSR(rp, ci, v, m, m)
@@ -610,7 +637,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        threads = args.cpu_threads or host_threads()
         try:
             cpu = cpu_baseline(rp, ci, v, m, b, threads)
         except Exception as e:  # the baseline is context only

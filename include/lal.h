@@ -85,7 +85,9 @@ int lal_axpbypgz_f64(lal_handle h, int n, double alpha, const double* x, double 
                      const double* y, double gamma, double* z);
 int lal_axpy_f32(lal_handle h, int n, float alpha, const float* x, float* y);
 int lal_axpby_f32(lal_handle h, int n, float alpha, const float* x, float beta, float* y);
-/* Result returned to host memory (blocks until ready). FP64 accumulation. */
+/* Result returned to host memory (blocks until ready). FP64 accumulation.
+ * lal_dot_f64 with n <= 1024 sums in index order (bit-identical to the host
+ * loop at one OpenMP thread); longer vectors use a fixed-order tree. */
 int lal_dot_f64(lal_handle h, int n, const double* x, const double* y, double* result);
 int lal_dot_f32(lal_handle h, int n, const float* x, const float* y, double* result);
 int lal_norm_inf_f64(lal_handle h, int n, const double* x, double* result);
@@ -141,7 +143,10 @@ int lal_transpose_f64(lal_handle h, int m, int n, int nnz, const int* row_ptr,
 /* C = A B, two passes.  Pass 1 fills row_ptr_C (m+1, device) and returns nnz
  * to *nnz_C (host).  Pass 2 fills col_ind_C / val_C with columns ascending
  * and every entry folded in A-row-then-B-row order from its first product,
- * bit-identical to backend/host/host_csrgemm.cpp:91-183. */
+ * bit-identical to backend/host/host_csrgemm.cpp:91-183.  Both passes check
+ * the operands first: a column index of A outside [0, k) or of B outside
+ * [0, n) gives LAL_ERR_INVALID, and a row of B that repeats a column index
+ * gives LAL_ERR_UNSUPPORTED (the host loop would sum the repeats). */
 int lal_csrgemm_nnz(lal_handle h, int m, int k, int n, int nnz_A, const int* row_ptr_A,
                     const int* col_ind_A, int nnz_B, const int* row_ptr_B,
                     const int* col_ind_B, int* row_ptr_C, int* nnz_C);
@@ -251,6 +256,17 @@ void lal_solve_default_options(lal_solve_options* opts);
 /* SA-AMG-preconditioned CG in FP64: x holds the initial guess on entry. */
 int lal_pcg_solve_f64(lal_handle h, lal_hierarchy hier, const double* b, double* x,
                       const lal_solve_options* opts, lal_solve_info* info);
+/* cg_solver::solve(A, x, b, precond, control) (krylov/cg.cpp:136-223) with an
+ * SA-AMG preconditioner as the reference takes them, independently: the
+ * Krylov products q = A p and the initial residual use the caller's square
+ * FP64 CSR matrix A (descr from lal_csrmv_analysis, or NULL for one lane per
+ * row), the preconditioner is one V-cycle of hier, which may have been built
+ * for another matrix of the same size.  lal_pcg_solve_f64 is this call with
+ * A = the matrix hier was built from (its own resident copy). */
+int lal_pcg_solve_op_f64(lal_handle h, lal_csrmv_descr descr, int m, int nnz,
+                         const int* row_ptr, const int* col_ind, const double* val,
+                         lal_hierarchy hier, const double* b, double* x,
+                         const lal_solve_options* opts, lal_solve_info* info);
 /* Mixed precision: FP64 outer residual r = b - A x and update x += d, FP32
  * inner SA-AMG-PCG (FP64 inner products) to inner_rel_tol; needs build_f32. */
 int lal_ir_solve(lal_handle h, lal_hierarchy hier, const double* b, double* x,

@@ -370,17 +370,30 @@ namespace lal_linalg
         vector<int>     rp, ci;
         vector<double>  v;
         mutable lal_csrmv_descr descr = nullptr;
+        int             exact = 0;
 
         lal_csrmv_descr analysed() const
         {
             if(!descr)
-                check(lal_csrmv_analysis(context(), m, n, nnz, rp.get_vec(), ci.get_vec(), 0,
-                                         &descr),
+                check(lal_csrmv_analysis(context(), m, n, nnz, rp.get_vec(), ci.get_vec(),
+                                         exact, &descr),
                       "csrmv_analysis");
             return descr;
         }
 
     public:
+        // Backend extension (no LinAlgLib counterpart): with exact = true every
+        // product and residual with this matrix sums rows sequentially in CSR
+        // order, bit-identical to the host path (host_matrix_vector.cpp:35-84).
+        void set_exact(bool e)
+        {
+            if((int)e != exact)
+            {
+                lal_csrmv_descr_destroy(descr);
+                descr = nullptr;
+                exact = e;
+            }
+        }
         // The SpMV analysis of this pattern (built on first use).
         lal_csrmv_descr descriptor() const
         {
@@ -840,7 +853,9 @@ namespace lal_linalg
         }
 
         // cg.cpp:136-223, every step a device call.  With an saamg_precond
-        // the whole loop runs in the library (lal_pcg_solve_f64) instead.
+        // the whole loop runs in the library instead (lal_pcg_solve_op_f64:
+        // the Krylov products on A, the V-cycle of the preconditioner's
+        // hierarchy, which may have been built for another matrix).
         int solve_precond(const csr_matrix<double>& A, vector<double>& x,
                           const vector<double>& b, const preconditioner* precond,
                           iter_control control)
@@ -855,7 +870,9 @@ namespace lal_linalg
                 o.n1       = sa->n1;
                 o.n2       = sa->n2;
                 lal_solve_info info{};
-                check(lal_pcg_solve_f64(context(), sa->h.h, b.get_vec(), x.get_vec(), &o, &info),
+                check(lal_pcg_solve_op_f64(context(), A.descriptor(), A.get_m(), A.get_nnz(),
+                                           A.get_row_ptr(), A.get_col_ind(), A.get_val(),
+                                           sa->h.h, b.get_vec(), x.get_vec(), &o, &info),
                       "cg_solver::solve_precond");
                 return info.iterations;
             }

@@ -128,9 +128,41 @@ namespace
         }
     }
 
+    // Dot products of at most SEQ_DOT entries are summed by one lane in index
+    // order, s = s + x_i * y_i with both roundings, which is the host loop
+    // (host_axpy.cpp:84-97) at one OpenMP thread bit for bit.  The wave first
+    // stages both vectors in LDS so that the sequential chain reads LDS only.
+    constexpr int SEQ_DOT = 1024;
+
+    __global__ void __launch_bounds__(64)
+        seq_dot_kernel(int n, const double* __restrict__ x, const double* __restrict__ y,
+                       double* __restrict__ out)
+    {
+        __shared__ double sx[SEQ_DOT], sy[SEQ_DOT];
+        for(int i = threadIdx.x; i < n; i += 64)
+        {
+            sx[i] = x[i];
+            sy[i] = y[i];
+        }
+        __syncthreads();
+        if(threadIdx.x != 0)
+            return;
+        double s = 0.0;
+        for(int i = 0; i < n; i++)
+            s = s + sx[i] * sy[i];
+        *out = s;
+    }
+
     template <typename T, int OP>
     int reduce_dev(lal_context* c, int n, const T* x, const T* y, double* d_out)
     {
+        if(OP == 0 && sizeof(T) == 8 && n > 0 && n <= SEQ_DOT)
+        {
+            seq_dot_kernel<<<1, 64, 0, c->stream>>>(n, (const double*)x, (const double*)y,
+                                                    d_out);
+            LAL_LAUNCHED();
+            return LAL_OK;
+        }
         int g = n <= 0 ? 1 : lal::grid_for(n, LAL_RED_THREADS, LAL_RED_BLOCKS);
         reduce_kernel<T, OP>
             <<<g, LAL_RED_THREADS, 0, c->stream>>>(n, x, y, c->d_part, c->d_ticket, d_out);

@@ -606,13 +606,16 @@ namespace
     }
 
     // cg_solver::solve_precond on the device.  T = double uses the FP64
-    // levels and workspaces, T = float the FP32 ones.
+    // levels and workspaces, T = float the FP32 ones.  The Krylov products
+    // use `op` when given (a caller's matrix of the finest level's size, FP64
+    // only), else the matrix the hierarchy was built from.
     template <typename T>
     int pcg(lal_hierarchy_t* h, const T* b, T* x, T* res, T* z, T* p, T* q, double rel_tol,
-            double abs_tol, int max_iter, int n1, int n2, lal_solve_info* info)
+            double abs_tol, int max_iter, int n1, int n2, lal_solve_info* info,
+            const csr_dev* op = nullptr)
     {
         lal_context*   c  = h->c;
-        const csr_dev& A  = h->L[0].A;
+        const csr_dev& A  = op ? *op : h->L[0].A;
         const int      n  = A.m;
         const T*       Av = vals<T>(A);
         double*        s  = h->s;
@@ -688,8 +691,12 @@ namespace
     // FP32 levels.  The residual's FP32 copy and the V-cycle's first Jacobi
     // iterate are written by the FP64 update kernel, and <z, r> / p = z + beta p
     // read the FP32 z directly, so the precision change costs no extra pass.
-    // The V-cycle input is r rounded to FP32: residual entries must lie in
-    // FP32's normal range.
+    // The V-cycle input is r scaled by sigma = 2^-k, k the binary exponent of
+    // ||r0||_inf, and rounded to FP32, so its entries stay in FP32's range
+    // whatever b's magnitude.  PCG is invariant to a constant scaling of the
+    // preconditioner (alpha takes 1/sigma, beta is unchanged), and a power of
+    // two scales every iterate exactly, so the iterates are those of the
+    // unscaled V-cycle.
     int pcg_mixed(lal_hierarchy_t* h, const double* b, double* x, double rel_tol,
                   double abs_tol, int max_iter, int n1, int n2, lal_solve_info* info)
     {
@@ -711,7 +718,9 @@ namespace
         LAL_TRY(lal::norm_partials<double>(c, n, res, part_r));
         LAL_TRY(post_scalar(h, 1, part_r, lal::S_RNORM0, &seq));
         LAL_TRY(wait_scalar(h, seq, lal::S_RNORM0, &init));
-        LAL_TRY(lal::scale_cast_f64_f32(c, n, res, 1.0, r32));
+        const double sigma = init > 0.0 && std::isfinite(init) ? std::ldexp(1.0, -std::ilogb(init))
+                                                               : 1.0;
+        LAL_TRY(lal::scale_cast_f64_f32(c, n, res, sigma, r32));
         LAL_TRY(precond<float>(h, r32, z32, n1, n2));
         LAL_TRY(lal_convert_f32_f64(c, z32, p, (size_t)n));
         int cur = 0;
@@ -728,7 +737,7 @@ namespace
             LAL_TRY(lal::dot_partials<double, double>(c, n, q, p, part_pq));
             LAL_TRY(lal::update_xr<double, float>(c, n, p, q, x, res, part_pq, part_g[cur],
                                                   part_r, s + lal::S_ALPHA, W0.dinv,
-                                                  fast ? W0.x : nullptr, r32));
+                                                  fast ? W0.x : nullptr, r32, sigma));
             LAL_TRY(post_scalar(h, 1, part_r, lal::S_RNORM, &seq));
             const bool near = lal::near_tolerance(r_prev, rate, init, rel_tol, abs_tol);
             if(!near)
@@ -1089,6 +1098,33 @@ extern "C" int lal_pcg_solve_f64(lal_handle c, lal_hierarchy h, const double* b,
     LAL_REQUIRE(o.max_iter >= 0 && o.n1 >= 0 && o.n2 >= 0);
     return pcg<double>(h, b, x, h->res, h->z, h->p, h->q, o.rel_tol, o.abs_tol, o.max_iter,
                        o.n1, o.n2, info);
+}
+
+extern "C" int lal_pcg_solve_op_f64(lal_handle c, lal_csrmv_descr descr, int m, int nnz,
+                                    const int* rp, const int* ci, const double* v,
+                                    lal_hierarchy h, const double* b, double* x,
+                                    const lal_solve_options* opts, lal_solve_info* info)
+{
+    LAL_REQUIRE(c != nullptr && h != nullptr && b != nullptr && x != nullptr && c == h->c);
+    LAL_REQUIRE(m == h->L[0].A.m && nnz >= 0 && rp != nullptr);
+    LAL_REQUIRE(nnz == 0 || (ci != nullptr && v != nullptr));
+    LAL_REQUIRE(descr == nullptr || (descr->m == m && descr->nnz == nnz));
+    lal_solve_options o;
+    lal_solve_default_options(&o);
+    if(opts)
+        o = *opts;
+    LAL_REQUIRE(o.max_iter >= 0 && o.n1 >= 0 && o.n2 >= 0);
+    // a non-owning view of the caller's matrix (never freed here)
+    csr_dev A;
+    A.m   = m;
+    A.n   = m;
+    A.nnz = nnz;
+    A.rp  = const_cast<int*>(rp);
+    A.ci  = const_cast<int*>(ci);
+    A.v   = const_cast<double*>(v);
+    A.d   = descr;
+    return pcg<double>(h, b, x, h->res, h->z, h->p, h->q, o.rel_tol, o.abs_tol, o.max_iter,
+                       o.n1, o.n2, info, &A);
 }
 
 extern "C" int lal_pcg_solve_mixed(lal_handle c, lal_hierarchy h, const double* b, double* x,

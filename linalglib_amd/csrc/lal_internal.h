@@ -126,4 +126,6 @@ namespace lal
     int scan_i32_dev(lal_context* c, int n, int32_t* x);
     int scan_i64_dev(lal_context* c, int n, int64_t* x);
     int max_i64_dev(lal_context* c, int n, const int64_t* x, int64_t* h_out);
+    // LAL_ERR_INVALID unless every ci[0..nnz) lies in [0, n) (blocks until known).
+    int check_columns(lal_context* c, int nnz, int n, const int* ci);
 }

@@ -528,6 +528,60 @@ namespace lal
     }
 }
 
+namespace
+{
+    // flag = 1 when a row of B holds a column index twice.  Strictly
+    // ascending rows (the usual case) take one pass; other rows compare every
+    // pair.
+    __global__ void __launch_bounds__(256)
+        dup_cols_kernel(int k, const int* __restrict__ brp, const int* __restrict__ bci,
+                        int* __restrict__ flag)
+    {
+        const int row = blockIdx.x * 256 + threadIdx.x;
+        if(row >= k)
+            return;
+        const int rs = brp[row], re = brp[row + 1];
+        bool      ascending = true;
+        for(int p = rs + 1; p < re && ascending; p++)
+            ascending = bci[p] > bci[p - 1];
+        if(ascending)
+            return;
+        for(int p = rs + 1; p < re; p++)
+            for(int q = rs; q < p; q++)
+                if(bci[q] == bci[p])
+                {
+                    atomicOr(flag, 1);
+                    return;
+                }
+    }
+
+    // Validates the operands of the C ABI products: column indices of A in
+    // [0, k) and of B in [0, n), and no column index repeated within a row of
+    // B (the hash pass folds each product into one slot per column, so two
+    // lanes meeting the same column of one B row would race; such a matrix is
+    // refused rather than summed).
+    int check_product_operands(lal_context* c, int k, int n, int nnz_A, const int* aci,
+                               int nnz_B, const int* brp, const int* bci)
+    {
+        LAL_TRY(lal::check_columns(c, nnz_A, k, aci));
+        LAL_TRY(lal::check_columns(c, nnz_B, n, bci));
+        if(k == 0 || nnz_B == 0)
+            return LAL_OK;
+        LAL_HIP(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
+        dup_cols_kernel<<<lal::grid_for(k, 256), 256, 0, c->stream>>>(k, brp, bci, c->d_flag);
+        LAL_LAUNCHED();
+        LAL_HIP(hipMemcpyAsync(c->h_flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost,
+                               c->stream));
+        LAL_HIP(hipStreamSynchronize(c->stream));
+        if(c->h_flag[0])
+        {
+            lal::set_last_error("csrgemm: a row of B repeats a column index");
+            return LAL_ERR_UNSUPPORTED;
+        }
+        return LAL_OK;
+    }
+}
+
 extern "C" int lal_transpose_f64(lal_handle c, int m, int n, int nnz, const int* rp,
                                  const int* ci, const double* v, int* trp, int* tci, double* tv)
 {
@@ -543,6 +597,7 @@ extern "C" int lal_csrgemm_nnz(lal_handle c, int m, int k, int n, int nnz_A, con
     LAL_REQUIRE(c != nullptr && m >= 0 && k >= 0 && n >= 0 && nnz_A >= 0 && nnz_B >= 0);
     LAL_REQUIRE(crp != nullptr && nnz_C != nullptr && arp != nullptr && brp != nullptr);
     LAL_REQUIRE((nnz_A == 0 || aci) && (nnz_B == 0 || bci));
+    LAL_TRY(check_product_operands(c, k, n, nnz_A, aci, nnz_B, brp, bci));
     return lal::csrgemm_nnz(c, m, k, n, arp, aci, brp, bci, crp, nnz_C);
 }
 
@@ -554,5 +609,6 @@ extern "C" int lal_csrgemm_solve_f64(lal_handle c, int m, int k, int n, int nnz_
     LAL_REQUIRE(c != nullptr && m >= 0 && k >= 0 && n >= 0 && nnz_A >= 0 && nnz_B >= 0);
     LAL_REQUIRE(crp != nullptr && arp != nullptr && brp != nullptr);
     LAL_REQUIRE((nnz_A == 0 || (aci && av)) && (nnz_B == 0 || (bci && bv)));
+    LAL_TRY(check_product_operands(c, k, n, nnz_A, aci, nnz_B, brp, bci));
     return lal::csrgemm_solve_f64(c, m, k, n, arp, aci, av, brp, bci, bv, crp, cci, cv);
 }

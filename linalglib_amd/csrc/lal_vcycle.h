@@ -79,12 +79,13 @@ namespace lal
     // is given (mapped pinned memory), into *host_val then *host_seq = seq.
     int finish_partials(lal_context* c, int op, const double* part, double* slot,
                         double* host_val, unsigned* host_seq, unsigned seq);
-    // x += alpha p, r -= alpha q, partial max|r|; with x0 also x0 = (S)r * dinv,
-    // with rs also rs = (S)r (the preconditioner's precision S).
+    // x += alpha p, r -= alpha q, partial max|r|; with x0 also
+    // x0 = (S)(r * rs_scale) * dinv, with rs also rs = (S)(r * rs_scale) (the
+    // preconditioner's precision S; rs_scale = 1 leaves r as it is).
     template <typename T, typename S>
     int update_xr(lal_context* c, int n, const T* p, const T* q, T* x, T* r,
                   const double* part_pq, const double* part_gamma, double* part_r,
-                  double* s_alpha, const S* dinv, S* x0, S* rs);
+                  double* s_alpha, const S* dinv, S* x0, S* rs, double rs_scale = 1.0);
     // x = b * dinv
     template <typename T>
     int mul_elem(lal_context* c, int n, const T* b, const T* dinv, T* x);

@@ -335,7 +335,7 @@ namespace
                          const double* __restrict__ part_pq,
                          const double* __restrict__ part_gamma, double* __restrict__ part_r,
                          double* __restrict__ s_alpha, const S* __restrict__ dinv,
-                         S* __restrict__ x0, S* __restrict__ rs)
+                         S* __restrict__ x0, S* __restrict__ rs, T rs_scale)
     {
         const int stride = gridDim.x * SPMV_THREADS;
         int       i      = blockIdx.x * SPMV_THREADS + threadIdx.x;
@@ -351,7 +351,7 @@ namespace
             x[at]    = a == (T)1 ? pi + xi : a * pi + xi;
             T rn     = ma == (T)1 ? qi + ri : ma * qi + ri;
             r[at]    = rn;
-            const S r_s = (S)rn;
+            const S r_s = (S)(rn * rs_scale);
             if(rs)
                 rs[at] = r_s;
             if(x0)
@@ -575,10 +575,10 @@ namespace lal
     template <typename T, typename S>
     int update_xr(lal_context* c, int n, const T* p, const T* q, T* x, T* r,
                   const double* part_pq, const double* part_gamma, double* part_r,
-                  double* s_alpha, const S* dinv, S* x0, S* rs)
+                  double* s_alpha, const S* dinv, S* x0, S* rs, double rs_scale)
     {
         update_xr_kernel<T, S><<<RB, SPMV_THREADS, 0, c->stream>>>(
-            n, p, q, x, r, part_pq, part_gamma, part_r, s_alpha, dinv, x0, rs);
+            n, p, q, x, r, part_pq, part_gamma, part_r, s_alpha, dinv, x0, rs, (T)rs_scale);
         LAL_LAUNCHED();
         return LAL_OK;
     }
@@ -654,7 +654,7 @@ namespace lal
     template int norm_partials<T>(lal_context*, int, const T*, double*);                     \
     template int update_xr<T, T>(lal_context*, int, const T*, const T*, T*, T*,              \
                                  const double*, const double*, double*, double*, const T*,   \
-                                 T*, T*);                                                    \
+                                 T*, T*, double);                                                    \
     template int update_p<T, T>(lal_context*, int, const T*, T*, const double*,              \
                                 const double*);                                              \
     template int mul_elem<T>(lal_context*, int, const T*, const T*, T*);                     \
@@ -668,7 +668,8 @@ namespace lal
                                              double*);
     template int update_xr<double, float>(lal_context*, int, const double*, const double*,
                                           double*, double*, const double*, const double*,
-                                          double*, double*, const float*, float*, float*);
+                                          double*, double*, const float*, float*, float*,
+                                          double);
     template int update_p<float, double>(lal_context*, int, const float*, double*,
                                          const double*, const double*);
 }

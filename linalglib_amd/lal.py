@@ -104,6 +104,8 @@ _SIGS = {
     "lal_amg_precond_f64": [vp, vp, vp, vp, i32, i32],
     "lal_solve_default_options": [C.POINTER(SolveOptions)],
     "lal_pcg_solve_f64": [vp, vp, vp, vp, C.POINTER(SolveOptions), C.POINTER(SolveInfo)],
+    "lal_pcg_solve_op_f64": [vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, C.POINTER(SolveOptions),
+                             C.POINTER(SolveInfo)],
     "lal_ir_solve": [vp, vp, vp, vp, C.POINTER(SolveOptions), C.POINTER(SolveInfo)],
     "lal_pcg_solve_mixed": [vp, vp, vp, vp, C.POINTER(SolveOptions), C.POINTER(SolveInfo)],
     "lal_pcg_solve_reliable": [vp, vp, vp, vp, C.POINTER(SolveOptions), C.POINTER(SolveInfo)],
@@ -294,6 +296,17 @@ class Hierarchy:
         info = SolveInfo()
         call("lal_pcg_solve_f64", self.handle.h, self.hp, ptr(b), ptr(x), C.byref(o),
              C.byref(info))
+        return info
+
+    def pcg_op(self, A, b, x, **opts):
+        """PCG whose Krylov products use A = (row_ptr, col_ind, val, m, nnz,
+        descr or None), a matrix of this hierarchy's size, preconditioned by
+        one V-cycle of this hierarchy (cg_solver::solve(A, x, b, precond))."""
+        rp, ci, v, m, nnz, d = A
+        o = solve_options(**opts)
+        info = SolveInfo()
+        call("lal_pcg_solve_op_f64", self.handle.h, d.d if d is not None else None, m, nnz,
+             ptr(rp), ptr(ci), ptr(v), self.hp, ptr(b), ptr(x), C.byref(o), C.byref(info))
         return info
 
     def ir(self, b, x, **opts):

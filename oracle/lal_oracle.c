@@ -739,15 +739,14 @@ void or_jacobi_iteration(int m, const int* rp, const int* ci, const double* v, d
     }
 }
 
-/* Dense Cholesky of the coarsest operator (no reference counterpart: the
- * reference runs 1000 Gauss-Seidel sweeps there). */
-static void coarse_cholesky_solve(const or_csr* A, double* x, const double* b)
+/* Dense Cholesky factor of an SPD matrix held row-major in L (N x N): on
+ * return the lower triangle holds L with L L^T = A, column by column
+ * (Cholesky-Banachiewicz order, every sum folded left to right); the strict
+ * upper triangle is left as it was.  No reference counterpart: the
+ * reference's coarsest solve is 1000 Gauss-Seidel sweeps (amg.cpp:238-264);
+ * this is the dense-factor option of the coarsest level. */
+void or_cholesky_factor(int N, double* L)
 {
-    int     N = A->m;
-    double* L = (double*)calloc((size_t)N * N, sizeof(double));
-    for(int i = 0; i < N; i++)
-        for(int j = A->row_ptr[i]; j < A->row_ptr[i + 1]; j++)
-            L[(size_t)i * N + A->col_ind[j]] = A->val[j];
     for(int j = 0; j < N; j++)
     {
         double s = L[(size_t)j * N + j];
@@ -763,6 +762,18 @@ static void coarse_cholesky_solve(const or_csr* A, double* x, const double* b)
             L[(size_t)i * N + j] = t / d;
         }
     }
+}
+
+/* x = A^-1 b for the coarsest operator: densify, factor, then forward and
+ * back substitution with L and L^T. */
+static void coarse_cholesky_solve(const or_csr* A, double* x, const double* b)
+{
+    int     N = A->m;
+    double* L = (double*)calloc((size_t)N * N, sizeof(double));
+    for(int i = 0; i < N; i++)
+        for(int j = A->row_ptr[i]; j < A->row_ptr[i + 1]; j++)
+            L[(size_t)i * N + A->col_ind[j]] = A->val[j];
+    or_cholesky_factor(N, L);
     for(int i = 0; i < N; i++)
     {
         double t = b[i];
@@ -1202,18 +1213,21 @@ static int converges(double res, double init, double rel_tol, double abs_tol)
     return 0;
 }
 
-/* krylov/cg.cpp:136-223 (solve_precond), restart_iter = -1. */
-int or_pcg_saamg(const or_hierarchy* h, double* x, const double* b, double rel_tol,
-                 double abs_tol, int max_iter, int n1, int n2, int coarse_solver,
-                 double* res_norm)
+/* krylov/cg.cpp:136-223 (solve_precond), restart_iter = -1, with the Krylov
+ * products on the operator (m, rp, ci, v) and the preconditioner one V-cycle
+ * of hierarchy h, which may have been built for another matrix of the same
+ * size: cg_solver::solve(A, x, b, precond, control) takes A and precond
+ * independently (cg.cpp:165-214 uses A, precond->solve the hierarchy). */
+int or_pcg_saamg_op(int m, const int* rp, const int* ci, const double* v, const or_hierarchy* h,
+                    double* x, const double* b, double rel_tol, double abs_tol, int max_iter,
+                    int n1, int n2, int coarse_solver, double* res_norm)
 {
-    const or_csr* A   = &h->A[0];
-    int           n   = A->m;
-    double*       res = (double*)calloc((size_t)n + 1, sizeof(double));
-    double*       z   = (double*)calloc((size_t)n + 1, sizeof(double));
-    double*       p   = (double*)calloc((size_t)n + 1, sizeof(double));
+    int     n   = m;
+    double* res = (double*)calloc((size_t)n + 1, sizeof(double));
+    double* z   = (double*)calloc((size_t)n + 1, sizeof(double));
+    double* p   = (double*)calloc((size_t)n + 1, sizeof(double));
 
-    or_residual(n, A->row_ptr, A->col_ind, A->val, x, b, res);
+    or_residual(n, rp, ci, v, x, b, res);
     double init = or_norm_inf(n, res);
     or_amg_precond(h, res, z, n1, n2, coarse_solver);
     memcpy(p, z, sizeof(double) * (size_t)n);
@@ -1223,7 +1237,7 @@ int or_pcg_saamg(const or_hierarchy* h, double* x, const double* b, double rel_t
     int iter = 0;
     while(!(iter > max_iter))
     {
-        or_csrmv(n, n, A->nnz, 1.0, A->row_ptr, A->col_ind, A->val, p, 0.0, z);
+        or_csrmv(n, n, rp[n], 1.0, rp, ci, v, p, 0.0, z);
         double alpha = gamma / or_dot(n, z, p);
         or_axpy(n, alpha, p, x);
         or_axpy(n, -1.0 * alpha, z, res);
@@ -1243,6 +1257,17 @@ int or_pcg_saamg(const or_hierarchy* h, double* x, const double* b, double rel_t
     free(z);
     free(p);
     return iter;
+}
+
+/* krylov/cg.cpp:136-223 (solve_precond), restart_iter = -1, on the matrix
+ * the hierarchy was built from. */
+int or_pcg_saamg(const or_hierarchy* h, double* x, const double* b, double rel_tol,
+                 double abs_tol, int max_iter, int n1, int n2, int coarse_solver,
+                 double* res_norm)
+{
+    const or_csr* A = &h->A[0];
+    return or_pcg_saamg_op(A->m, A->row_ptr, A->col_ind, A->val, h, x, b, rel_tol, abs_tol,
+                           max_iter, n1, n2, coarse_solver, res_norm);
 }
 
 /* krylov/cg.cpp:60-134 (solve_nonprecond).  restart_iter = -1 makes

@@ -84,6 +84,9 @@ void or_hierarchy_copy(const or_hierarchy* h, int level, int kind, int* rp, int*
 /* ---- solve ---- */
 void or_jacobi_iteration(int m, const int* rp, const int* ci, const double* v, double* x,
                          const double* xold, const double* b);
+/* Dense Cholesky factor, in place on a row-major N x N SPD matrix (lower
+ * triangle = L with L L^T = A). */
+void or_cholesky_factor(int N, double* L);
 /* coarse_solver: 0 = 1000 Gauss-Seidel sweeps (reference), 1 = dense Cholesky */
 void or_vcycle(const or_hierarchy* h, double* x, const double* b, int n1, int n2, int level,
                int coarse_solver);
@@ -118,6 +121,11 @@ void or_amg_precond(const or_hierarchy* h, const double* r, double* z, int n1, i
 int or_pcg_saamg(const or_hierarchy* h, double* x, const double* b, double rel_tol,
                  double abs_tol, int max_iter, int n1, int n2, int coarse_solver,
                  double* res_norm);
+/* The same loop with the Krylov products on (m, rp, ci, v) and the V-cycle
+ * of h (built for any matrix of that size). */
+int or_pcg_saamg_op(int m, const int* rp, const int* ci, const double* v, const or_hierarchy* h,
+                    double* x, const double* b, double rel_tol, double abs_tol, int max_iter,
+                    int n1, int n2, int coarse_solver, double* res_norm);
 int or_cg(int m, const int* rp, const int* ci, const double* v, double* x, const double* b,
           double rel_tol, double abs_tol, int max_iter, double* res_norm);
 

@@ -120,8 +120,12 @@ class Oracle:
         L.or_hierarchy_levels.argtypes = [C.c_void_p]
         L.or_hierarchy_dims.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _ip, _ip]
         L.or_hierarchy_copy.argtypes = [C.c_void_p, C.c_int, C.c_int, _i32p, _i32p, _f64p]
+        L.or_cholesky_factor.argtypes = [C.c_int, _f64p]
         L.or_jacobi_iteration.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p]
         L.or_amg_precond.argtypes = [C.c_void_p, _f64p, _f64p, C.c_int, C.c_int, C.c_int]
+        L.or_pcg_saamg_op.argtypes = [C.c_int, _i32p, _i32p, _f64p, C.c_void_p, _f64p, _f64p,
+                                      C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                      C.c_int, _dp]
         L.or_pcg_saamg.argtypes = [C.c_void_p, _f64p, _f64p, C.c_double, C.c_double, C.c_int,
                                    C.c_int, C.c_int, C.c_int, _dp]
         L.or_cg.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_double,
@@ -261,6 +265,12 @@ class Oracle:
         self.L.or_jacobi_iteration(A.m, A.row_ptr, A.col_ind, A.val, x, _f64(xold), _f64(b))
         return x
 
+    def cholesky_factor(self, dense):
+        """Lower-triangular L with L L^T = dense (or_cholesky_factor)."""
+        a = np.ascontiguousarray(dense, dtype=np.float64).copy()
+        self.L.or_cholesky_factor(a.shape[0], a)
+        return np.tril(a)
+
     def cg(self, A, b, x0=None, rel_tol=1e-6, abs_tol=1e-6, max_iter=1000):
         x = np.zeros(A.m) if x0 is None else _f64(x0).copy()
         rn = C.c_double(0.0)
@@ -319,6 +329,17 @@ class OracleHierarchy:
                                    coarse_solver, C.byref(rn))
         return x, it, rn.value
 
+    def pcg_op(self, A, b, x0=None, rel_tol=1e-8, abs_tol=1e-300, max_iter=1000, n1=1, n2=1,
+               coarse_solver=0):
+        """PCG on operator A (a CSR of the finest level's size) preconditioned
+        by one V-cycle of this hierarchy."""
+        x = np.zeros(len(b)) if x0 is None else _f64(x0).copy()
+        rn = C.c_double(0.0)
+        it = self.o.L.or_pcg_saamg_op(A.m, A.row_ptr, A.col_ind, A.val, self.h, x, _f64(b),
+                                      rel_tol, abs_tol, max_iter, n1, n2, coarse_solver,
+                                      C.byref(rn))
+        return x, it, rn.value
+
 
 class Reference:
     """LinAlgLib's own host library, compiled from its sources by oracle/Makefile."""
@@ -336,6 +357,9 @@ class Reference:
         L.ref_hierarchy_dims.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _ip, _ip]
         L.ref_hierarchy_copy.argtypes = [C.c_void_p, C.c_int, C.c_int, _i32p, _i32p, _f64p]
         L.ref_amg_precond.argtypes = [C.c_void_p, _f64p, _f64p, C.c_int, C.c_int]
+        L.ref_pcg_saamg_op.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, C.c_void_p,
+                                       _f64p, _f64p, C.c_double, C.c_double, C.c_int, C.c_int,
+                                       C.c_int]
         L.ref_pcg_saamg.argtypes = [C.c_void_p, _f64p, _f64p, C.c_double, C.c_double, C.c_int,
                                     C.c_int, C.c_int]
         L.ref_cg.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_double,
@@ -504,4 +528,10 @@ class RefHierarchy:
     def pcg(self, b, x0=None, rel_tol=1e-8, abs_tol=1e-300, max_iter=1000, n1=1, n2=1):
         x = np.zeros(len(b)) if x0 is None else _f64(x0).copy()
         it = self.r.L.ref_pcg_saamg(self.h, x, _f64(b), rel_tol, abs_tol, max_iter, n1, n2)
+        return x, it
+
+    def pcg_op(self, A, b, x0=None, rel_tol=1e-8, abs_tol=1e-300, max_iter=1000, n1=1, n2=1):
+        x = np.zeros(len(b)) if x0 is None else _f64(x0).copy()
+        it = self.r.L.ref_pcg_saamg_op(A.m, A.nnz, A.row_ptr, A.col_ind, A.val, self.h, x,
+                                       _f64(b), rel_tol, abs_tol, max_iter, n1, n2)
         return x, it

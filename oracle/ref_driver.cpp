@@ -204,6 +204,33 @@ int ref_pcg_saamg(void* h, double* x, const double* b, double rel_tol, double ab
     return it;
 }
 
+// cg_solver::solve(A, x, b, precond, control) with the Krylov operator A given
+// separately from the hierarchy behind the composed SA-AMG preconditioner.
+int ref_pcg_saamg_op(int m, int nnz, const int* rp, const int* ci, const double* v, void* h,
+                     double* x, const double* b, double rel_tol, double abs_tol, int max_iter,
+                     int n1, int n2)
+{
+    quiet q;
+    ref_hier*                   rh = static_cast<ref_hier*>(h);
+    linalg::csr_matrix<double>* A  = make_csr(m, m, nnz, rp, ci, v);
+    linalg::vector<double>      xv(std::vector<double>(x, x + m));
+    linalg::vector<double>      bv(std::vector<double>(b, b + m));
+    amg_precond                 p;
+    p.h  = &rh->h;
+    p.n1 = n1;
+    p.n2 = n2;
+    linalg::cg_solver cg;
+    cg.build(*A);
+    linalg::iter_control c;
+    c.rel_tol  = rel_tol;
+    c.abs_tol  = abs_tol;
+    c.max_iter = max_iter;
+    int it     = cg.solve(*A, xv, bv, &p, c);
+    std::memcpy(x, xv.get_vec(), sizeof(double) * m);
+    delete A;
+    return it;
+}
+
 // amg_solve with the given cycle (0 V, 1 W, 2 F) and smoother (0 Jacobi,
 // 1 Gauss-Seidel, 2 symmetric Gauss-Seidel, 3 SOR, 4 SSOR); x in place.
 int ref_amg_solve(void* h, double* x, const double* b, int n1, int n2, int cycle, int smoother,

@@ -206,9 +206,52 @@ def krylov_case(ref):
     print("krylov written")
 
 
+def op_precond_matrix():
+    """8^3 diffusion with log-normal cell conductivities (sigma = 1), the
+    matrix a preconditioner is built for while the Krylov operator is 3D 8^3
+    Poisson."""
+    k = np.exp(1.0 * pb.normal(8 ** 3, seed=5))
+    return pb.diffusion3d(8, cell_kappa=k)
+
+
+def cg_case(ref):
+    """cg_solver::solve without a preconditioner (solve_nonprecond, whose
+    restart block runs on every iteration) on 3D 6^3 and 2D 12^2 Poisson at
+    two tolerances: iteration counts and solutions."""
+    out = {}
+    for name, gen in (("3d6", lambda: pb.poisson3d(6)), ("2d12", lambda: pb.poisson2d(12))):
+        rp, ci, v, m = gen()
+        A = CSR(rp, ci, v, m, m)
+        b = pb.rhs_from_solution(rp, ci, v, pb.uniform(m, seed=3))
+        out[f"{name}_b"] = b
+        for tag, rt, at in (("6", 1e-6, 1e-6), ("10", 1e-10, 1e-300)):
+            x, it = ref.cg(A, b, rel_tol=rt, abs_tol=at, max_iter=2000)
+            out[f"{name}_t{tag}_x"] = x
+            out[f"{name}_t{tag}_it"] = np.array([it])
+    # the C++ API program's case: b = 1 on 3D 6^3 Poisson, rel_tol 1e-6
+    rp, ci, v, m = pb.poisson3d(6)
+    x, it = ref.cg(CSR(rp, ci, v, m, m), np.ones(m), rel_tol=1e-6, abs_tol=1e-300,
+                   max_iter=5000)
+    out.update(ones6_x=x, ones6_it=np.array([it]))
+    # cg_solver::solve(A, x, b, precond) with the SA-AMG V-cycle of a hierarchy
+    # built for another operator of the same size (log-normal diffusion 8^3)
+    # and the Krylov products on 3D 8^3 Poisson
+    rp, ci, v, m = pb.poisson3d(8)
+    drp, dci, dv, _ = op_precond_matrix()
+    h = ref.saamg_setup(CSR(drp, dci, dv, m, m))
+    b = pb.rhs_from_solution(rp, ci, v, pb.uniform(m, seed=2))
+    x, it = h.pcg_op(CSR(rp, ci, v, m, m), b, rel_tol=1e-10)
+    out.update(op_b=b, op_x=x, op_it=np.array([it]))
+    np.savez_compressed(os.path.join(OUT, "golden_cg.npz"), **out)
+    print("cg written")
+
+
 def main():
     ref = Reference()
     ref.set_threads(1)
+    if len(sys.argv) > 1 and sys.argv[1] == "cg":
+        cg_case(ref)
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "krylov":
         krylov_case(ref)
         return
@@ -227,6 +270,7 @@ def main():
     cycles_case(ref)
     ua_case(ref)
     krylov_case(ref)
+    cg_case(ref)
 
 
 if __name__ == "__main__":

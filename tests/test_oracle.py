@@ -259,6 +259,66 @@ def test_pcg_search_directions_a_orthogonal(oracle):
     assert abs(a - b) <= 1e-12 * abs(a)
 
 
+def _dense(M):
+    D = np.zeros((M.m, M.n))
+    for i in range(M.m):
+        for k in range(M.row_ptr[i], M.row_ptr[i + 1]):
+            D[i, M.col_ind[k]] += M.val[k]
+    return D
+
+
+@pytest.mark.parametrize("name,max_level", [("3d8", 2), ("3d8", 3), ("2d128", 3),
+                                            ("diff12", 4), ("2d8", 0), ("3d8", 1)])
+def test_cholesky_factor_reconstructs_coarsest_operator(oracle, name, max_level):
+    """or_cholesky_factor on coarsest operators with N > 1: L is lower
+    triangular with a positive diagonal, L L^T = A_c to 1e-14 relative, and L
+    equals numpy's Cholesky factor to 1e-13.  A transposed index or a dropped
+    term in the factor breaks the reconstruction."""
+    rp, ci, v, m = grid(name)
+    h = oracle.saamg_setup(CSR(rp, ci, v, m, m), max_level=max_level)
+    Ac = _dense(h.get(h.levels, "A"))
+    assert Ac.shape[0] > 1
+    L = oracle.cholesky_factor(Ac)
+    assert np.all(np.diag(L) > 0) and np.array_equal(L, np.tril(L))
+    assert np.abs(L @ L.T - Ac).max() <= 1e-14 * np.abs(Ac).max() * Ac.shape[0]
+    assert np.abs(L - np.linalg.cholesky(Ac)).max() <= 1e-13 * np.abs(L).max()
+
+
+@pytest.mark.parametrize("name,max_level", [("3d8", 2), ("3d8", 3), ("2d128", 3),
+                                            ("diff12", 4), ("2d8", 0), ("3d8", 0)])
+def test_vcycle_cholesky_coarse_branch(oracle, name, max_level):
+    """The dense-Cholesky coarsest solve inside or_vcycle: with no smoothing
+    sweeps (n1 = n2 = 0) one V-cycle from zero is z = P_0 ... P_{L-1}
+    A_L^-1 R_{L-1} ... R_0 r, which numpy's dense solve reproduces to 1e-13;
+    with max_level = 0 this is z = A^-1 r on the whole operator."""
+    rp, ci, v, m = grid(name)
+    h = oracle.saamg_setup(CSR(rp, ci, v, m, m), max_level=max_level)
+    r = pb.uniform(m, seed=11)
+    z = h.precond(r, 0, 0, coarse_solver=1)
+    w = r.copy()
+    for lvl in range(h.levels):
+        w = _dense(h.get(lvl, "R")) @ w
+    w = np.linalg.solve(_dense(h.get(h.levels, "A")), w)
+    for lvl in reversed(range(h.levels)):
+        w = _dense(h.get(lvl, "P")) @ w
+    assert np.abs(z - w).max() <= 1e-13 * np.abs(w).max()
+
+
+@pytest.mark.parametrize("name", ["3d6", "2d12"])
+def test_cg_matches_reference_golden(oracle, golden, name):
+    """or_cg (cg_solver::solve_nonprecond, cg.cpp:60-134, whose restart block
+    runs every iteration since restart_iter = -1) bit for bit against the
+    compiled reference's iteration counts and solutions."""
+    g = golden("cg")
+    rp, ci, v, m = pb.poisson3d(6) if name == "3d6" else pb.poisson2d(12)
+    A = CSR(rp, ci, v, m, m)
+    b = g[f"{name}_b"]
+    for tag, rt, at in (("6", 1e-6, 1e-6), ("10", 1e-10, 1e-300)):
+        x, it, _ = oracle.cg(A, b, rel_tol=rt, abs_tol=at, max_iter=2000)
+        assert it == int(g[f"{name}_t{tag}_it"][0])
+        assert bits_equal(x, g[f"{name}_t{tag}_x"])
+
+
 # ---------------------------------------------------------------------------
 # the compiled reference (in a container that holds its sources)
 
@@ -282,6 +342,31 @@ def test_oracle_equals_compiled_reference(oracle, reference, gen):
     xo, ito, _ = ho.pcg(b, rel_tol=1e-9)
     xr, itr = hr.pcg(b, rel_tol=1e-9)
     assert ito == itr and bits_equal(xo, xr)
+
+
+def test_pcg_operator_apart_from_hierarchy_matches_reference_golden(oracle, golden):
+    """cg_solver::solve(A, x, b, precond) with A different from the matrix
+    the SA-AMG hierarchy was built for: iteration count and solution bit for
+    bit against the reference's."""
+    from tests.golden.make_golden import op_precond_matrix
+
+    g = golden("cg")
+    rp, ci, v, m = pb.poisson3d(8)
+    drp, dci, dv, _ = op_precond_matrix()
+    h = oracle.saamg_setup(CSR(drp, dci, dv, m, m))
+    x, it, _ = h.pcg_op(CSR(rp, ci, v, m, m), g["op_b"], rel_tol=1e-10)
+    assert it == int(g["op_it"][0])
+    assert bits_equal(x, g["op_x"])
+
+
+def test_cg_equals_compiled_reference(oracle, reference):
+    rp, ci, v, m = pb.poisson3d(9, 7, 5)
+    A = CSR(rp, ci, v, m, m)
+    b = pb.rhs_from_solution(rp, ci, v, pb.uniform(m, seed=4))
+    for rt, at in ((1e-6, 1e-6), (1e-10, 1e-300)):
+        xo, ito, _ = oracle.cg(A, b, rel_tol=rt, abs_tol=at, max_iter=2000)
+        xr, itr = reference.cg(A, b, rel_tol=rt, abs_tol=at, max_iter=2000)
+        assert ito == itr and bits_equal(xo, xr)
 
 
 # ---------------------------------------------------------------------------
