@@ -90,6 +90,12 @@ int lal_axpby_f32(lal_handle h, int n, float alpha, const float* x, float beta, 
  * loop at one OpenMP thread); longer vectors use a fixed-order tree. */
 int lal_dot_f64(lal_handle h, int n, const double* x, const double* y, double* result);
 int lal_dot_f32(lal_handle h, int n, const float* x, const float* y, double* result);
+/* Fused y = alpha x + y (lal_axpy_f64's arithmetic) and *result = <y, z>
+ * over the updated y (z may be y), in one pass: the CG residual update and
+ * its inner product (cg.cpp:177-203).  Bit-identical to lal_axpy_f64 followed
+ * by lal_dot_f64. */
+int lal_axpy_dot_f64(lal_handle h, int n, double alpha, const double* x, double* y,
+                     const double* z, double* result);
 int lal_norm_inf_f64(lal_handle h, int n, const double* x, double* result);
 int lal_norm_inf_f32(lal_handle h, int n, const float* x, double* result);
 /* In-place exclusive prefix sum. */
@@ -126,6 +132,10 @@ int lal_csrmv_f32(lal_handle h, lal_csrmv_descr descr, int m, int n, int nnz, fl
 int lal_residual_f64(lal_handle h, lal_csrmv_descr descr, int m, int n, int nnz,
                      const int* row_ptr, const int* col_ind, const double* val,
                      const double* x, const double* b, double* r);
+/* The same in FP32 (values, x, b, r), for the FP32 copies of an operator. */
+int lal_residual_f32(lal_handle h, lal_csrmv_descr descr, int m, int n, int nnz,
+                     const int* row_ptr, const int* col_ind, const float* val, const float* x,
+                     const float* b, float* r);
 /* d_i = first a_ii of row i; untouched when absent (host_extract.cpp:34-52). */
 int lal_diagonal_f64(lal_handle h, int m, const int* row_ptr, const int* col_ind,
                      const double* val, double* d);
@@ -140,6 +150,10 @@ int lal_diagonal_f64(lal_handle h, int m, const int* row_ptr, const int* col_ind
 int lal_transpose_f64(lal_handle h, int m, int n, int nnz, const int* row_ptr,
                       const int* col_ind, const double* val, int* t_row_ptr, int* t_col_ind,
                       double* t_val);
+/* The same with FP32 values (the pattern is that of lal_transpose_f64). */
+int lal_transpose_f32(lal_handle h, int m, int n, int nnz, const int* row_ptr,
+                      const int* col_ind, const float* val, int* t_row_ptr, int* t_col_ind,
+                      float* t_val);
 /* C = A B, two passes.  Pass 1 fills row_ptr_C (m+1, device) and returns nnz
  * to *nnz_C (host).  Pass 2 fills col_ind_C / val_C with columns ascending
  * and every entry folded in A-row-then-B-row order from its first product,
@@ -169,6 +183,36 @@ int lal_sa_strength_f64(lal_handle h, int m, int nnz, const int* row_ptr, const 
  * written completely (zeroing is done here). */
 int lal_pmis_aggregate(lal_handle h, int m, int nnz, const int* row_ptr, const int* col_ind,
                        const int* connections, int64_t* aggregates);
+/* The three PMIS steps one to one, for a driver that runs
+ * amg_aggregation.cpp:106-215 itself:
+ *   lal_pmis_init  -- device_initialize_pmis_state (device_amg_aggregation.h:40-43;
+ *                     host_amg_aggregation.cpp:32-68): state_i = 0 when row i
+ *                     has a strong connection, else -2; hash_i = hash1(i).
+ *   lal_pmis_round -- device_find_maximum_distance_two_node
+ *                     (device_amg_aggregation.h:45-51; host_amg_aggregation.cpp:
+ *                     77-180): one round on the snapshot `state`; an
+ *                     undecided row that is the lexicographic (state, hash)
+ *                     maximum of its distance-two strong neighbourhood becomes
+ *                     a root (aggregates 1, max_state 1), one whose maximum is
+ *                     a root becomes -1 (aggregates 0); *complete (host
+ *                     memory, the call blocks until it is known) is 1 when no
+ *                     row was left undecided, as the reference's bool.
+ *   lal_pmis_join  -- device_add_unassigned_nodes_to_closest_aggregation
+ *                     (device_amg_aggregation.h:53-58; host_amg_aggregation.cpp:
+ *                     182-221): a state -1 row takes the aggregate of its first
+ *                     strong state-1 neighbour in CSR order and max_state 1; a
+ *                     state -2 row gets aggregate -2.  (The reference's
+ *                     aggregate_root_nodes argument is unused there.)
+ * The driver copies max_state into state before each round and each join,
+ * and scans the aggregate flags in between (lal_exclusive_scan_i64). */
+int lal_pmis_init(lal_handle h, int m, int nnz, const int* row_ptr, const int* connections,
+                  int* state, int* hash);
+int lal_pmis_round(lal_handle h, int m, int nnz, const int* row_ptr, const int* col_ind,
+                   const int* connections, const int* state, const int* hash,
+                   int64_t* aggregates, int* max_state, int* complete);
+int lal_pmis_join(lal_handle h, int m, int nnz, const int* row_ptr, const int* col_ind,
+                  const int* connections, const int* state, int64_t* aggregates,
+                  int* max_state);
 /* P of iterative_solvers/amg/saamg.cpp:51-207 in two passes: the first
  * fills p_row_ptr (m+1) and returns nnz_P and the column count n_c; the
  * second fills p_col_ind / p_val. */
@@ -216,6 +260,8 @@ int  lal_uaamg_setup(lal_handle h, int m, int n, int nnz, const int* row_ptr, co
 int lal_saamg_setup(lal_handle h, int m, int n, int nnz, const int* row_ptr,
                     const int* col_ind, const double* val, const lal_saamg_options* opts,
                     lal_hierarchy* out);
+/* May also be called after lal_destroy of the context it was built on (the
+ * hierarchy is then only freed); no other call takes such a hierarchy. */
 int lal_hierarchy_destroy(lal_hierarchy hier);
 int lal_hierarchy_levels(lal_hierarchy hier, int* total_levels);
 /* kind: 0 = A, 1 = P, 2 = R.  Device pointers stay owned by the hierarchy. */
@@ -238,6 +284,11 @@ typedef struct
     double inner_rel_tol;
     int    inner_max_iter;
     int    max_outer;
+    /* smoother of the FP64 PCG V-cycle (lal_pcg_solve_f64 / _op_f64 and the
+       FP64 adjoint): 0 undamped Jacobi (the reference composition; default),
+       or any lal_amg_solve smoother 1-6, e.g. 6 for symmetric multicolour
+       Gauss-Seidel.  The mixed-precision schemes take 0 only. */
+    int    smoother;
 } lal_solve_options;
 
 typedef struct
@@ -297,7 +348,12 @@ int lal_pcg_solve_reliable(lal_handle h, lal_hierarchy hier, const double* b, do
  * (iterative_solvers/amg/amg.cpp:516-571; enums of include/iterative_solvers/
  * amg/amg.h:155-240).  cycle: 0 V, 1 W, 2 F; smoother: 0 Jacobi,
  * 1 Gauss-Seidel, 2 symmetric Gauss-Seidel (a forward sweep, as the reference
- * runs it), 3 SOR, 4 SSOR (omega = 0.5).  x is the initial guess on entry;
+ * runs it), 3 SOR, 4 SSOR (omega = 0.5); and, with no reference counterpart,
+ * 5 multicolour Gauss-Seidel (rows greedily coloured so that no two coupled
+ * rows share a colour, colours swept in order, each colour one parallel pass)
+ * and 6 symmetric multicolour Gauss-Seidel (forward then backward colour
+ * order; a V-cycle with it is a symmetric preconditioner).  The colouring is
+ * built on first use per level.  x is the initial guess on entry;
  * cycles run until the inf-norm residual meets rel_tol / abs_tol or more
  * than opts->max_iter cycles ran (iter_control::max_cycle);
  * info->iterations is the reference's return value.  Gauss-Seidel-type sweeps

@@ -575,6 +575,50 @@ extern "C" int lal_pmis_aggregate(lal_handle c, int m, int nnz, const int* rp, c
     return lal::pmis_aggregate(c, m, nnz, rp, ci, conn, agg);
 }
 
+extern "C" int lal_pmis_init(lal_handle c, int m, int nnz, const int* rp, const int* conn,
+                             int* state, int* hash)
+{
+    LAL_REQUIRE(c != nullptr && m >= 0 && nnz >= 0);
+    LAL_REQUIRE(m == 0 || (rp && state && hash && (nnz == 0 || conn)));
+    if(m == 0)
+        return LAL_OK;
+    pmis_init_kernel<<<lal::grid_for(m, 256), 256, 0, c->stream>>>(m, rp, conn, state, hash);
+    LAL_LAUNCHED();
+    return LAL_OK;
+}
+
+extern "C" int lal_pmis_round(lal_handle c, int m, int nnz, const int* rp, const int* ci,
+                              const int* conn, const int* state, const int* hash, int64_t* agg,
+                              int* max_state, int* complete)
+{
+    LAL_REQUIRE(c != nullptr && m >= 0 && nnz >= 0 && complete != nullptr);
+    LAL_REQUIRE(m == 0 || (rp && state && hash && agg && max_state && (nnz == 0 || (ci && conn))));
+    *complete = 1;
+    if(m == 0)
+        return LAL_OK;
+    LAL_HIP(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
+    pmis_round_kernel<<<lal::grid_for(m, 256), 256, 0, c->stream>>>(m, rp, ci, conn, state, hash,
+                                                                    agg, max_state, c->d_flag);
+    LAL_LAUNCHED();
+    LAL_HIP(hipMemcpyAsync(c->h_flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    LAL_HIP(hipStreamSynchronize(c->stream));
+    *complete = c->h_flag[0] == 0;
+    return LAL_OK;
+}
+
+extern "C" int lal_pmis_join(lal_handle c, int m, int nnz, const int* rp, const int* ci,
+                             const int* conn, const int* state, int64_t* agg, int* max_state)
+{
+    LAL_REQUIRE(c != nullptr && m >= 0 && nnz >= 0);
+    LAL_REQUIRE(m == 0 || (rp && state && agg && max_state && (nnz == 0 || (ci && conn))));
+    if(m == 0)
+        return LAL_OK;
+    pmis_join_kernel<<<lal::grid_for(m, 256), 256, 0, c->stream>>>(m, rp, ci, conn, state, agg,
+                                                                   max_state);
+    LAL_LAUNCHED();
+    return LAL_OK;
+}
+
 extern "C" int lal_sa_prolongator_nnz(lal_handle c, int m, int nnz, const int* rp,
                                       const int* ci, const int* conn, const int64_t* agg,
                                       int* prp, int* nnz_P, int* n_c)

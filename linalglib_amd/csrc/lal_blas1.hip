@@ -53,12 +53,15 @@ namespace
     }
 
     // ----------------------------------------------------------- reductions
-    // OP 0: sum x_i*y_i ; OP 1: max |x_i|.  Accumulation is in double.
+    // OP 0: sum x_i*y_i ; OP 1: max |x_i| ; OP 2: y_i = alpha x_i + y_i (the
+    // axpy kernel's arithmetic) then sum y_i*z_i, which is lal_axpy then
+    // lal_dot in one pass with the same partition and order.  Accumulation is
+    // in double.
     template <typename T, int OP>
     __global__ void __launch_bounds__(LAL_RED_THREADS)
-        reduce_kernel(int n, const T* __restrict__ x, const T* __restrict__ y,
-                      double* __restrict__ part, unsigned* __restrict__ ticket,
-                      double* __restrict__ out)
+        reduce_kernel(int n, const T* __restrict__ x, const T* y, double* __restrict__ part,
+                      unsigned* __restrict__ ticket, double* __restrict__ out, T alpha = 0,
+                      T* yw = nullptr, const T* z = nullptr)
     {
         __shared__ double sh[LAL_RED_THREADS / 64];
         __shared__ bool   last;
@@ -75,22 +78,50 @@ namespace
             }
         };
         int i = blockIdx.x * LAL_RED_THREADS + tid;
-        for(; i + 3 * stride < n; i += 4 * stride)
+        if(OP == 2)
         {
-            T a[4], b[4];
-#pragma unroll
-            for(int k = 0; k < 4; k++)
+            // y = alpha x + y, then the products y_i * z_i; z may be y
+            auto upd = [&](int k) {
+                const T yn = alpha == (T)1 ? x[k] + yw[k] : alpha * x[k] + yw[k];
+                yw[k]      = yn;
+                return yn;
+            };
+            for(; i + 3 * stride < n; i += 4 * stride)
             {
-                a[k] = x[i + k * stride];
-                b[k] = OP == 0 ? y[i + k * stride] : a[k];
-            }
+                T a[4];
 #pragma unroll
-            for(int k = 0; k < 4; k++)
-                add(a[k], b[k]);
+                for(int k = 0; k < 4; k++)
+                    a[k] = upd(i + k * stride);
+#pragma unroll
+                for(int k = 0; k < 4; k++)
+                    acc = acc + (double)a[k] * (double)(z == yw ? a[k] : z[i + k * stride]);
+            }
+            for(; i < n; i += stride)
+            {
+                const T a = upd(i);
+                acc       = acc + (double)a * (double)(z == yw ? a : z[i]);
+            }
         }
-        for(; i < n; i += stride)
-            add(x[i], OP == 0 ? y[i] : x[i]);
-        acc = OP == 0 ? lal::wave_sum(acc) : lal::wave_max(acc);
+        else
+        {
+            for(; i + 3 * stride < n; i += 4 * stride)
+            {
+                T a[4], b[4];
+#pragma unroll
+                for(int k = 0; k < 4; k++)
+                {
+                    a[k] = x[i + k * stride];
+                    b[k] = OP == 0 ? y[i + k * stride] : a[k];
+                }
+#pragma unroll
+                for(int k = 0; k < 4; k++)
+                    add(a[k], b[k]);
+            }
+            for(; i < n; i += stride)
+                add(x[i], OP == 0 ? y[i] : x[i]);
+        }
+        constexpr bool SUM = OP != 1;
+        acc = SUM ? lal::wave_sum(acc) : lal::wave_max(acc);
         if((tid & 63) == 0)
             sh[tid >> 6] = acc;
         __syncthreads();
@@ -98,7 +129,7 @@ namespace
         {
             double t = sh[0];
             for(int w = 1; w < LAL_RED_THREADS / 64; w++)
-                t = OP == 0 ? t + sh[w] : ((sh[w] < t) ? t : sh[w]);
+                t = SUM ? t + sh[w] : ((sh[w] < t) ? t : sh[w]);
             part[blockIdx.x] = t;
             __threadfence();
             unsigned k = atomicAdd(ticket, 1u);
@@ -112,9 +143,9 @@ namespace
         for(int k = tid; k < (int)gridDim.x; k += LAL_RED_THREADS)
         {
             double p = __hip_atomic_load(&part[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            v        = OP == 0 ? v + p : ((p < v) ? v : p);
+            v        = SUM ? v + p : ((p < v) ? v : p);
         }
-        v = OP == 0 ? lal::wave_sum(v) : lal::wave_max(v);
+        v = SUM ? lal::wave_sum(v) : lal::wave_max(v);
         if((tid & 63) == 0)
             sh[tid >> 6] = v;
         __syncthreads();
@@ -122,7 +153,7 @@ namespace
         {
             double t = sh[0];
             for(int w = 1; w < LAL_RED_THREADS / 64; w++)
-                t = OP == 0 ? t + sh[w] : ((sh[w] < t) ? t : sh[w]);
+                t = SUM ? t + sh[w] : ((sh[w] < t) ? t : sh[w]);
             *out    = t;
             *ticket = 0u;
         }
@@ -264,6 +295,21 @@ namespace
         }
     }
 
+    // The three passes of the scan with the tile sums in `sums` (ntiles
+    // entries, every one written by the first pass before it is read).
+    template <typename T>
+    int scan_passes(lal_context* c, int n, T* x, T* sums)
+    {
+        const int ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+        scan_tile_sums<T><<<ntiles, SCAN_THREADS, 0, c->stream>>>(n, x, sums);
+        LAL_LAUNCHED();
+        scan_sums<T><<<1, SCAN_THREADS, 0, c->stream>>>(ntiles, sums);
+        LAL_LAUNCHED();
+        scan_apply<T><<<ntiles, SCAN_THREADS, 0, c->stream>>>(n, x, sums);
+        LAL_LAUNCHED();
+        return LAL_OK;
+    }
+
     template <typename T>
     int scan_dev(lal_context* c, int n, T* x)
     {
@@ -272,14 +318,9 @@ namespace
         int ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
         T*  sums   = nullptr;
         LAL_HIP(hipMallocAsync((void**)&sums, sizeof(T) * ntiles, c->stream));
-        scan_tile_sums<T><<<ntiles, SCAN_THREADS, 0, c->stream>>>(n, x, sums);
-        LAL_LAUNCHED();
-        scan_sums<T><<<1, SCAN_THREADS, 0, c->stream>>>(ntiles, sums);
-        LAL_LAUNCHED();
-        scan_apply<T><<<ntiles, SCAN_THREADS, 0, c->stream>>>(n, x, sums);
-        LAL_LAUNCHED();
+        const int st = scan_passes<T>(c, n, x, sums);
         LAL_HIP(hipFreeAsync(sums, c->stream));
-        return LAL_OK;
+        return st;
     }
 
     __global__ void __launch_bounds__(256)
@@ -413,6 +454,28 @@ extern "C" int lal_dot_f64(lal_handle c, int n, const double* x, const double* y
 {
     return reduce_host<double, 0>(c, n, x, y, r);
 }
+
+extern "C" int lal_axpy_dot_f64(lal_handle c, int n, double alpha, const double* x, double* y,
+                                const double* z, double* result)
+{
+    LAL_REQUIRE(c != nullptr && n >= 0 && result != nullptr);
+    LAL_REQUIRE(n == 0 || (x != nullptr && y != nullptr && z != nullptr));
+    if(n > 0 && n <= SEQ_DOT)
+    {
+        // short vectors: the axpy, then the dot of lal_dot_f64's short path
+        LAL_TRY(axpy_t<double>(c, n, alpha, x, y));
+        return reduce_host<double, 0>(c, n, y, z, result);
+    }
+    const int g = n <= 0 ? 1 : lal::grid_for(n, LAL_RED_THREADS, LAL_RED_BLOCKS);
+    reduce_kernel<double, 2><<<g, LAL_RED_THREADS, 0, c->stream>>>(
+        n, x, y, c->d_part, c->d_ticket, c->d_slots, alpha, y, z);
+    LAL_LAUNCHED();
+    LAL_HIP(hipMemcpyAsync(c->h_slots, c->d_slots, sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+    LAL_HIP(hipStreamSynchronize(c->stream));
+    *result = c->h_slots[0];
+    return LAL_OK;
+}
 extern "C" int lal_dot_f32(lal_handle c, int n, const float* x, const float* y, double* r)
 {
     return reduce_host<float, 0>(c, n, x, y, r);
@@ -435,3 +498,4 @@ extern "C" int lal_exclusive_scan_i64(lal_handle c, int n, int64_t* x)
     LAL_REQUIRE(c != nullptr && n >= 0 && (n == 0 || x != nullptr));
     return lal::scan_i64_dev(c, n, x);
 }
+

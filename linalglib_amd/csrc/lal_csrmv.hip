@@ -328,6 +328,13 @@ extern "C" int lal_residual_f64(lal_handle c, lal_csrmv_descr d, int m, int n, i
     return lal::residual_f64(c, d, m, n, nnz, rp, ci, v, x, b, r);
 }
 
+extern "C" int lal_residual_f32(lal_handle c, lal_csrmv_descr d, int m, int n, int nnz,
+                                const int* rp, const int* ci, const float* v, const float* x,
+                                const float* b, float* r)
+{
+    return lal::residual_f32(c, d, m, n, nnz, rp, ci, v, x, b, r);
+}
+
 extern "C" int lal_diagonal_f64(lal_handle c, int m, const int* rp, const int* ci,
                                 const double* v, double* d)
 {

@@ -458,6 +458,12 @@ namespace
             return lal::jacobi_sweep<double>(c, A.d, A.m, A.rp, A.ci, A.v, W.diag, W.dinv, W.t, b,
                                              x);
         }
+        if(smoother >= 5)
+        {
+            if(!L.mc.built)
+                LAL_TRY(lal::build_mc(c, A, L.mc));
+            return lal::mc_sweep(c, L.mc, A.m, b, x, smoother == 6);
+        }
         if(!L.gs_fwd.perm)
             LAL_TRY(build_sched(c, A, true, L.gs_fwd));
         if(smoother == 4 && !L.gs_bwd.perm)
@@ -521,13 +527,23 @@ namespace
     }
 }
 
+namespace lal
+{
+    int vcycle_smoothed(lal_hierarchy_t* h, const double* r, double* z, int n1, int n2,
+                        int smoother)
+    {
+        LAL_HIP(hipMemsetAsync(z, 0, sizeof(double) * h->L[0].A.m, h->c->stream));
+        return cycle_on(h, 0, z, r, n1, n2, n2, 0, smoother);
+    }
+}
+
 extern "C" int lal_amg_solve(lal_handle c, lal_hierarchy h, const double* b, double* x, int n1,
                              int n2, int cycle, int smoother, const lal_solve_options* opts,
                              lal_solve_info* info)
 {
     LAL_REQUIRE(c != nullptr && h != nullptr && b != nullptr && x != nullptr && c == h->c);
     LAL_REQUIRE(n1 >= 0 && n2 >= 0 && cycle >= 0 && cycle <= 2 && smoother >= 0 &&
-                smoother <= 4);
+                smoother <= 6);
     lal_solve_options o;
     lal_solve_default_options(&o);
     if(opts)

@@ -8,6 +8,8 @@
 #include <algorithm>
 #include <vector>
 
+struct lal_hierarchy_t;
+
 namespace lal
 {
     struct csr_dev
@@ -64,13 +66,43 @@ namespace lal
         int              ntiles = 0;
     };
 
+    // Multicolour Gauss-Seidel schedule of one operator (lal_mcgs.hip): rows
+    // grouped by colour (colour k: perm[off[k] .. off[k+1])) and a copy of the
+    // matrix in that row order without its diagonal, plus 1/a_ii per row.
+    struct mc_sched
+    {
+        bool             built = false;
+        int              ncol  = 0;
+        int              lanes = 1;     // lanes per row
+        bool             wg    = false; // every colour in one workgroup
+        std::vector<int> off;
+        int*             off_d = nullptr;
+        int*             perm  = nullptr;
+        int*             prp   = nullptr;
+        int*             pci   = nullptr;
+        double*          pv    = nullptr;
+        double*          pdinv = nullptr;
+    };
+
     struct level_t
     {
         csr_dev      A, P, R;
         work<double> w64;
         work<float>  w32;
         gs_sched     gs_fwd, gs_bwd;
+        mc_sched     mc;
     };
+
+    // z = one V-cycle from zero on r with smoother 0-6 of the AMG cycles
+    // (lal_cycles.hip: n1 pre and n2 post sweeps per level, the reference's
+    // coarsest solve).
+    int vcycle_smoothed(lal_hierarchy_t* h, const double* r, double* z, int n1, int n2,
+                        int smoother);
+    // Builds the multicolour schedule of A (host colouring, device copy).
+    int build_mc(lal_context* c, const csr_dev& A, mc_sched& s);
+    // One multicolour Gauss-Seidel sweep on x in place (forward colours, then
+    // backward when sym).
+    int mc_sweep(lal_context* c, const mc_sched& s, int m, const double* b, double* x, bool sym);
 
     inline int alloc_bytes(void** p, size_t bytes)
     {
@@ -142,6 +174,9 @@ struct lal_hierarchy_t
                 for(void* ptr : {(void*)g->perm, (void*)g->off_d, (void*)g->prp, (void*)g->pci,
                                  (void*)g->pv, (void*)g->pdiag, (void*)g->tiles})
                     (void)hipFree(ptr);
+            for(void* ptr : {(void*)L[l].mc.off_d, (void*)L[l].mc.perm, (void*)L[l].mc.prp,
+                             (void*)L[l].mc.pci, (void*)L[l].mc.pv, (void*)L[l].mc.pdinv})
+                (void)hipFree(ptr);
         }
         for(void* ptr : {(void*)coarse_inv, (void*)res, (void*)z, (void*)p, (void*)q,
                          (void*)res32, (void*)z32, (void*)p32, (void*)q32, (void*)d32, (void*)s,

@@ -612,7 +612,7 @@ namespace
     template <typename T>
     int pcg(lal_hierarchy_t* h, const T* b, T* x, T* res, T* z, T* p, T* q, double rel_tol,
             double abs_tol, int max_iter, int n1, int n2, lal_solve_info* info,
-            const csr_dev* op = nullptr)
+            const csr_dev* op = nullptr, int smoother = 0)
     {
         lal_context*   c  = h->c;
         const csr_dev& A  = op ? *op : h->L[0].A;
@@ -634,7 +634,14 @@ namespace
         LAL_TRY(lal::norm_partials<T>(c, n, res, part_r));
         LAL_TRY(post_scalar(h, 1, part_r, lal::S_RNORM0, &seq));
         LAL_TRY(wait_scalar(h, seq, lal::S_RNORM0, &init));
-        LAL_TRY(precond<T>(h, res, z, n1, n2));
+        // the preconditioner: one V-cycle from zero, with the Jacobi fast path
+        // or (smoother != 0, FP64 only) another smoother of the AMG cycles
+        auto apply = [&](bool xinit) -> int {
+            if(smoother != 0)
+                return lal::vcycle_smoothed(h, (const double*)res, (double*)z, n1, n2, smoother);
+            return precond<T>(h, res, z, n1, n2, xinit);
+        };
+        LAL_TRY(apply(false));
         LAL_HIP(hipMemcpyAsync(p, z, sizeof(T) * n, hipMemcpyDeviceToDevice, c->stream));
         int cur = 0;
         LAL_TRY(lal::dot_partials<T, T>(c, n, z, res, part_g[cur]));
@@ -648,7 +655,7 @@ namespace
             LAL_TRY(lal::dot_partials<T, T>(c, n, q, p, part_pq));
             // with one pre-sweep on the fast path, the update also writes the
             // V-cycle's first Jacobi iterate x0 = r / a_ii
-            const bool fast = !h->opts.exact && n1 == 1 && h->levels > 0;
+            const bool fast = !h->opts.exact && n1 == 1 && h->levels > 0 && smoother == 0;
             work<T>&   W0   = h->w<T>(0);
             LAL_TRY(lal::update_xr<T, T>(c, n, p, q, x, res, part_pq, part_g[cur], part_r,
                                          s + lal::S_ALPHA, W0.dinv, fast ? W0.x : nullptr,
@@ -661,7 +668,7 @@ namespace
             // decides first so that no V-cycle is spent after the last step.
             const bool near = lal::near_tolerance(r_prev, rate, init, rel_tol, abs_tol);
             if(!near)
-                LAL_TRY(precond<T>(h, res, z, n1, n2, fast));
+                LAL_TRY(apply(fast));
             LAL_TRY(wait_scalar(h, seq, lal::S_RNORM, &rn));
             lal::observe_rate(rn, &r_prev, &rate);
             if(converges(rn, init, rel_tol, abs_tol))
@@ -670,7 +677,7 @@ namespace
                 break;
             }
             if(near)
-                LAL_TRY(precond<T>(h, res, z, n1, n2, fast));
+                LAL_TRY(apply(fast));
             LAL_TRY(lal::dot_partials<T, T>(c, n, z, res, part_g[cur ^ 1]));
             LAL_TRY(lal::update_p<T, T>(c, n, z, p, part_g[cur ^ 1], part_g[cur]));
             cur ^= 1;
@@ -988,6 +995,7 @@ extern "C" void lal_solve_default_options(lal_solve_options* o)
     o->inner_rel_tol  = 1e-4;
     o->inner_max_iter = 200;
     o->max_outer      = 30;
+    o->smoother       = 0;
 }
 
 namespace
@@ -1020,8 +1028,19 @@ namespace
             delete h;
             return s;
         }
+        c->hierarchies.push_back(h);
         *out = h;
         return LAL_OK;
+    }
+}
+
+namespace lal
+{
+    void detach_hierarchies(lal_context* c)
+    {
+        for(lal_hierarchy_t* h : c->hierarchies)
+            h->c = nullptr;
+        c->hierarchies.clear();
     }
 }
 
@@ -1050,7 +1069,14 @@ extern "C" int lal_hierarchy_destroy(lal_hierarchy h)
 {
     if(h)
     {
-        (void)hipStreamSynchronize(h->c->stream);
+        if(h->c)
+        {
+            (void)hipStreamSynchronize(h->c->stream);
+            auto& v = h->c->hierarchies;
+            v.erase(std::remove(v.begin(), v.end(), h), v.end());
+        }
+        else
+            (void)hipDeviceSynchronize(); // its context is gone: wait for all work
         delete h;
     }
     return LAL_OK;
@@ -1096,8 +1122,9 @@ extern "C" int lal_pcg_solve_f64(lal_handle c, lal_hierarchy h, const double* b,
     if(opts)
         o = *opts;
     LAL_REQUIRE(o.max_iter >= 0 && o.n1 >= 0 && o.n2 >= 0);
+    LAL_REQUIRE(o.smoother >= 0 && o.smoother <= 6);
     return pcg<double>(h, b, x, h->res, h->z, h->p, h->q, o.rel_tol, o.abs_tol, o.max_iter,
-                       o.n1, o.n2, info);
+                       o.n1, o.n2, info, nullptr, o.smoother);
 }
 
 extern "C" int lal_pcg_solve_op_f64(lal_handle c, lal_csrmv_descr descr, int m, int nnz,
@@ -1114,6 +1141,7 @@ extern "C" int lal_pcg_solve_op_f64(lal_handle c, lal_csrmv_descr descr, int m, 
     if(opts)
         o = *opts;
     LAL_REQUIRE(o.max_iter >= 0 && o.n1 >= 0 && o.n2 >= 0);
+    LAL_REQUIRE(o.smoother >= 0 && o.smoother <= 6);
     // a non-owning view of the caller's matrix (never freed here)
     csr_dev A;
     A.m   = m;
@@ -1124,7 +1152,7 @@ extern "C" int lal_pcg_solve_op_f64(lal_handle c, lal_csrmv_descr descr, int m, 
     A.v   = const_cast<double*>(v);
     A.d   = descr;
     return pcg<double>(h, b, x, h->res, h->z, h->p, h->q, o.rel_tol, o.abs_tol, o.max_iter,
-                       o.n1, o.n2, info, &A);
+                       o.n1, o.n2, info, &A, o.smoother);
 }
 
 extern "C" int lal_pcg_solve_mixed(lal_handle c, lal_hierarchy h, const double* b, double* x,
@@ -1140,6 +1168,7 @@ extern "C" int lal_pcg_solve_mixed(lal_handle c, lal_hierarchy h, const double* 
     lal_solve_default_options(&o);
     if(opts)
         o = *opts;
+    LAL_REQUIRE(o.smoother == 0); // the FP32 levels run the Jacobi V-cycle only
     LAL_REQUIRE(o.max_iter >= 0 && o.n1 >= 0 && o.n2 >= 0);
     return pcg_mixed(h, b, x, o.rel_tol, o.abs_tol, o.max_iter, o.n1, o.n2, info);
 }
@@ -1158,6 +1187,7 @@ extern "C" int lal_pcg_solve_reliable(lal_handle c, lal_hierarchy h, const doubl
     lal_solve_default_options(&o);
     if(opts)
         o = *opts;
+    LAL_REQUIRE(o.smoother == 0); // the FP32 levels run the Jacobi V-cycle only
     LAL_REQUIRE(o.max_iter >= 0 && o.n1 >= 0 && o.n2 >= 0);
     // without options the update threshold is 1e-2, which also holds the
     // iteration count on badly conditioned operators
@@ -1179,6 +1209,7 @@ extern "C" int lal_ir_solve(lal_handle c, lal_hierarchy h, const double* b, doub
     lal_solve_default_options(&o);
     if(opts)
         o = *opts;
+    LAL_REQUIRE(o.smoother == 0); // the FP32 levels run the Jacobi V-cycle only
     const csr_dev& A = h->L[0].A;
     const int      n = A.m;
     double*        r = h->q; // FP64 outer residual lives in the FP64 q buffer

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <vector>
 
 #include "../../include/lal.h"
 
@@ -13,6 +14,8 @@
 #define LAL_RED_THREADS 256
 // Device result slots in the context (dot/norm outputs, solver scalars).
 #define LAL_NUM_SLOTS 64
+
+struct lal_hierarchy_t;
 
 struct lal_context
 {
@@ -27,6 +30,9 @@ struct lal_context
     double*   h_slots  = nullptr; // pinned mirror
     int*      d_flag   = nullptr;
     int*      h_flag   = nullptr;
+    // hierarchies built on this context: lal_destroy detaches them, so that
+    // one destroyed after its context no longer touches it
+    std::vector<lal_hierarchy_t*> hierarchies;
 };
 
 namespace lal
@@ -126,6 +132,8 @@ namespace lal
     int scan_i32_dev(lal_context* c, int n, int32_t* x);
     int scan_i64_dev(lal_context* c, int n, int64_t* x);
     int max_i64_dev(lal_context* c, int n, const int64_t* x, int64_t* h_out);
+    // Clears the context pointer of every hierarchy built on c (lal_destroy).
+    void detach_hierarchies(lal_context* c);
     // LAL_ERR_INVALID unless every ci[0..nnz) lies in [0, n) (blocks until known).
     int check_columns(lal_context* c, int nnz, int n, const int* ci);
 }

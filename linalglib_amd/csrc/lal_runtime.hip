@@ -95,6 +95,7 @@ extern "C" int lal_destroy(lal_handle c)
         return LAL_OK;
     if(c->stream)
         (void)hipStreamSynchronize(c->stream);
+    lal::detach_hierarchies(c);
     (void)hipFree(c->d_part);
     (void)hipFree(c->d_ticket);
     (void)hipFree(c->d_slots);

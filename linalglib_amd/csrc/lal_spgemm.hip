@@ -29,10 +29,11 @@ namespace
             atomicAdd(&cnt[ci[k]], 1);
     }
 
+    template <typename T>
     __global__ void __launch_bounds__(256)
         transpose_scatter_kernel(int m, const int* __restrict__ rp, const int* __restrict__ ci,
-                                 const double* __restrict__ v, int* __restrict__ cursor,
-                                 int* __restrict__ tci, double* __restrict__ tv)
+                                 const T* __restrict__ v, int* __restrict__ cursor,
+                                 int* __restrict__ tci, T* __restrict__ tv)
     {
         int i = blockIdx.x * 256 + threadIdx.x;
         if(i >= m)
@@ -50,9 +51,10 @@ namespace
     // ties broken by position; every load of the row completes before the
     // first store, so the row is permuted in place.  Rows longer than 256
     // entries take lane 0's insertion sort.
+    template <typename T>
     __global__ void __launch_bounds__(256)
         row_sort_wave_kernel(int m, const int* __restrict__ rp, int* __restrict__ ci,
-                             double* __restrict__ v)
+                             T* __restrict__ v)
     {
         const int lane = threadIdx.x & 63;
         const int i    = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
@@ -67,8 +69,8 @@ namespace
                 return;
             for(int j = s + 1; j < e; j++)
             {
-                int    kc = ci[j];
-                double kv = v[j];
+                int kc = ci[j];
+                T   kv = v[j];
                 int    t  = j - 1;
                 while(t >= s && ci[t] > kc)
                 {
@@ -81,14 +83,14 @@ namespace
             }
             return;
         }
-        int    key[4], rank[4];
-        double val[4];
+        int key[4], rank[4];
+        T   val[4];
 #pragma unroll
         for(int u = 0; u < 4; u++)
         {
             const int j = s + lane + 64 * u;
             key[u]      = j < e ? ci[j] : INT_MAX;
-            val[u]      = j < e ? v[j] : 0.0;
+            val[u]      = j < e ? v[j] : (T)0;
             rank[u]     = 0;
         }
         for(int t = s; t < e; t++)
@@ -462,8 +464,9 @@ namespace
 
 namespace lal
 {
-    int transpose_f64(lal_context* c, int m, int n, int nnz, const int* rp, const int* ci,
-                      const double* v, int* trp, int* tci, double* tv)
+    template <typename T>
+    int transpose_t(lal_context* c, int m, int n, int nnz, const int* rp, const int* ci,
+                    const T* v, int* trp, int* tci, T* tv)
     {
         LAL_HIP(hipMemsetAsync(trp, 0, sizeof(int) * (n + 1), c->stream));
         if(nnz > 0)
@@ -477,13 +480,19 @@ namespace lal
         int* cursor = nullptr;
         LAL_HIP(hipMallocAsync((void**)&cursor, sizeof(int) * std::max(n, 1), c->stream));
         LAL_HIP(hipMemcpyAsync(cursor, trp, sizeof(int) * n, hipMemcpyDeviceToDevice, c->stream));
-        transpose_scatter_kernel<<<grid_for(m, 256), 256, 0, c->stream>>>(m, rp, ci, v, cursor,
-                                                                          tci, tv);
+        transpose_scatter_kernel<T><<<grid_for(m, 256), 256, 0, c->stream>>>(m, rp, ci, v,
+                                                                             cursor, tci, tv);
         LAL_LAUNCHED();
-        row_sort_wave_kernel<<<(n + 3) / 4, 256, 0, c->stream>>>(n, trp, tci, tv);
+        row_sort_wave_kernel<T><<<(n + 3) / 4, 256, 0, c->stream>>>(n, trp, tci, tv);
         LAL_LAUNCHED();
         LAL_HIP(hipFreeAsync(cursor, c->stream));
         return LAL_OK;
+    }
+
+    int transpose_f64(lal_context* c, int m, int n, int nnz, const int* rp, const int* ci,
+                      const double* v, int* trp, int* tci, double* tv)
+    {
+        return transpose_t<double>(c, m, n, nnz, rp, ci, v, trp, tci, tv);
     }
 
     int csrgemm_nnz(lal_context* c, int m, int k, int n, const int* arp, const int* aci,
@@ -588,6 +597,14 @@ extern "C" int lal_transpose_f64(lal_handle c, int m, int n, int nnz, const int*
     LAL_REQUIRE(c != nullptr && m >= 0 && n >= 0 && nnz >= 0 && trp != nullptr);
     LAL_REQUIRE(nnz == 0 || (rp && ci && v && tci && tv));
     return lal::transpose_f64(c, m, n, nnz, rp, ci, v, trp, tci, tv);
+}
+
+extern "C" int lal_transpose_f32(lal_handle c, int m, int n, int nnz, const int* rp,
+                                 const int* ci, const float* v, int* trp, int* tci, float* tv)
+{
+    LAL_REQUIRE(c != nullptr && m >= 0 && n >= 0 && nnz >= 0 && trp != nullptr);
+    LAL_REQUIRE(nnz == 0 || (rp && ci && v && tci && tv));
+    return lal::transpose_t<float>(c, m, n, nnz, rp, ci, v, trp, tci, tv);
 }
 
 extern "C" int lal_csrgemm_nnz(lal_handle c, int m, int k, int n, int nnz_A, const int* arp,

@@ -42,7 +42,7 @@ class SaamgOptions(C.Structure):
 class SolveOptions(C.Structure):
     _fields_ = [("rel_tol", f64), ("abs_tol", f64), ("max_iter", i32), ("n1", i32),
                 ("n2", i32), ("inner_rel_tol", f64), ("inner_max_iter", i32),
-                ("max_outer", i32)]
+                ("max_outer", i32), ("smoother", i32)]
 
 
 class SolveInfo(C.Structure):
@@ -74,6 +74,12 @@ _SIGS = {
     "lal_axpy_f32": [vp, i32, f32, vp, vp],
     "lal_axpby_f32": [vp, i32, f32, vp, f32, vp],
     "lal_dot_f64": [vp, i32, vp, vp, C.POINTER(f64)],
+    "lal_axpy_dot_f64": [vp, i32, f64, vp, vp, vp, C.POINTER(f64)],
+    "lal_residual_f32": [vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, vp],
+    "lal_transpose_f32": [vp, i32, i32, i32, vp, vp, vp, vp, vp, vp],
+    "lal_pmis_init": [vp, i32, i32, vp, vp, vp, vp],
+    "lal_pmis_round": [vp, i32, i32, vp, vp, vp, vp, vp, vp, vp, C.POINTER(i32)],
+    "lal_pmis_join": [vp, i32, i32, vp, vp, vp, vp, vp, vp],
     "lal_dot_f32": [vp, i32, vp, vp, C.POINTER(f64)],
     "lal_norm_inf_f64": [vp, i32, vp, C.POINTER(f64)],
     "lal_norm_inf_f32": [vp, i32, vp, C.POINTER(f64)],
@@ -221,10 +227,15 @@ def saamg_options(**kw):
     return o
 
 
+SMOOTHER_CODES = {"jacobi": 0, "gs": 1, "sgs": 2, "sor": 3, "ssor": 4, "mcgs": 5, "mcsgs": 6}
+
+
 def solve_options(**kw):
     o = SolveOptions()
     load().lal_solve_default_options(C.byref(o))
     for k, v in kw.items():
+        if k == "smoother" and isinstance(v, str):
+            v = SMOOTHER_CODES[v]
         setattr(o, k, v)
     return o
 
@@ -316,7 +327,7 @@ class Hierarchy:
         return info
 
     CYCLES = {"v": 0, "w": 1, "f": 2}
-    SMOOTHERS = {"jacobi": 0, "gs": 1, "sgs": 2, "sor": 3, "ssor": 4}
+    SMOOTHERS = SMOOTHER_CODES
 
     def amg_solve(self, b, x, n1=1, n2=1, cycle="v", smoother="jacobi", **opts):
         """AMG as a solver (x in place); max_iter caps the number of cycles."""

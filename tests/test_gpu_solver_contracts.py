@@ -70,10 +70,14 @@ def test_pcg_op_against_oracle_32(h, oracle):
     log("pcg_op 32^3 vs oracle")
     info = H.pcg_op(A, dev(b), X, rel_tol=1e-10, abs_tol=1e-300, max_iter=1000)
     x = host(X)
-    assert info.converged and abs(info.iterations - ito) <= 1, (info.iterations, ito)
-    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= 1e-10
     r = b - pb.rhs_from_solution(rp, ci, v, x)
-    assert np.abs(r).max() <= 1.01e-10 * np.abs(b).max()
+    err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+    res = np.abs(r).max() / np.abs(b).max()
+    log(f"pcg_op 32^3: {info.iterations} iterations (oracle {ito}), rel. error {err:.2e}, "
+        f"rel. residual {res:.2e}")
+    assert info.converged and abs(info.iterations - ito) <= 1, (info.iterations, ito)
+    assert err <= 1e-10
+    assert res <= 1.01e-10
 
 
 def test_pcg_op_scaled_hierarchy_gives_own_iterates(h):
