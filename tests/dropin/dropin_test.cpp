@@ -1,0 +1,475 @@
+// LinAlgLib's own front end on the MI355X backend: this program is written
+// against the reference's public API (linalg.h) and linked with the
+// reference's sources plus the device shim bodies of lal_device_shims.cpp.
+// Every device-resident call below goes backend_dispatch (utility.h:163-193)
+// -> device_* shim -> liblal.  Each result is compared with the same call on
+// the host path of the same library.  Exit status 0 = every check passed.
+//
+// The reference's SA setup, PMIS driver and V-cycle are host-only code
+// (saamg.cpp reads the matrix arrays directly; amg_aggregation.cpp:167-170
+// indexes device vectors on the host; amg.cpp:144-267 smooths with host
+// loops), so on the device the SA-AMG preconditioner is the new
+// saamg_precond below, built on lal_saamg_setup and applied with
+// lal_amg_precond_f64, and the PMIS shims are driven by this program the way
+// amg_aggregation.cpp:106-215 drives them.
+#include "linalg.h"
+
+#include "backend/device/device_amg_aggregation.h"
+#include "iterative_solvers/amg/amg_aggregation.h"
+#include "iterative_solvers/amg/amg_strength.h"
+#include "lal.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fcntl.h>
+#include <iostream>
+#include <random>
+#include <unistd.h>
+#include <vector>
+
+using namespace linalg;
+
+// the shims' context (lal_device_shims.cpp): one stream for all device work
+extern "C" lal_handle lal_dropin_context();
+
+namespace
+{
+    int failures = 0;
+
+    void expect(bool ok, const char* what)
+    {
+        std::printf("%s %s\n", ok ? "OK  " : "FAIL", what);
+        std::fflush(stdout);
+        if(!ok)
+            failures++;
+    }
+
+    // LinAlgLib prints progress to stdout; keep it out of this program's log
+    struct quiet
+    {
+        int saved;
+        quiet()
+        {
+            std::fflush(stdout);
+            std::cout.flush();
+            saved  = dup(1);
+            int nd = open("/dev/null", O_WRONLY);
+            dup2(nd, 1);
+            close(nd);
+        }
+        ~quiet()
+        {
+            std::cout.flush();
+            std::fflush(stdout);
+            dup2(saved, 1);
+            close(saved);
+        }
+    };
+
+    void poisson3d(int n, std::vector<int>& rp, std::vector<int>& ci, std::vector<double>& v)
+    {
+        int m = n * n * n;
+        rp.assign(m + 1, 0);
+        ci.clear();
+        v.clear();
+        for(int k = 0; k < n; k++)
+            for(int j = 0; j < n; j++)
+                for(int i = 0; i < n; i++)
+                {
+                    int r = i + n * (j + n * k);
+                    rp[r] = (int)ci.size();
+                    auto add = [&](int c, double a) {
+                        ci.push_back(c);
+                        v.push_back(a);
+                    };
+                    if(k > 0)
+                        add(r - n * n, -1);
+                    if(j > 0)
+                        add(r - n, -1);
+                    if(i > 0)
+                        add(r - 1, -1);
+                    add(r, 6);
+                    if(i < n - 1)
+                        add(r + 1, -1);
+                    if(j < n - 1)
+                        add(r + n, -1);
+                    if(k < n - 1)
+                        add(r + n * n, -1);
+                }
+        rp[m] = (int)ci.size();
+    }
+
+    template <typename T>
+    std::vector<T> host_copy(vector<T>& v)
+    {
+        std::vector<T> h(v.get_size());
+        if(v.is_on_host())
+            std::memcpy(h.data(), v.get_vec(), sizeof(T) * h.size());
+        else
+            lal_memcpy_d2h(lal_dropin_context(), h.data(), v.get_vec(), sizeof(T) * h.size());
+        return h;
+    }
+
+    template <typename T>
+    bool bits_equal(const std::vector<T>& a, const std::vector<T>& b)
+    {
+        return a.size() == b.size() && std::memcmp(a.data(), b.data(), sizeof(T) * a.size()) == 0;
+    }
+
+    double rel_diff(const std::vector<double>& a, const std::vector<double>& b)
+    {
+        double num = 0, den = 0;
+        for(size_t i = 0; i < a.size(); i++)
+        {
+            num += (a[i] - b[i]) * (a[i] - b[i]);
+            den += b[i] * b[i];
+        }
+        return std::sqrt(num / (den > 0 ? den : 1));
+    }
+
+    void csr_host(csr_matrix<double>& M, std::vector<int>& rp, std::vector<int>& ci,
+                  std::vector<double>& v)
+    {
+        rp.resize(M.get_m() + 1);
+        ci.resize(M.get_nnz());
+        v.resize(M.get_nnz());
+        if(M.is_on_host())
+        {
+            std::memcpy(rp.data(), M.get_row_ptr(), sizeof(int) * rp.size());
+            std::memcpy(ci.data(), M.get_col_ind(), sizeof(int) * ci.size());
+            std::memcpy(v.data(), M.get_val(), sizeof(double) * v.size());
+        }
+        else
+        {
+            lal_handle c = lal_dropin_context();
+            lal_memcpy_d2h(c, rp.data(), M.get_row_ptr(), sizeof(int) * rp.size());
+            lal_memcpy_d2h(c, ci.data(), M.get_col_ind(), sizeof(int) * ci.size());
+            lal_memcpy_d2h(c, v.data(), M.get_val(), sizeof(double) * v.size());
+        }
+    }
+
+    // The SA-AMG preconditioner of the device path: lal_saamg_setup on the
+    // device matrix, then one V-cycle per application (the reference's
+    // composition x.zeros(); amg_solve(h, x, rhs, 1, 1, vcycle, jacobi,
+    // {max_cycle = 0}) as lal_amg_precond_f64).
+    class saamg_precond : public preconditioner
+    {
+    public:
+        lal_handle    ctx = nullptr;
+        lal_hierarchy h   = nullptr;
+        void          build(const csr_matrix<double>& A) override
+        {
+            lal_saamg_options o;
+            lal_saamg_default_options(&o);
+            if(lal_saamg_setup(ctx, A.get_m(), A.get_n(), A.get_nnz(), A.get_row_ptr(),
+                               A.get_col_ind(), A.get_val(), &o, &h)
+               != LAL_OK)
+                std::printf("lal_saamg_setup failed: %s\n", lal_last_error());
+        }
+        void solve(const vector<double>& rhs, vector<double>& x) const override
+        {
+            if(lal_amg_precond_f64(ctx, h, rhs.get_vec(), x.get_vec(), 1, 1) != LAL_OK)
+                std::printf("lal_amg_precond_f64 failed: %s\n", lal_last_error());
+        }
+        void move_to_device() override {}
+        void move_to_host() override {}
+        bool is_on_host() const override
+        {
+            return false;
+        }
+        ~saamg_precond()
+        {
+            lal_hierarchy_destroy(h);
+        }
+    };
+
+    // The host composition of the same preconditioner (reference code only).
+    class host_amg_precond : public preconditioner
+    {
+    public:
+        const hierarchy* h = nullptr;
+        void             build(const csr_matrix<double>&) override {}
+        void             solve(const vector<double>& rhs, vector<double>& x) const override
+        {
+            x.zeros();
+            iter_control c;
+            c.max_cycle = 0;
+            amg_solve(*h, x, rhs, 1, 1, cycle::vcycle, smoother::jacobi, c);
+        }
+        void move_to_device() override {}
+        void move_to_host() override {}
+        bool is_on_host() const override
+        {
+            return true;
+        }
+    };
+}
+
+int main()
+{
+    const int           n = 20;
+    std::vector<int>    rp, ci;
+    std::vector<double> v;
+    poisson3d(n, rp, ci, v);
+    const int m = n * n * n, nnz = rp[m];
+    std::mt19937                           gen(7);
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    std::vector<double>                    xs(m), ys(m);
+    for(auto& t : xs)
+        t = U(gen);
+    for(auto& t : ys)
+        t = U(gen);
+
+    csr_matrix<double> Ah(rp, ci, v, m, m, nnz), Ad(rp, ci, v, m, m, nnz);
+    Ad.move_to_device();
+    expect(!Ad.is_on_host() && Ah.is_on_host(), "csr_matrix::move_to_device keeps A on the GPU");
+
+    // SpMV (csrmv_analysis + csrmv_solve shims), residual, BLAS-1
+    {
+        vector<double> xh(xs), xd(xs), yh(m), yd(m), bh(ys), bd(ys), rh(m), rd(m);
+        xd.move_to_device();
+        yd.move_to_device();
+        bd.move_to_device();
+        rd.move_to_device();
+        Ah.multiply_by_vector(yh, xh);
+        Ad.multiply_by_vector(yd, xd);
+        auto y1 = host_copy(yh), y2 = host_copy(yd);
+        double md = 0, mx = 0;
+        for(int i = 0; i < m; i++)
+        {
+            md = std::fmax(md, std::fabs(y1[i] - y2[i]));
+            mx = std::fmax(mx, std::fabs(y1[i]));
+        }
+        expect(md <= 1e-14 * mx, "multiply_by_vector: device within the summation bound");
+        compute_residual(Ah, xh, bh, rh);
+        compute_residual(Ad, xd, bd, rd);
+        expect(bits_equal(host_copy(rh), host_copy(rd)), "compute_residual: bit for bit");
+        const double dh = dot_product(xh, bh), dd = dot_product(xd, bd);
+        expect(std::fabs(dh - dd) <= 1e-13 * std::fabs(dh) + 1e-13, "dot_product");
+        expect(norm_inf(xh) == norm_inf(xd), "norm_inf: exact");
+        axpy(0.37, xh, bh);
+        axpy(0.37, xd, bd);
+        axpby(-1.5, xh, 0.25, bh);
+        axpby(-1.5, xd, 0.25, bd);
+        expect(bits_equal(host_copy(bh), host_copy(bd)), "axpy, axpby: bit for bit");
+        vector<double> dh_(m), dd_(m);
+        dd_.move_to_device();
+        dd_.zeros();
+        dh_.zeros();
+        Ah.extract_diagonal(dh_);
+        Ad.extract_diagonal(dd_);
+        expect(bits_equal(host_copy(dh_), host_copy(dd_)), "extract_diagonal: bit for bit");
+        std::vector<int64_t> s(m);
+        for(int i = 0; i < m; i++)
+            s[i] = (i * 7919) % 5 - 1;
+        vector<int64_t> sh(s), sd(s);
+        sd.move_to_device();
+        exclusive_scan(sh);
+        exclusive_scan(sd);
+        expect(bits_equal(host_copy(sh), host_copy(sd)), "exclusive_scan<int64_t>: bit for bit");
+    }
+
+    // SpGEMM (csrgemm_nnz + csrgemm_solve shims) and transpose
+    {
+        csr_matrix<double> Ch, Cd, Th, Td;
+        Cd.move_to_device();
+        Td.move_to_device();
+        Ah.multiply_by_matrix(Ch, Ah);
+        Ad.multiply_by_matrix(Cd, Ad);
+        std::vector<int>    r1, c1, r2, c2;
+        std::vector<double> v1, v2;
+        csr_host(Ch, r1, c1, v1);
+        csr_host(Cd, r2, c2, v2);
+        expect(bits_equal(r1, r2) && bits_equal(c1, c2) && bits_equal(v1, v2),
+               "multiply_by_matrix (A*A): bit for bit");
+        // a rectangular operator: rows 0..m/2 of A*A
+        const int           mh = m / 2;
+        std::vector<int>    rr(r1.begin(), r1.begin() + mh + 1);
+        std::vector<int>    rc(c1.begin(), c1.begin() + rr[mh]);
+        std::vector<double> rv(v1.begin(), v1.begin() + rr[mh]);
+        csr_matrix<double>  Bh(rr, rc, rv, mh, m, rr[mh]), Bd(rr, rc, rv, mh, m, rr[mh]);
+        Bd.move_to_device();
+        Bh.transpose(Th);
+        Bd.transpose(Td);
+        csr_host(Th, r1, c1, v1);
+        csr_host(Td, r2, c2, v2);
+        expect(bits_equal(r1, r2) && bits_equal(c1, c2) && bits_equal(v1, v2),
+               "transpose (rectangular): bit for bit");
+    }
+
+    // strength of connection and the three PMIS shims
+    {
+        bool ok_conn = false, ok_agg = false;
+        {
+            quiet       q;
+            vector<int> ch(nnz), cd(nnz);
+            cd.move_to_device();
+            ch.zeros();
+            cd.zeros();
+            compute_strong_connections(Ah, 0.08, ch);
+            compute_strong_connections(Ad, 0.08, cd);
+            ok_conn = bits_equal(host_copy(ch), host_copy(cd));
+            vector<int64_t> ah(m), roots_h;
+            ah.zeros();
+            compute_aggregates_using_pmis(Ah, ch, ah, roots_h);
+            // amg_aggregation.cpp:106-215 over the device shims
+            vector<int>     hash(m), state(m), max_state(m);
+            vector<int64_t> ad(m), roots_d(m);
+            hash.move_to_device();
+            state.move_to_device();
+            max_state.move_to_device();
+            ad.move_to_device();
+            roots_d.move_to_device();
+            ad.zeros();
+            device_initialize_pmis_state(Ad, cd, max_state, hash);
+            for(int iter = 0; iter < 20; iter++)
+            {
+                state.copy_from(max_state);
+                bool complete = true;
+                device_find_maximum_distance_two_node(Ad, cd, state, hash, ad, max_state,
+                                                      complete);
+                if(complete)
+                    break;
+            }
+            exclusive_scan(ad);
+            for(int k = 0; k < 2; k++)
+            {
+                state.copy_from(max_state);
+                device_add_unassigned_nodes_to_closest_aggregation(Ad, cd, state, ad, roots_d,
+                                                                   max_state);
+            }
+            ok_agg = bits_equal(host_copy(ah), host_copy(ad));
+        }
+        expect(ok_conn, "compute_strong_connections: bit for bit");
+        expect(ok_agg, "PMIS init/round/join shims: the host aggregates bit for bit");
+    }
+
+    iter_control control;
+    control.rel_tol  = 1e-10;
+    control.abs_tol  = 1e-300;
+    control.max_iter = 1000;
+    std::vector<double> bvec(m);
+    {
+        vector<double> xsv(xs), bh(m);
+        Ah.multiply_by_vector(bh, xsv);
+        bvec = host_copy(bh);
+    }
+
+    // cg_solver::solve with jacobi_precond, host against device
+    {
+        int            ith, itd;
+        vector<double> xh(m), xd(m), bh(bvec), bd(bvec);
+        xh.zeros();
+        {
+            quiet          q;
+            jacobi_precond ph, pd;
+            ph.build(Ah);
+            pd.build(Ah);
+            pd.move_to_device();
+            xd.move_to_device();
+            bd.move_to_device();
+            xd.zeros();
+            cg_solver ch, cd;
+            ch.build(Ah);
+            cd.build(Ah);
+            cd.move_to_device();
+            ith = ch.solve(Ah, xh, bh, &ph, control);
+            itd = cd.solve(Ad, xd, bd, &pd, control);
+        }
+        const double e = rel_diff(host_copy(xd), host_copy(xh));
+        std::printf("     Jacobi-PCG iterations host %d, device %d, rel. diff %.2e\n", ith, itd, e);
+        expect(std::abs(ith - itd) <= 1 && e <= 1e-9, "cg_solver + jacobi_precond on the device");
+    }
+
+    // cg_solver::solve with the SA-AMG preconditioner, host against device
+    {
+        lal_handle     ctx = lal_dropin_context();
+        int            ith, itd;
+        vector<double> xh(m), xd(m), bh(bvec), bd(bvec);
+        xh.zeros();
+        hierarchy        hh;
+        host_amg_precond ph;
+        saamg_precond    pd;
+        pd.ctx = ctx;
+        {
+            quiet q;
+            saamg_setup(Ah, 20, hh);
+            ph.h = &hh;
+            pd.build(Ad);
+            xd.move_to_device();
+            bd.move_to_device();
+            xd.zeros();
+            cg_solver ch, cd;
+            ch.build(Ah);
+            cd.build(Ah);
+            cd.move_to_device();
+            ith = ch.solve(Ah, xh, bh, &ph, control);
+            itd = cd.solve(Ad, xd, bd, &pd, control);
+        }
+        // the device hierarchy against the host one, level by level
+        int levels = -1;
+        lal_hierarchy_levels(pd.h, &levels);
+        bool same = levels == hh.total_levels;
+        for(int l = 0; same && l <= levels; l++)
+            for(int kind = 0; kind < 3 && same; kind++)
+            {
+                if(kind > 0 && l == levels)
+                    continue;
+                csr_matrix<double>& H = kind == 0   ? hh.A_cs[l]
+                                        : kind == 1 ? hh.prolongations[l]
+                                                    : hh.restrictions[l];
+                int           dm, dn, dz;
+                const int *   drp, *dci;
+                const double* dv;
+                lal_hierarchy_level(pd.h, l, kind, &dm, &dn, &dz, &drp, &dci, &dv);
+                std::vector<int>    r1, c1, r2(dm + 1), c2(dz);
+                std::vector<double> v1, v2(dz);
+                csr_host(H, r1, c1, v1);
+                lal_memcpy_d2h(ctx, r2.data(), drp, sizeof(int) * r2.size());
+                lal_memcpy_d2h(ctx, c2.data(), dci, sizeof(int) * c2.size());
+                lal_memcpy_d2h(ctx, v2.data(), dv, sizeof(double) * v2.size());
+                same = dm == H.get_m() && dn == H.get_n() && bits_equal(r1, r2)
+                       && bits_equal(c1, c2) && bits_equal(v1, v2);
+            }
+        expect(same, "saamg_setup on the host and lal_saamg_setup: identical hierarchies");
+        const double e = rel_diff(host_copy(xd), host_copy(xh));
+        std::printf("     SA-AMG-PCG iterations host %d, device %d, rel. diff %.2e\n", ith, itd,
+                    e);
+        expect(std::abs(ith - itd) <= 1 && e <= 1e-10,
+               "cg_solver + SA-AMG preconditioner on the device");
+        lal_hierarchy_destroy(pd.h);
+        pd.h = nullptr;
+    }
+
+    // bicgstab_solver with jacobi_precond, host against device
+    {
+        int            ith, itd;
+        vector<double> xh(m), xd(m), bh(bvec), bd(bvec);
+        xh.zeros();
+        {
+            quiet          q;
+            jacobi_precond ph, pd;
+            ph.build(Ah);
+            pd.build(Ah);
+            pd.move_to_device();
+            xd.move_to_device();
+            bd.move_to_device();
+            xd.zeros();
+            bicgstab_solver sh, sd;
+            sh.build(Ah);
+            sd.build(Ah);
+            sd.move_to_device();
+            ith = sh.solve(Ah, xh, bh, &ph, control);
+            itd = sd.solve(Ad, xd, bd, &pd, control);
+        }
+        const double e = rel_diff(host_copy(xd), host_copy(xh));
+        std::printf("     Jacobi-BiCGSTAB iterations host %d, device %d, rel. diff %.2e\n", ith,
+                    itd, e);
+        expect(std::abs(ith - itd) <= 2 && e <= 1e-8,
+               "bicgstab_solver + jacobi_precond on the device");
+    }
+
+    std::printf(failures ? "FAILURES %d\n" : "DROPIN OK\n", failures);
+    return failures ? 1 : 0;
+}
