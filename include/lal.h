@@ -101,6 +101,16 @@ int lal_norm_inf_f32(lal_handle h, int n, const float* x, double* result);
 /* In-place exclusive prefix sum. */
 int lal_exclusive_scan_i32(lal_handle h, int n, int32_t* x);
 int lal_exclusive_scan_i64(lal_handle h, int n, int64_t* x);
+/* lal_exclusive_scan_i64 without any allocation.  No counterpart in
+ * LinAlgLib: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same kernels and result as lal_exclusive_scan_i64, bit for bit; the tile
+ * sums live in a static per-device array of LAL_SCAN_NOALLOC_MAX_N / 2048
+ * int64 entries (256 KiB), so n > LAL_SCAN_NOALLOC_MAX_N is refused with
+ * LAL_ERR_INVALID.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (two streams or two handles at once) race. */
+#define LAL_SCAN_NOALLOC_MAX_N 67108864
+int lal_exclusive_scan_i64_noalloc(lal_handle h, int n, int64_t* x);
 
 /* ------------------------------------------------------------------------ */
 /* CSR SpMV -- replaces backend/device/device_matrix_vector.h:35-45          */

@@ -6,6 +6,19 @@
 // repeatable run to run (no floating-point atomics).
 #include "lal_internal.h"
 
+#include <mutex>
+
+// Scratch of lal_exclusive_scan_i64_noalloc: the scan's tile sums, one int64
+// per 2048 entries (its tile), in a static device array so that the entry
+// point allocates nothing.  Every GPU the library runs on holds its own copy
+// until the process ends.  Host code never passes the array's name to a
+// kernel (on the host it is a stand-in, not a device address): it uses the
+// address hipGetSymbolAddress returned for that device (g_scratch).
+static __device__ __align__(256) int64_t scratch[LAL_SCAN_NOALLOC_MAX_N / 2048];
+
+static std::mutex g_scratch_mu;
+static int64_t*   g_scratch[LAL_MAX_DEVICES] = {};
+
 namespace
 {
     // ---------------------------------------------------------------- axpy
@@ -295,6 +308,8 @@ namespace
         }
     }
 
+    static_assert(SCAN_TILE == 2048, "the no-allocation scratch holds one sum per 2048 entries");
+
     // The three passes of the scan with the tile sums in `sums` (ntiles
     // entries, every one written by the first pass before it is read).
     template <typename T>
@@ -386,6 +401,29 @@ namespace lal
     int scan_i64_dev(lal_context* c, int n, int64_t* x)
     {
         return scan_dev<int64_t>(c, n, x);
+    }
+
+    // Called by lal_create with `device` current: the device address of that
+    // device's scratch array, looked up once per device.
+    int noalloc_scratch_init(int device)
+    {
+        if(device < 0 || device >= LAL_MAX_DEVICES)
+        {
+            set_last_error("device ordinal beyond LAL_MAX_DEVICES");
+            return LAL_ERR_NO_DEVICE;
+        }
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if(g_scratch[device] != nullptr)
+            return LAL_OK;
+        void* p = nullptr;
+        LAL_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if(p == nullptr)
+        {
+            set_last_error("hipGetSymbolAddress returned no address for the scan scratch");
+            return LAL_ERR_HIP;
+        }
+        g_scratch[device] = static_cast<int64_t*>(p);
+        return LAL_OK;
     }
     int max_i64_dev(lal_context* c, int n, const int64_t* x, int64_t* h_out)
     {
@@ -497,5 +535,36 @@ extern "C" int lal_exclusive_scan_i64(lal_handle c, int n, int64_t* x)
 {
     LAL_REQUIRE(c != nullptr && n >= 0 && (n == 0 || x != nullptr));
     return lal::scan_i64_dev(c, n, x);
+}
+
+extern "C" int lal_exclusive_scan_i64_noalloc(lal_handle c, int n, int64_t* x)
+{
+    LAL_REQUIRE(c != nullptr && n >= 0 && (n == 0 || x != nullptr));
+    if(n > LAL_SCAN_NOALLOC_MAX_N)
+    {
+        char buf[160];
+        snprintf(buf, sizeof(buf),
+                 "lal_exclusive_scan_i64_noalloc: n = %d exceeds the static scratch limit "
+                 "LAL_SCAN_NOALLOC_MAX_N = %d (use lal_exclusive_scan_i64)",
+                 n, LAL_SCAN_NOALLOC_MAX_N);
+        lal::set_last_error(buf);
+        return LAL_ERR_INVALID;
+    }
+    if(n == 0)
+        return LAL_OK;
+    int64_t* sums = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if(c->device >= 0 && c->device < LAL_MAX_DEVICES)
+            sums = g_scratch[c->device];
+    }
+    if(sums == nullptr)
+    {
+        lal::set_last_error("lal_exclusive_scan_i64_noalloc: no scratch address for the "
+                            "handle's device (lal_create did not set it up)");
+        return LAL_ERR_INVALID;
+    }
+    // the first pass writes every tile sum the next two read: nothing to clear
+    return scan_passes<int64_t>(c, n, x, sums);
 }
 

@@ -17,6 +17,9 @@
 
 struct lal_hierarchy_t;
 
+// Devices the per-device static scratch table covers.
+#define LAL_MAX_DEVICES 64
+
 struct lal_context
 {
     hipStream_t stream     = nullptr;
@@ -131,6 +134,9 @@ namespace lal
     int norm_inf_f64_dev(lal_context* c, int n, const double* x, double* d_out, int slot);
     int scan_i32_dev(lal_context* c, int n, int32_t* x);
     int scan_i64_dev(lal_context* c, int n, int64_t* x);
+    // Device address of the allocation-free scan's static scratch for
+    // `device` (which must be current), looked up once per device.
+    int noalloc_scratch_init(int device);
     int max_i64_dev(lal_context* c, int n, const int64_t* x, int64_t* h_out);
     // Clears the context pointer of every hierarchy built on c (lal_destroy).
     void detach_hierarchies(lal_context* c);

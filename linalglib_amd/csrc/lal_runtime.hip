@@ -73,6 +73,14 @@ extern "C" int lal_create(lal_handle* handle)
         return LAL_ERR_NO_DEVICE;
     }
     c->num_cu = prop.multiProcessorCount;
+    {
+        const int st = lal::noalloc_scratch_init(c->device);
+        if(st != LAL_OK)
+        {
+            delete c;
+            return st;
+        }
+    }
     LAL_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     c->own_stream = true;
     LAL_HIP(hipMalloc(&c->d_part, sizeof(double) * LAL_RED_BLOCKS * 2));

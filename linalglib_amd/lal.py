@@ -85,6 +85,7 @@ _SIGS = {
     "lal_norm_inf_f32": [vp, i32, vp, C.POINTER(f64)],
     "lal_exclusive_scan_i32": [vp, i32, vp],
     "lal_exclusive_scan_i64": [vp, i32, vp],
+    "lal_exclusive_scan_i64_noalloc": [vp, i32, vp],
     "lal_csrmv_analysis": [vp, i32, i32, i32, vp, vp, i32, C.POINTER(vp)],
     "lal_csrmv_descr_destroy": [vp],
     "lal_csrmv_f64": [vp, vp, i32, i32, i32, f64, vp, vp, vp, vp, f64, vp],
