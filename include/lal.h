@@ -131,7 +131,8 @@ typedef struct lal_csrmv_info* lal_csrmv_descr;
 int lal_csrmv_analysis(lal_handle h, int m, int n, int nnz, const int* row_ptr,
                        const int* col_ind, int exact, lal_csrmv_descr* descr);
 int lal_csrmv_descr_destroy(lal_csrmv_descr descr);
-/* descr may be NULL: rows are then taken 256 per block. */
+/* descr may be NULL: rows are then taken 256 per block and each is summed by
+ * one lane in CSR order, as with an exact descriptor. */
 int lal_csrmv_f64(lal_handle h, lal_csrmv_descr descr, int m, int n, int nnz, double alpha,
                   const int* row_ptr, const int* col_ind, const double* val, const double* x,
                   double beta, double* y);

@@ -130,10 +130,11 @@ namespace
             a.keep = d->keep;
             return lal::rowvec_launch<T>(c, MODE == 0 ? lal::RV_SPMV : lal::RV_RESIDUAL, a);
         }
+        // an exact descriptor, or none: one lane per row in CSR order (the
+        // host loop's rounding)
         int nb = lal::spmv_grid(d, m);
         csrmv_kernel<T, ACC, MODE><<<nb, SPMV_THREADS, 0, c->stream>>>(
-            d ? d->d_blocks : nullptr, nb, m, rp, ci, v, x, alpha, beta, b, y,
-            d ? d->exact : 0);
+            d ? d->d_blocks : nullptr, nb, m, rp, ci, v, x, alpha, beta, b, y, 1);
         LAL_LAUNCHED();
         return LAL_OK;
     }

@@ -671,6 +671,8 @@ namespace
                 LAL_TRY(apply(fast));
             LAL_TRY(wait_scalar(h, seq, lal::S_RNORM, &rn));
             lal::observe_rate(rn, &r_prev, &rate);
+            if(std::isnan(rn))
+                break; // a NaN in the residual: not converged
             if(converges(rn, init, rel_tol, abs_tol))
             {
                 conv = true;
@@ -751,6 +753,8 @@ namespace
                 LAL_TRY(precond<float>(h, r32, z32, n1, n2, fast));
             LAL_TRY(wait_scalar(h, seq, lal::S_RNORM, &rn));
             lal::observe_rate(rn, &r_prev, &rate);
+            if(std::isnan(rn))
+                break; // a NaN in the residual: not converged
             if(converges(rn, init, rel_tol, abs_tol))
             {
                 conv = true;
@@ -787,9 +791,9 @@ namespace
     // the tolerance, the FP64 solution takes the correction (x += y ||r0||)
     // and the recursive residual is replaced by the true FP64 residual
     // b - A x; convergence is declared on that true residual only.  The
-    // Krylov space is kept across updates (no restart), so the iteration
-    // count stays that of the FP64 solve while every pass but the updates
-    // moves FP32 data.
+    // Krylov space is kept across updates, so the iteration count stays that
+    // of the FP64 solve while every pass but the updates moves FP32 data;
+    // only an update that fails to cut the true residual tenfold restarts it.
     int pcg_reliable(lal_hierarchy_t* h, const double* b, double* x, double rel_tol,
                      double abs_tol, int max_iter, int n1, int n2, double delta,
                      lal_solve_info* info)
@@ -856,6 +860,7 @@ namespace
                 if(std::isnan(rn))
                     break;
                 const bool update = rn <= delta * r_upd || converges(rn, init, rel_tol, abs_tol);
+                bool       restart = false;
                 if(update)
                 {
                     // x += ||r0|| y in FP64; r <- the true residual (scaled)
@@ -868,8 +873,18 @@ namespace
                     LAL_TRY(lal::scale_cast_f64_f32(c, n, r64, sigma, r32));
                     LAL_TRY(wait_scalar(h, seq, lal::S_RNORM, &rn));
                     updates++;
-                    r_upd  = rn;
-                    r_prev = rn;
+                    // An update that did not cut the true residual tenfold
+                    // means the FP32 recurrence has drifted from the FP64 system
+                    // (badly conditioned operators): the kept search direction
+                    // no longer belongs to the new residual, so the Krylov space
+                    // restarts from it (p = z), as iterative refinement would.
+                    // Healthy runs (a reduction near delta per update) never
+                    // restart and keep their iterates.
+                    restart = rn > 0.1 * r_upd;
+                    r_upd   = rn;
+                    r_prev  = rn;
+                    if(std::isnan(rn))
+                        break;
                     if(converges(rn, init, rel_tol, abs_tol))
                     {
                         conv = true;
@@ -880,8 +895,12 @@ namespace
                 else if(near)
                     LAL_TRY(precond<float>(h, r32, z32, n1, n2, fast));
                 LAL_TRY(lal::dot_partials<float, float>(c, n, z32, r32, part_g[cur ^ 1]));
-                LAL_TRY(lal::update_p<float, float>(c, n, z32, p32, part_g[cur ^ 1],
-                                                    part_g[cur]));
+                if(restart)
+                    LAL_HIP(hipMemcpyAsync(p32, z32, sizeof(float) * n, hipMemcpyDeviceToDevice,
+                                           c->stream));
+                else
+                    LAL_TRY(lal::update_p<float, float>(c, n, z32, p32, part_g[cur ^ 1],
+                                                        part_g[cur]));
                 cur ^= 1;
                 iter++;
             }
@@ -1248,6 +1267,8 @@ extern "C" int lal_ir_solve(lal_handle c, lal_hierarchy h, const double* b, doub
         LAL_TRY(post_scalar(h, 1, part_r, lal::S_RNORM, &seq));
         LAL_TRY(wait_scalar(h, seq, lal::S_RNORM, &rn));
         outer++;
+        if(std::isnan(rn))
+            break;
     }
     if(info)
     {

@@ -125,6 +125,19 @@ namespace lal
         return v;
     }
 
+    // Wavefront maximum that returns NaN when any lane holds one.
+    template <typename T>
+    __device__ __forceinline__ T wave_max_nan(T v)
+    {
+#pragma unroll
+        for(int o = 32; o > 0; o >>= 1)
+        {
+            T w = __shfl_xor(v, o, 64);
+            v   = (v != v || w < v) ? v : w;
+        }
+        return v;
+    }
+
     // Internal launchers shared across translation units (device-resident
     // results; nothing is copied to the host).
     int dot_f64_dev(lal_context* c, int n, const double* x, const double* y, double* d_out,

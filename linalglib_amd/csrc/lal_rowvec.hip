@@ -5,6 +5,10 @@ namespace
 {
     using lal::SPMV_THREADS;
 
+    // LDS bytes of a block's row starts (nrows + 1 <= SPMV_THREADS + 1 ints,
+    // rounded up to 16 bytes so that the staged window stays aligned).
+    constexpr int RP_LDS_BYTES = (int)(((SPMV_THREADS + 1) * sizeof(int) + 15) & ~15);
+
     template <typename T>
     __device__ __forceinline__ T stream_load(const T* p)
     {
@@ -238,10 +242,12 @@ namespace
                     const T* __restrict__ d, const T* __restrict__ b, T alpha, T beta,
                     T* __restrict__ y, T* __restrict__ r)
     {
-        // win entries of columns, then win values (win: a multiple of 4)
+        // the block's row starts, then win entries of columns, then win
+        // values (win: a multiple of 4)
         extern __shared__ __attribute__((aligned(16))) unsigned char s_raw[];
-        int* s_c = reinterpret_cast<int*>(s_raw);
-        T*   s_v = reinterpret_cast<T*>(s_raw + sizeof(int) * win);
+        int* s_rp = reinterpret_cast<int*>(s_raw);
+        int* s_c  = reinterpret_cast<int*>(s_raw + RP_LDS_BYTES);
+        T*   s_v  = reinterpret_cast<T*>(s_raw + RP_LDS_BYTES + sizeof(int) * win);
         const int WIN = win;
 
         const int   tid  = threadIdx.x;
@@ -256,9 +262,13 @@ namespace
         const int  row    = r0 + lrow;
         const int  base   = mt.z;
         const int  end    = mt.w;
-        const int  rs     = active ? rp[row] : 0;
-        const int  re     = active ? rp[row + 1] : 0;
         const int  nnz4   = nnz & ~3;
+        // The row bounds travel with the first window's loads (through LDS)
+        // instead of being awaited before them: rp[r0] and rp[r1] are the
+        // descriptor's base and end, lane i loads rp[r0 + i] for 0 < i < nrows.
+        const int rpv = (tid > 0 && tid < nrows) ? rp[r0 + tid] : 0;
+        int       rs = 0, re = 0;
+        bool      bounds = false;
 
         T acc = 0;
         for(int ws = base & ~3; ws < end; ws += WIN)
@@ -269,23 +279,26 @@ namespace
             {
                 int4v   cq[2];
                 quad<T> vq[2];
+                int     qq[2];
 #pragma unroll
                 for(int u = 0; u < 2; u++)
                 {
-                    const int q = min(q0 + 4 * SPMV_THREADS * u, vend - 4);
-                    const int4v* cp = reinterpret_cast<const int4v*>(ci + q);
+                    qq[u]           = min(q0 + 4 * SPMV_THREADS * u, vend - 4);
+                    const int4v* cp = reinterpret_cast<const int4v*>(ci + qq[u]);
                     cq[u]           = KEEP ? *cp : __builtin_nontemporal_load(cp);
-                    vq[u].template load<KEEP>(val + q);
+                    vq[u].template load<KEEP>(val + qq[u]);
                 }
+                // Every quad is stored unconditionally: a lane whose second
+                // quad lies past the window's end stores the window's last
+                // quad again, the same values its owner stores.  A stored
+                // value used only under a condition lets the compiler sink
+                // its load into that branch, after the first quad's wait,
+                // which makes two HBM round trips of one.
 #pragma unroll
                 for(int u = 0; u < 2; u++)
                 {
-                    const int q = q0 + 4 * SPMV_THREADS * u;
-                    if(q < vend)
-                    {
-                        *reinterpret_cast<int4v*>(s_c + (q - ws)) = cq[u];
-                        vq[u].store(s_v + (q - ws));
-                    }
+                    *reinterpret_cast<int4v*>(s_c + (qq[u] - ws)) = cq[u];
+                    vq[u].store(s_v + (qq[u] - ws));
                 }
             }
             // the arrays' last nnz % 4 entries
@@ -294,7 +307,23 @@ namespace
                 s_c[k - ws] = ci[k];
                 s_v[k - ws] = val[k];
             }
+            if(!bounds)
+            {
+                if(tid > 0 && tid < nrows)
+                    s_rp[tid] = rpv;
+                if(tid == 0)
+                {
+                    s_rp[0]     = base;
+                    s_rp[nrows] = end;
+                }
+            }
             __syncthreads();
+            if(!bounds)
+            {
+                rs     = active ? s_rp[lrow] : 0;
+                re     = active ? s_rp[lrow + 1] : 0;
+                bounds = true;
+            }
             if(active)
             {
                 const int a = max(rs, ws) - ws + lig;
@@ -324,9 +353,11 @@ namespace
                         const T* __restrict__ d, const T* __restrict__ b, T alpha, T beta,
                         T* __restrict__ y, T* __restrict__ r)
     {
+        // the block's row starts, then win products
         extern __shared__ __attribute__((aligned(16))) unsigned char s_raw[];
-        T*        s_p = reinterpret_cast<T*>(s_raw);
-        const int WIN = win;
+        int*      s_rp = reinterpret_cast<int*>(s_raw);
+        T*        s_p  = reinterpret_cast<T*>(s_raw + RP_LDS_BYTES);
+        const int WIN  = win;
 
         const int   tid  = threadIdx.x;
         const int   tile = lal::xcd_tile(blockIdx.x, nb);
@@ -340,32 +371,66 @@ namespace
         const int  row    = r0 + lrow;
         const int  base   = mt.z;
         const int  end    = mt.w;
-        const int  rs     = active ? rp[row] : 0;
-        const int  re     = active ? rp[row + 1] : 0;
         const int  nnz4   = nnz & ~3;
+        // row bounds through LDS, loaded with the first window (quad_kernel)
+        const int rpv = (tid > 0 && tid < nrows) ? rp[r0 + tid] : 0;
+        int       rs = 0, re = 0;
+        bool      bounds = false;
 
         T acc = 0;
         for(int ws = base & ~3; ws < end; ws += WIN)
         {
             const int we   = min(ws + WIN, end);
             const int vend = min((we + 3) & ~3, nnz4);
-            for(int q = ws + 4 * tid; q < vend; q += 4 * SPMV_THREADS)
+            for(int q0 = ws + 4 * tid; q0 < vend; q0 += 8 * SPMV_THREADS)
             {
-                const int4v cq = __builtin_nontemporal_load(reinterpret_cast<const int4v*>(ci + q));
-                quad<T>     vq;
-                vq.load(val + q);
-                T v4[4], x4[4];
-                vq.store(v4);
+                // two quads per lane, both loaded before either's gathers (a
+                // second quad past the window's end repeats its last quad and
+                // stores the same products as that quad's owner)
+                int4v   cq[2];
+                quad<T> vq[2];
+                int     qq[2];
 #pragma unroll
-                for(int e = 0; e < 4; e++)
-                    x4[e] = gather<T, MODE>(x, d, b, cq[e]);
+                for(int u = 0; u < 2; u++)
+                {
+                    qq[u] = min(q0 + 4 * SPMV_THREADS * u, vend - 4);
+                    cq[u] = __builtin_nontemporal_load(reinterpret_cast<const int4v*>(ci + qq[u]));
+                    vq[u].load(val + qq[u]);
+                }
+                T v8[8], x8[8];
 #pragma unroll
-                for(int e = 0; e < 4; e++)
-                    s_p[q - ws + e] = v4[e] * x4[e];
+                for(int u = 0; u < 2; u++)
+                    vq[u].store(v8 + 4 * u);
+#pragma unroll
+                for(int u = 0; u < 2; u++)
+#pragma unroll
+                    for(int e = 0; e < 4; e++)
+                        x8[4 * u + e] = gather<T, MODE>(x, d, b, cq[u][e]);
+#pragma unroll
+                for(int u = 0; u < 2; u++)
+#pragma unroll
+                    for(int e = 0; e < 4; e++)
+                        s_p[qq[u] - ws + e] = v8[4 * u + e] * x8[4 * u + e];
             }
             for(int k = max(vend, ws) + tid; k < we; k += SPMV_THREADS)
                 s_p[k - ws] = val[k] * gather<T, MODE>(x, d, b, ci[k]);
+            if(!bounds)
+            {
+                if(tid > 0 && tid < nrows)
+                    s_rp[tid] = rpv;
+                if(tid == 0)
+                {
+                    s_rp[0]     = base;
+                    s_rp[nrows] = end;
+                }
+            }
             __syncthreads();
+            if(!bounds)
+            {
+                rs     = active ? s_rp[lrow] : 0;
+                re     = active ? s_rp[lrow + 1] : 0;
+                bounds = true;
+            }
             if(active)
             {
                 const int a = max(rs, ws) - ws + lig;
@@ -394,17 +459,18 @@ namespace
         // blocks leave room for more resident workgroups
         const int win = a.win;
         if(aligned && a.blocks && a.by_nnz)
-            quad_nnz_kernel<T, MODE><<<nb, SPMV_THREADS, sizeof(T) * win, c->stream>>>(
+            quad_nnz_kernel<T, MODE><<<nb, SPMV_THREADS, RP_LDS_BYTES + sizeof(T) * win,
+                                      c->stream>>>(
                 a.m, a.nnz, a.blocks, nb, win, a.rp, a.ci, (const T*)a.val, (const T*)a.x,
                 (const T*)a.d, (const T*)a.b, (T)a.alpha, (T)a.beta, (T*)a.y, (T*)a.r);
         else if(aligned && a.blocks && a.keep &&
                 (sizeof(T) + sizeof(int)) * (size_t)a.nnz <= lal::CACHE_KEEP_BYTES)
-            quad_kernel<T, MODE, true><<<nb, SPMV_THREADS, (sizeof(int) + sizeof(T)) * win,
+            quad_kernel<T, MODE, true><<<nb, SPMV_THREADS, RP_LDS_BYTES + (sizeof(int) + sizeof(T)) * win,
                                          c->stream>>>(
                 a.m, a.nnz, a.blocks, nb, win, a.rp, a.ci, (const T*)a.val, (const T*)a.x,
                 (const T*)a.d, (const T*)a.b, (T)a.alpha, (T)a.beta, (T*)a.y, (T*)a.r);
         else if(aligned && a.blocks)
-            quad_kernel<T, MODE, false><<<nb, SPMV_THREADS, (sizeof(int) + sizeof(T)) * win,
+            quad_kernel<T, MODE, false><<<nb, SPMV_THREADS, RP_LDS_BYTES + (sizeof(int) + sizeof(T)) * win,
                                           c->stream>>>(
                 a.m, a.nnz, a.blocks, nb, win, a.rp, a.ci, (const T*)a.val, (const T*)a.x,
                 (const T*)a.d, (const T*)a.b, (T)a.alpha, (T)a.beta, (T*)a.y, (T*)a.r);

@@ -236,10 +236,13 @@ namespace
     }
 
     // ---------------------------------------------------- PCG vector updates
+    // OP 0: sum; OP 1: maximum that keeps a NaN from either side, so that a
+    // NaN anywhere in a vector reaches the solver's norm (and stops it)
+    // whatever the order of the combination.
     template <int OP>
     __device__ __forceinline__ double combine(double a, double b)
     {
-        return OP == 0 ? a + b : ((b < a) ? a : b);
+        return OP == 0 ? a + b : ((a != a || b < a) ? a : b);
     }
 
     // Block total of one value per lane (all lanes receive it).
@@ -247,7 +250,7 @@ namespace
     __device__ double block_total(double v)
     {
         __shared__ double sh[SPMV_THREADS / 64];
-        v = OP == 0 ? lal::wave_sum(v) : lal::wave_max(v);
+        v = OP == 0 ? lal::wave_sum(v) : lal::wave_max_nan(v);
         __syncthreads();
         if((threadIdx.x & 63) == 0)
             sh[threadIdx.x >> 6] = v;
@@ -473,7 +476,7 @@ namespace lal
         int nb = spmv_grid(d, m);
         jacobi_kernel<T, 0><<<nb, SPMV_THREADS, 0, c->stream>>>(
             d ? d->d_blocks : nullptr, nb, m, rp, ci, v, xold, nullptr, b, xnew, nullptr,
-            d ? d->exact : 0);
+            1); // no fast descriptor: one lane per row
         LAL_LAUNCHED();
         (void)diag;
         return LAL_OK;
@@ -495,7 +498,7 @@ namespace lal
         int nb = spmv_grid(d, m);
         jacobi_kernel<T, 1><<<nb, SPMV_THREADS, 0, c->stream>>>(
             d ? d->d_blocks : nullptr, nb, m, rp, ci, v, nullptr, diag, b, x, r,
-            d ? d->exact : 0);
+            1); // no fast descriptor: one lane per row
         LAL_LAUNCHED();
         return LAL_OK;
     }

@@ -19,6 +19,7 @@
 #include "iterative_solvers/amg/amg_strength.h"
 #include "lal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -466,7 +467,11 @@ int main()
         const double e = rel_diff(host_copy(xd), host_copy(xh));
         std::printf("     Jacobi-BiCGSTAB iterations host %d, device %d, rel. diff %.2e\n", ith,
                     itd, e);
-        expect(std::abs(ith - itd) <= 2 && e <= 1e-8,
+        // the device inner products are fixed-order trees, not the host's
+        // sequential sums, so BiCGSTAB's iterates part in the last bits: the
+        // iteration count within max(3, 10 %), as for the library's own
+        // BiCGSTAB against the host restatement
+        expect(std::abs(ith - itd) <= std::max(3, ith / 10) && e <= 1e-8,
                "bicgstab_solver + jacobi_precond on the device");
     }
 

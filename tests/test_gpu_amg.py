@@ -192,10 +192,11 @@ def test_spgemm_refuses_inconsistent_row_pointer(h, oracle):
         gci = empty(nnz.value + 64, torch.int32)
         guard.fill_(7.0)
         gci.fill_(-7)
+        bad_rp = dev(brp, np.int32)  # kept alive until the kernels have read it
         with pytest.raises(lal.LalError):
             lal.call("lal_csrgemm_solve_f64", h.h, A.m, A.n, B.n, A.nnz, lal.ptr(da.row_ptr),
                      lal.ptr(da.col_ind), lal.ptr(da.val), B.nnz, lal.ptr(db.row_ptr),
-                     lal.ptr(db.col_ind), lal.ptr(db.val), lal.ptr(dev(brp, np.int32)),
+                     lal.ptr(db.col_ind), lal.ptr(db.val), lal.ptr(bad_rp),
                      lal.ptr(gci), lal.ptr(guard))
         assert np.all(host(guard)[nnz.value:] == 7.0) and np.all(host(gci)[nnz.value:] == -7)
 

@@ -106,10 +106,10 @@ def test_residual_f32_and_transpose_f32(h):
     x = rng.uniform(-1, 1, m).astype(np.float32)
     b = rng.uniform(-1, 1, m).astype(np.float32)
     R = empty(m, torch.float32)
+    X, Bv = dev(x), dev(b)  # kept alive until the kernels have read them
     for dd in (desc, None):
         lal.call("lal_residual_f32", h.h, dd.d if dd else None, m, m, A.nnz, lal.ptr(d32.row_ptr),
-                 lal.ptr(d32.col_ind), lal.ptr(d32.val), lal.ptr(dev(x)), lal.ptr(dev(b)),
-                 lal.ptr(R))
+                 lal.ptr(d32.col_ind), lal.ptr(d32.val), lal.ptr(X), lal.ptr(Bv), lal.ptr(R))
         r = host(R).astype(np.float64)
         v32 = v.astype(np.float32).astype(np.float64)
         ax = np.zeros(m)

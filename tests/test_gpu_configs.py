@@ -11,7 +11,11 @@ For each: the exact-mode device hierarchy equals the oracle's level for level
 to 1e-10 takes the oracle's iteration count within one and lands within 1e-10
 relative of the oracle's solution; the reliable-update mixed solve meets the
 tolerance on the true FP64 residual and lands within 1e-8 of the oracle's
-solution.
+solution.  On the diffusion operator the scheme that workload names -- an FP32
+SA-AMG-PCG inner solve to 1e-4 inside FP64 iterative refinement -- meets the
+same bars; on the kNN Laplacian the gradient with respect to the edge weights
+from the mixed (reliable-update) differentiable solve agrees with the oracle's
+closed form dL/dw_e = -(lambda_p - lambda_q)(x_p - x_q) to 1e-6 relative.
 """
 import hashlib
 import os
@@ -43,12 +47,14 @@ def diffusion160():
     return CSR(rp, ci, v, m, m), b
 
 
-def knn2m():
+def knn2m(full=False):
     rng = np.random.default_rng(2024)
     rp, ci, m, E, entry, edge, sign, dpos = pb.knn_laplacian(2_000_000, seed=5)
     w = rng.uniform(0.5, 2.0, len(E))
     v = pb.assemble_knn(len(ci), entry, edge, sign, dpos, w)
     b = rng.standard_normal(m)
+    if full:
+        return CSR(rp, ci, v, m, m), b, (E, entry, edge, sign, dpos, w)
     return CSR(rp, ci, v, m, m), b
 
 
@@ -109,4 +115,53 @@ def test_workload_hierarchy_and_solves(h, oracle_mt, workload):
     assert info.converged, info.iterations
     assert np.abs(rm).max() <= 1e-10 * np.abs(b).max() * 1.01
     assert np.linalg.norm(xm - xo) <= 1e-8 * np.linalg.norm(xo)
+    if workload == "diffusion160":
+        # the workload's own scheme: FP32 inner PCG to 1e-4, FP64 refinement
+        x.zero_()
+        info = H.ir(B, x, rel_tol=1e-10, abs_tol=1e-300, inner_rel_tol=1e-4)
+        xi = host(x)
+        ri = oracle_mt.residual(A, xi, b)
+        log(f"{workload}: IR {info.outer_iterations} refinements, {info.iterations} inner it, "
+            f"rel. residual {np.abs(ri).max() / np.abs(b).max():.3e}, "
+            f"rel. diff {np.linalg.norm(xi - xo) / np.linalg.norm(xo):.3e}")
+        assert info.converged, (info.iterations, info.outer_iterations)
+        assert np.abs(ri).max() <= 1e-10 * np.abs(b).max() * 1.01
+        assert np.linalg.norm(xi - xo) <= 1e-8 * np.linalg.norm(xo)
     H.close()
+
+
+def test_knn2m_edge_weight_gradient(oracle_mt):
+    """dL/dw at 2M points: the mixed (FP32 PCG, FP64 reliable updates)
+    forward and adjoint solves to 1e-12 and the SDDMM, through torch's
+    autograd of the assembly, against the oracle's FP64 closed form."""
+    from linalglib_amd.torch_solve import saamg_solve
+
+    log("knn2m gradient: build")
+    A, b, (E, entry, edge, sign, dpos, w) = knn2m(full=True)
+    m = A.m
+    xt = np.random.default_rng(77).standard_normal(m)
+    log("knn2m gradient: oracle setup and the two FP64 solves")
+    ho = oracle_mt.saamg_setup(A)
+    xo, _, _ = ho.pcg(b, rel_tol=1e-13, coarse_solver=1)
+    lam, _, _ = ho.pcg(xo - xt, rel_tol=1e-13, coarse_solver=1)
+    del ho
+    p, q = E[:, 0], E[:, 1]
+    dwo = -(lam[p] - lam[q]) * (xo[p] - xo[q])
+
+    log("knn2m gradient: device forward + backward (reliable updates)")
+    cuda = torch.device("cuda:0")
+    W = torch.tensor(w, device=cuda, requires_grad=True)
+    Ent, Edg, S, D = (torch.from_numpy(a).to(cuda) for a in (entry, edge, sign, dpos))
+    V = torch.zeros(A.nnz, dtype=torch.float64, device=cuda).index_add(0, Ent, S * W[Edg])
+    V = V.index_add(0, D, torch.full((m,), 1e-2, dtype=torch.float64, device=cuda))
+    x = saamg_solve(V, dev(b), dev(A.row_ptr, np.int32), dev(A.col_ind, np.int32),
+                    mixed="reliable", tol=1e-12)
+    L = 0.5 * ((x - dev(xt)) ** 2).sum()
+    L.backward()
+    dw = host(W.grad)
+    xg = host(x)
+    ex = np.linalg.norm(xg - xo) / np.linalg.norm(xo)
+    eg = np.linalg.norm(dw - dwo) / np.linalg.norm(dwo)
+    log(f"knn2m gradient: x rel. diff {ex:.3e}, dL/dw rel. diff {eg:.3e}")
+    assert ex <= 1e-9
+    assert eg <= 1e-6

@@ -97,7 +97,7 @@ def test_amg_solve_argument_checks(h):
     H = _hier(h, CSR(rp, ci, v, m, m))
     B = dev(np.ones(m))
     x = torch.zeros(m, dtype=torch.float64, device="cuda")
-    for cyc, sm in ((3, 0), (0, 5), (-1, 0)):
+    for cyc, sm in ((3, 0), (0, 7), (-1, 0), (0, -1)):
         with pytest.raises(lal.LalError):
             lal.call("lal_amg_solve", h.h, H.hp, lal.ptr(B), lal.ptr(x), 1, 1, cyc, sm, None,
                      None)

@@ -280,3 +280,71 @@ def test_adjoint_gradient_96(oracle_big, mixed):
     assert np.linalg.norm(x.detach().cpu().numpy() - xo) <= 1e-9 * np.linalg.norm(xo)
     assert np.linalg.norm(db - lam) <= 1e-6 * np.linalg.norm(lam)
     assert np.linalg.norm(dk - dko) <= 1e-6 * np.linalg.norm(dko)
+
+
+def test_adjoint_gradient_96_central_differences():
+    """The differentiable-solve workload's own check: dL/dk from the mixed
+    backward pass (FP64 outer refinement, FP32 inner SA-AMG-PCG, then the
+    SDDMM) against central differences with h = 1e-6 on 64 seeded faces of the
+    96^3 operator.
+
+    Each difference comes from two FP64 device solves to 1e-13 of A(k +- h e_f),
+    both preconditioned by the V-cycle of the hierarchy built for A(k) (PCG on
+    the perturbed operator, lal_pcg_solve_op_f64), so that the two solves
+    share their preconditioner and differ only through the perturbation.  The
+    loss difference is formed as 0.5 (x+ - x-) . (x+ + x- - 2 x_t), which is
+    L(k+h) - L(k-h) without subtracting two losses of size ~1e5.  What is left
+    is the FP64 rounding noise of the two solves, which the 2h = 2e-6 step
+    divides: about 1e-4 of max|g| on this operator, so the bar is 2e-3
+    relative to max(|g_f|, 1e-3 max|g|)."""
+    from linalglib_amd.torch_solve import saamg_solve
+
+    nx = 96
+    rp, ci, m, nf, entry, face, sign = pb.diffusion_faces(nx)
+    rng = np.random.default_rng(4096)
+    k = np.exp(rng.standard_normal(nf))
+    b = rng.standard_normal(m)
+    xt = rng.standard_normal(m)
+    faces = rng.choice(nf, size=64, replace=False)
+    hstep = 1e-6
+
+    d = torch.device("cuda:0")
+    RP, CI = torch.from_numpy(rp).to(d), torch.from_numpy(ci).to(d)
+    E, F, S = (torch.from_numpy(a).to(d) for a in (entry, face, sign))
+    Bd, XT = torch.from_numpy(b).to(d), torch.from_numpy(xt).to(d)
+
+    log("96^3 gradient (mixed ir) for the central-difference check")
+    K = torch.tensor(k, device=d, requires_grad=True)
+    V = torch.zeros(len(ci), dtype=torch.float64, device=d).index_add(0, E, S * K[F])
+    x = saamg_solve(V, Bd, RP, CI, mixed="ir", tol=1e-12)
+    (0.5 * ((x - XT) ** 2).sum()).backward()
+    g = K.grad.cpu().numpy()
+
+    h = lal.Handle()
+    V0 = V.detach().clone()
+    H = lal.Hierarchy(h, m, len(ci), RP, CI, V0)
+
+    def solve(kk):
+        Kd = torch.from_numpy(kk).to(d)
+        Vd = torch.zeros(len(ci), dtype=torch.float64, device=d).index_add(0, E, S * Kd[F])
+        xs = torch.zeros(m, dtype=torch.float64, device=d)
+        info = H.pcg_op((RP, CI, Vd, m, len(ci), None), Bd, xs, rel_tol=1e-13, abs_tol=1e-300,
+                        max_iter=1000)
+        assert info.converged
+        return xs
+
+    log("96^3 central differences on 64 faces (128 FP64 device solves)")
+    fd = np.empty(64)
+    for j, f in enumerate(faces):
+        kp, km = k.copy(), k.copy()
+        kp[f] += hstep
+        km[f] -= hstep
+        xp, xm = solve(kp), solve(km)
+        dL = 0.5 * torch.dot(xp - xm, xp + xm - 2 * XT).item()
+        fd[j] = dL / (2 * hstep)
+    H.close()
+    gf = g[faces]
+    scale = np.maximum(np.abs(gf), 1e-3 * np.abs(g).max())
+    err = np.abs(fd - gf) / scale
+    log(f"96^3 FD: max rel. err {err.max():.3e}, median {np.median(err):.3e}")
+    assert err.max() <= 2e-3, (err.max(), faces[np.argmax(err)])
