@@ -98,13 +98,7 @@ namespace
     // tile flags (low 3 bits of the fourth word; the group's end row above)
     constexpr int GS_START = 1, GS_END_GROUP = 2, GS_END_SET = 4;
 
-    // Barrier that orders LDS accesses only: global loads stay in flight.
-    __device__ __forceinline__ void lds_barrier()
-    {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-        __builtin_amdgcn_s_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-    }
+    using lal::lds_barrier;
 
     __device__ __forceinline__ void gs_tile_loads(int c0, int c1, int tid,
                                                   const int* __restrict__ pci,
@@ -496,7 +490,7 @@ namespace
             LAL_TRY(smooth(h, l, x, b, smoother));
         LAL_TRY(lal::residual_f64(c, A.d, A.m, A.n, A.nnz, A.rp, A.ci, A.v, x, b, W.r));
         LAL_TRY(lal::csrmv(c, R.d, R.m, R.n, R.nnz, 1.0, R.rp, R.ci, R.v, W.r, 0.0, Wc.b));
-        LAL_HIP(hipMemsetAsync(Wc.x, 0, sizeof(double) * R.m, c->stream));
+        LAL_TRY(lal_fill_f64(c, Wc.x, (size_t)R.m, 0.0)); // a kernel: the cycle may be captured
         LAL_TRY(cycle_on(h, l + 1, Wc.x, Wc.b, n1, n2, n3, cycle, smoother));
         LAL_TRY(lal::csrmv(c, P.d, P.m, P.n, P.nnz, 1.0, P.rp, P.ci, P.v, Wc.x, 1.0, x));
         for(int i = 0; i < n2; i++)
@@ -527,13 +521,87 @@ namespace
     }
 }
 
+namespace
+{
+    // One cycle on level 0, x in place.  The multicolour smoothers (5, 6)
+    // launch one kernel per colour and level, several hundred per cycle at
+    // 128^3, each short enough that the host's launch rate bounds the cycle:
+    // their cycles are captured once into a HIP graph and replayed.  A
+    // (x, b, sweeps, cycle, smoother) combination runs eagerly the first
+    // time -- which also builds every level's colouring, whose host copies
+    // may not happen under capture -- and is captured on its second use and
+    // replayed from then on.  A capture that fails leaves the combination
+    // eager.  The graph holds kernels only (the cycle zeroes with a fill
+    // kernel, not hipMemsetAsync) and is replayed on the caller's stream.
+    int run_cycle(lal_hierarchy_t* h, double* x, const double* b, int n1, int n2, int n3,
+                  int cycle, int smoother)
+    {
+        lal_context* c = h->c;
+        if(smoother < 5)
+            return cycle_on(h, 0, x, b, n1, n2, n3, cycle, smoother);
+        lal_hierarchy_t::cycle_graph* g = nullptr;
+        for(auto& e : h->cycle_graphs)
+            if(e.x == x && e.b == b && e.n1 == n1 && e.n2 == n2 && e.n3 == n3 &&
+               e.cycle == cycle && e.smoother == smoother)
+                g = &e;
+        if(g == nullptr)
+        {
+            LAL_TRY(cycle_on(h, 0, x, b, n1, n2, n3, cycle, smoother));
+            h->cycle_graphs.push_back({x, b, n1, n2, n3, cycle, smoother, false, nullptr});
+            return LAL_OK;
+        }
+        if(g->exec)
+        {
+            LAL_HIP(hipGraphLaunch(g->exec, c->stream));
+            return LAL_OK;
+        }
+        if(g->failed)
+            return cycle_on(h, 0, x, b, n1, n2, n3, cycle, smoother);
+        // Captured on the hierarchy's own stream (the caller's may be the
+        // null stream, which cannot be captured): the launches of cycle_on go
+        // to c->stream, so it points at the capture stream meanwhile.
+        if(h->capture_stream == nullptr &&
+           hipStreamCreateWithFlags(&h->capture_stream, hipStreamNonBlocking) != hipSuccess)
+        {
+            (void)hipGetLastError();
+            h->capture_stream = nullptr;
+            g->failed         = true;
+            return cycle_on(h, 0, x, b, n1, n2, n3, cycle, smoother);
+        }
+        hipGraph_t        graph = nullptr;
+        hipGraphExec_t    exec  = nullptr;
+        const hipStream_t user  = c->stream;
+        if(hipStreamBeginCapture(h->capture_stream, hipStreamCaptureModeThreadLocal) ==
+           hipSuccess)
+        {
+            c->stream          = h->capture_stream;
+            const int st       = cycle_on(h, 0, x, b, n1, n2, n3, cycle, smoother);
+            c->stream          = user;
+            const hipError_t e = hipStreamEndCapture(h->capture_stream, &graph);
+            if(st == LAL_OK && e == hipSuccess && graph != nullptr &&
+               hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess)
+                g->exec = exec;
+            if(graph)
+                (void)hipGraphDestroy(graph);
+        }
+        (void)hipGetLastError();
+        if(g->exec == nullptr)
+        {
+            g->failed = true;
+            return cycle_on(h, 0, x, b, n1, n2, n3, cycle, smoother);
+        }
+        LAL_HIP(hipGraphLaunch(g->exec, c->stream));
+        return LAL_OK;
+    }
+}
+
 namespace lal
 {
     int vcycle_smoothed(lal_hierarchy_t* h, const double* r, double* z, int n1, int n2,
                         int smoother)
     {
-        LAL_HIP(hipMemsetAsync(z, 0, sizeof(double) * h->L[0].A.m, h->c->stream));
-        return cycle_on(h, 0, z, r, n1, n2, n2, 0, smoother);
+        LAL_TRY(lal_fill_f64(h->c, z, (size_t)h->L[0].A.m, 0.0));
+        return run_cycle(h, z, r, n1, n2, n2, 0, smoother);
     }
 }
 
@@ -556,7 +624,7 @@ extern "C" int lal_amg_solve(lal_handle c, lal_hierarchy h, const double* b, dou
     bool conv  = false;
     while(!(count > o.max_iter))
     {
-        LAL_TRY(cycle_on(h, 0, x, b, n1, n2, n2, cycle, smoother));
+        LAL_TRY(run_cycle(h, x, b, n1, n2, n2, cycle, smoother));
         LAL_TRY(residual_norm(h, x, b, lal::S_RNORM, &rn));
         if(lal::converges(rn, init, o.rel_tol, o.abs_tol))
         {

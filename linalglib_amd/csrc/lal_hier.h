@@ -82,6 +82,12 @@ namespace lal
         int*             pci   = nullptr;
         double*          pv    = nullptr;
         double*          pdinv = nullptr;
+        int*             plan    = nullptr; // one-workgroup sweep: 8 ints per chunk
+        std::vector<int> plan_h;            // the same plan on the host
+        int              nchunks = 0;       // chunks of one sweep direction
+        int              nnz     = 0;       // entries of the copy
+        int              wg_nt   = 0;       // lanes of the one-workgroup sweep
+        int              wg_lg   = 0;       // log2 of its lanes per row
     };
 
     struct level_t
@@ -156,9 +162,26 @@ struct lal_hierarchy_t
     int*           tail_img32 = nullptr;
     lal::tail_desc tail64{};
     lal::tail_desc tail32{};
+    // captured cycles of the multicolour smoothers (lal_cycles.hip), one per
+    // (x, b, sweeps, cycle, smoother) combination
+    struct cycle_graph
+    {
+        const double*  x;
+        const double*  b;
+        int            n1, n2, n3, cycle, smoother;
+        bool           failed;
+        hipGraphExec_t exec;
+    };
+    std::vector<cycle_graph> cycle_graphs;
+    hipStream_t              capture_stream = nullptr; // where they are captured
 
     ~lal_hierarchy_t()
     {
+        for(auto& g : cycle_graphs)
+            if(g.exec)
+                (void)hipGraphExecDestroy(g.exec);
+        if(capture_stream)
+            (void)hipStreamDestroy(capture_stream);
         for(int l = 0; l <= 20; l++)
         {
             L[l].A.free_all();
@@ -175,7 +198,8 @@ struct lal_hierarchy_t
                                  (void*)g->pv, (void*)g->pdiag, (void*)g->tiles})
                     (void)hipFree(ptr);
             for(void* ptr : {(void*)L[l].mc.off_d, (void*)L[l].mc.perm, (void*)L[l].mc.prp,
-                             (void*)L[l].mc.pci, (void*)L[l].mc.pv, (void*)L[l].mc.pdinv})
+                             (void*)L[l].mc.pci, (void*)L[l].mc.pv, (void*)L[l].mc.pdinv,
+                             (void*)L[l].mc.plan})
                 (void)hipFree(ptr);
         }
         for(void* ptr : {(void*)coarse_inv, (void*)res, (void*)z, (void*)p, (void*)q,

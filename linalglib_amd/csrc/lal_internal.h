@@ -125,6 +125,15 @@ namespace lal
         return v;
     }
 
+    // Workgroup barrier that orders LDS accesses only: global loads issued
+    // before it stay in flight (__syncthreads() waits for them).
+    __device__ __forceinline__ void lds_barrier()
+    {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    }
+
     // Wavefront maximum that returns NaN when any lane holds one.
     template <typename T>
     __device__ __forceinline__ T wave_max_nan(T v)

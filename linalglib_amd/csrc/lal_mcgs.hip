@@ -21,8 +21,8 @@
 // rows grouped by colour (off-diagonal entries only, and 1/a_ii per row), so
 // a colour's rows are one contiguous stream.  Colours of many rows are one
 // launch each, with 1 to 64 lanes per row by the level's mean row length;
-// levels of at most MC_WG_ROWS rows sweep every colour in one workgroup with
-// x in LDS and a barrier between colours.
+// levels small enough to keep x and b in one workgroup's LDS sweep every
+// colour in one launch of one workgroup (mc_wg_kernel).
 #include "lal_hier.h"
 
 #include <vector>
@@ -32,9 +32,67 @@ namespace
     using lal::csr_dev;
     using lal::mc_sched;
 
-    constexpr int MC_WG_ROWS = 8192; // x of a one-workgroup level in LDS (64 KiB)
+    constexpr int MC_WG_U       = 4;          // entries per lane and chunk
+    constexpr int MC_WG_PAD     = 64;         // doubles past the product buffers
+    constexpr int MC_WG_PF      = 3;          // chunks whose loads are issued ahead, plus one
+    constexpr int MC_WG_LDS_MAX = 160 * 1024; // LDS of one workgroup on gfx950
+    constexpr int MC_WG_MAX_NNZ = 262144;     // entries of a one-workgroup level
 
-    // Rows t in [0, cnt) of one colour, LANES lanes per row.
+    typedef int int4v __attribute__((ext_vector_type(4)));
+
+    // v + the value of v in lane SRC of the same quad, half row or row (DPP
+    // control CTRL), for a double.
+    template <int CTRL>
+    __device__ __forceinline__ double dpp_add(double v)
+    {
+        const long long b  = __builtin_bit_cast(long long, v);
+        const int       lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+        const int       hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+        return v + __builtin_bit_cast(double, (long long)(unsigned)lo | ((long long)hi << 32));
+    }
+
+    // Sum over aligned groups of 2^lg <= 16 lanes, left in every lane of the
+    // group: quad permutations (lane ^ 1, lane ^ 2), then the half-row and
+    // row mirrors, each adding the other half's partial sum.
+    __device__ __forceinline__ double group_sum16(double v, int lg)
+    {
+        if(lg >= 1)
+            v = dpp_add<0xB1>(v); // quad_perm [1, 0, 3, 2]
+        if(lg >= 2)
+            v = dpp_add<0x4E>(v); // quad_perm [2, 3, 0, 1]
+        if(lg >= 3)
+            v = dpp_add<0x141>(v); // row_half_mirror
+        if(lg >= 4)
+            v = dpp_add<0x140>(v); // row_mirror
+        return v;
+    }
+
+    // Sum over aligned groups of LANES (a power of two <= 64) lanes, left in
+    // every lane of the group: group_sum16's steps, then exchanges with the
+    // lanes 16 and 32 away.
+    template <int LANES>
+    __device__ __forceinline__ double lane_sum(double v)
+    {
+        if(LANES >= 2)
+            v = dpp_add<0xB1>(v);
+        if(LANES >= 4)
+            v = dpp_add<0x4E>(v);
+        if(LANES >= 8)
+            v = dpp_add<0x141>(v);
+        if(LANES >= 16)
+            v = dpp_add<0x140>(v);
+        if(LANES >= 32)
+            v += __shfl_xor(v, 16, 64);
+        if(LANES >= 64)
+            v += __shfl_xor(v, 32, 64);
+        return v;
+    }
+
+    // Rows t in [0, cnt) of one colour, LANES lanes per row.  A row's entries
+    // are taken 4 * LANES at a time, four per lane, all loads of a round
+    // issued before the first is used (an index past the row reads the row's
+    // last entry and contributes zero), so a row of up to 4 * LANES entries
+    // costs one round of dependent loads.
     template <int LANES>
     __global__ void __launch_bounds__(256)
         mc_colour_kernel(int cnt, const int* __restrict__ perm, const int* __restrict__ prp,
@@ -48,98 +106,232 @@ namespace
         if(t >= cnt) // whole lane groups leave together (LANES divides 64)
             return;
         const int rs = prp[t], re = prp[t + 1];
-        double    s  = 0.0;
-        int       k  = rs + lig;
-        for(; k + 3 * LANES < re; k += 4 * LANES)
+        // the row's position, b_i and 1/a_ii do not depend on the sum: load
+        // them now, beside the row bounds, rather than after the reduction
+        const int    i  = perm[t];
+        const double di = pdinv[t];
+        const double bi = b[i];
+        double       s  = 0.0;
+        for(int k0 = rs + lig; k0 - lig < re; k0 += 4 * LANES) // re > rs inside
         {
             int    c[4];
             double v[4], xx[4];
 #pragma unroll
             for(int u = 0; u < 4; u++)
             {
-                c[u] = pci[k + u * LANES];
-                v[u] = pv[k + u * LANES];
+                const int k = min(k0 + u * LANES, re - 1);
+                c[u]        = pci[k];
+                v[u]        = pv[k];
             }
 #pragma unroll
             for(int u = 0; u < 4; u++)
                 xx[u] = x[c[u]];
 #pragma unroll
             for(int u = 0; u < 4; u++)
-                s += v[u] * xx[u];
+                s += k0 + u * LANES < re ? v[u] * xx[u] : 0.0;
         }
-        for(; k < re; k += LANES)
-            s += pv[k] * x[pci[k]];
-#pragma unroll
-        for(int o = LANES >> 1; o > 0; o >>= 1)
-            s += __shfl_xor(s, o, 64);
+        s = lane_sum<LANES>(s);
         if(lig == 0)
-        {
-            const int i = perm[t];
-            x[i]        = (b[i] - s) * pdinv[t];
-        }
+            x[i] = (bi - s) * di;
     }
 
-    // Every colour of a small level in one workgroup of 1024 lanes, x in LDS
-    // (m <= MC_WG_ROWS); forward colour order, then backward when `sym`.
-    __global__ void __launch_bounds__(1024)
-        mc_wg_kernel(int m, int ncol, const int* __restrict__ off, const int* __restrict__ perm,
-                     const int* __restrict__ prp, const int* __restrict__ pci,
-                     const double* __restrict__ pv, const double* __restrict__ pdinv,
-                     const double* __restrict__ b, double* x, int lanes, int sym)
+    // Buffer resource over `bytes` bytes at p (gfx9-family raw buffer): a
+    // load at a byte offset past the end returns 0 and touches no memory.
+    __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, int bytes)
     {
-        extern __shared__ double s_x[];
-        const int tid = threadIdx.x;
-        for(int i = tid; i < m; i += 1024)
+        return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, bytes, 0x00020000);
+    }
+
+    __device__ __forceinline__ int buf_i32(__amdgpu_buffer_rsrc_t r, int i)
+    {
+        return __builtin_amdgcn_raw_buffer_load_b32(r, i * 4, 0, 0);
+    }
+
+    __device__ __forceinline__ double buf_f64(__amdgpu_buffer_rsrc_t r, int i)
+    {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, i * 8, 0, 0));
+    }
+
+    // Every colour of a small level in one workgroup of NT lanes, with x, b
+    // and the chunk plan in LDS (mc_wg_lds_bytes).  The host cuts each
+    // colour's rows, in order, into chunks of whole rows with at most
+    // NT * MC_WG_U entries and at most NT / G rows, G = 2^lg <= 16 lanes per
+    // row; the plan holds two int4 per chunk ({first entry, end entry, first
+    // row, end row} and {1 when the chunk ends its colour}), the forward
+    // sweep's chunks, then the backward sweep's, each padded with empty
+    // chunks to a multiple of MC_WG_PF.  Per chunk the lanes multiply MC_WG_U
+    // entries each by x read from LDS into one of two LDS product buffers,
+    // and after a barrier the G lanes of row slot k add the products of the
+    // chunk's row k (lane-strided, then group_sum16) and the first of them
+    // writes the row's new x.  Rows of one colour are uncoupled, so the next
+    // chunk of the same colour may read x while these rows are written; a new
+    // colour waits at a second barrier.  Nothing but x depends on an earlier
+    // chunk, so the loads of the next MC_WG_PF - 1 chunks (entries and the
+    // rows' bounds, positions and 1/a_ii) are in flight while a chunk is
+    // worked on; the barriers order LDS only and leave them outstanding.  The
+    // matrix is read through buffer resources, so an entry or row index past
+    // its array (a chunk's last lanes, slots past its rows) loads 0 without
+    // an address clamp; such values are never used.
+    template <int NT>
+    __global__ void __launch_bounds__(NT)
+        mc_wg_kernel(int m, int nnz, int nsteps, int lg, const int4v* __restrict__ plan,
+                     const int* __restrict__ perm, const int* __restrict__ prp,
+                     const int* __restrict__ pci, const double* __restrict__ pv,
+                     const double* __restrict__ pdinv, const double* __restrict__ b, double* x)
+    {
+        constexpr int CH = NT * MC_WG_U;
+        constexpr int PF = MC_WG_PF;
+        extern __shared__ double s_mem[];
+        double*   s_x  = s_mem;
+        double*   s_b  = s_mem + m;
+        double*   s_p  = s_mem + 2 * m; // two buffers of CH products
+        int4v*    s_pl = reinterpret_cast<int4v*>(s_p + 2 * CH + MC_WG_PAD); // the plan
+        const int tid  = threadIdx.x;
+        const int G    = 1 << lg;
+        const int S    = NT >> lg; // row slots
+        const int slot = tid >> lg;
+        const int lig  = tid & (G - 1);
+        const auto r_ci = buf_rsrc(pci, nnz * 4);
+        const auto r_v  = buf_rsrc(pv, nnz * 8);
+        const auto r_rp = buf_rsrc(prp, (m + 1) * 4);
+        const auto r_pm = buf_rsrc(perm, m * 4);
+        const auto r_dv = buf_rsrc(pdinv, m * 8);
+        for(int i = tid; i < m; i += NT)
             s_x[i] = x[i];
-        __syncthreads();
-        const int groups = 1024 / lanes;
-        const int grp    = tid / lanes;
-        const int lig    = tid % lanes;
-        for(int pass = 0; pass < 1 + sym; pass++)
-            for(int q = 0; q < ncol; q++)
+        for(int i = tid; i < m; i += NT)
+            s_b[i] = b[perm[i]];
+        for(int i = tid; i < 2 * nsteps; i += NT)
+            s_pl[i] = plan[i];
+        lal::lds_barrier();
+
+        int    C[PF][MC_WG_U], RS[PF], RE[PF], RR[PF];
+        double V[PF][MC_WG_U], DV[PF];
+
+        // chunk j's loads into register set r
+#define MC_WG_FETCH(j, r)                                                                    \
+    do                                                                                       \
+    {                                                                                        \
+        const int4v h_  = s_pl[2 * (j)];                                                     \
+        const int   c0_ = __builtin_amdgcn_readfirstlane(h_.x);                              \
+        const int   t_  = __builtin_amdgcn_readfirstlane(h_.z) + slot;                       \
+        _Pragma("unroll") for(int u = 0; u < MC_WG_U; u++)                                   \
+        {                                                                                    \
+            C[r][u] = buf_i32(r_ci, c0_ + tid + NT * u);                                     \
+            V[r][u] = buf_f64(r_v, c0_ + tid + NT * u);                                      \
+        }                                                                                    \
+        RS[r] = buf_i32(r_rp, t_);                                                           \
+        RE[r] = buf_i32(r_rp, t_ + 1);                                                       \
+        RR[r] = buf_i32(r_pm, t_);                                                           \
+        DV[r] = buf_f64(r_dv, t_);                                                           \
+    } while(0)
+
+        // chunk j from register set r: products, row sums, x of its rows
+#define MC_WG_WORK(j, r)                                                                     \
+    do                                                                                       \
+    {                                                                                        \
+        const int4v h_  = s_pl[2 * (j)];                                                     \
+        const int   c0_ = __builtin_amdgcn_readfirstlane(h_.x);                              \
+        const int   c1_ = __builtin_amdgcn_readfirstlane(h_.y);                              \
+        const int   ta_ = __builtin_amdgcn_readfirstlane(h_.z);                              \
+        const int   tb_ = __builtin_amdgcn_readfirstlane(h_.w);                              \
+        const int   fl_ = __builtin_amdgcn_readfirstlane(s_pl[2 * (j) + 1].x);               \
+        double*     bp_ = s_p + ((j) & 1) * CH;                                              \
+        double      xv_[MC_WG_U];                                                            \
+        _Pragma("unroll") for(int u = 0; u < MC_WG_U; u++) xv_[u] = s_x[C[r][u]];            \
+        _Pragma("unroll") for(int u = 0; u < MC_WG_U; u++)                                   \
+        {                                                                                    \
+            const int e_ = tid + NT * u;                                                     \
+            if(e_ < c1_ - c0_)                                                               \
+                bp_[e_] = V[r][u] * xv_[u];                                                  \
+        }                                                                                    \
+        lal::lds_barrier();                                                                  \
+        const bool ok_ = ta_ + slot < tb_;                                                   \
+        const int  n_  = ok_ ? RE[r] - RS[r] : 0;                                            \
+        const int  lo_ = RS[r] - c0_;                                                        \
+        double     a_  = 0.0;                                                                \
+        for(int k = lig; k < n_; k += 4 * G)                                                 \
+        {                                                                                    \
+            double p_[4];                                                                    \
+            _Pragma("unroll") for(int u = 0; u < 4; u++) p_[u] = bp_[lo_ + k + u * G];       \
+            _Pragma("unroll") for(int u = 0; u < 4; u++) a_ += k + u * G < n_ ? p_[u] : 0.0; \
+        }                                                                                    \
+        a_ = group_sum16(a_, lg);                                                            \
+        if(ok_ && lig == 0)                                                                  \
+            s_x[RR[r]] = (s_b[ta_ + slot] - a_) * DV[r];                                     \
+        if(fl_)                                                                              \
+            lal::lds_barrier();                                                              \
+    } while(0)
+
+        // Prologue: set 0 is complete before set 1 is issued (the empty asm
+        // takes its values and orders memory), so that on entry to the loop,
+        // as after each round of it, every set but the newest is older than
+        // the newest set's loads and the waits in the loop keep MC_WG_PF - 1
+        // chunks in flight.
+        MC_WG_FETCH(0, 0);
+        {
+            int    dep = RS[0] ^ RE[0] ^ RR[0];
+            double dd  = DV[0];
+#pragma unroll
+            for(int u = 0; u < MC_WG_U; u++)
             {
-                const int k  = pass == 0 ? q : ncol - 1 - q;
-                const int t0 = off[k], t1 = off[k + 1];
-                for(int tb = t0; tb < t1; tb += groups)
-                {
-                    const int t = tb + grp;
-                    double    s = 0.0;
-                    if(t < t1)
-                    {
-                        const int re = prp[t + 1];
-                        int       e  = prp[t] + lig;
-                        for(; e + 3 * lanes < re; e += 4 * lanes)
-                        {
-                            int    c[4];
-                            double v[4];
-#pragma unroll
-                            for(int u = 0; u < 4; u++)
-                            {
-                                c[u] = pci[e + u * lanes];
-                                v[u] = pv[e + u * lanes];
-                            }
-#pragma unroll
-                            for(int u = 0; u < 4; u++)
-                                s += v[u] * s_x[c[u]];
-                        }
-                        for(; e < re; e += lanes)
-                            s += pv[e] * s_x[pci[e]];
-                    }
-                    for(int o = lanes >> 1; o > 0; o >>= 1)
-                        s += __shfl_xor(s, o, 64);
-                    // rows of one colour are uncoupled: a row's sum reads no
-                    // x that another row of its colour writes, so the writes
-                    // of one colour need no barrier against its reads
-                    if(t < t1 && lig == 0)
-                    {
-                        const int i = perm[t];
-                        s_x[i]      = (b[i] - s) * pdinv[t];
-                    }
-                }
-                __syncthreads();
+                dep ^= C[0][u];
+                dd += V[0][u];
             }
-        for(int i = tid; i < m; i += 1024)
+            asm volatile("" ::"v"(dep), "v"(dd) : "memory");
+        }
+#pragma unroll
+        for(int r = 1; r < PF - 1; r++)
+            MC_WG_FETCH(min(r, nsteps - 1), r);
+        for(int j = 0; j < nsteps; j += PF)
+        {
+#pragma unroll
+            for(int r = 0; r < PF; r++)
+            {
+                MC_WG_FETCH(min(j + r + PF - 1, nsteps - 1), (r + PF - 1) % PF);
+                MC_WG_WORK(j + r, r);
+            }
+        }
+#undef MC_WG_FETCH
+#undef MC_WG_WORK
+        for(int i = tid; i < m; i += NT)
             x[i] = s_x[i];
+    }
+
+    // LDS of a one-workgroup sweep of NT lanes over m rows and nsteps chunks:
+    // x, b, two product buffers (and MC_WG_PAD doubles that the row sums may
+    // read, never use, past the second) and the plan.
+    long long mc_wg_lds_bytes(int nt, int m, int nsteps)
+    {
+        return (2LL * m + 2LL * nt * MC_WG_U + MC_WG_PAD) * (long long)sizeof(double) +
+               32LL * nsteps;
+    }
+
+    // Enables MC_WG_LDS_MAX bytes of LDS (beyond the 64 KiB default) for the
+    // three one-workgroup sweeps once; false when the device does not allow it.
+    bool mc_wg_lds_ok()
+    {
+        static int ok = -1;
+        if(ok < 0)
+        {
+            int dev = 0, v = 0;
+            ok = 0;
+            if(hipGetDevice(&dev) == hipSuccess &&
+               hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) ==
+                   hipSuccess &&
+               v >= MC_WG_LDS_MAX &&
+               hipFuncSetAttribute(reinterpret_cast<const void*>(&mc_wg_kernel<64>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   MC_WG_LDS_MAX) == hipSuccess &&
+               hipFuncSetAttribute(reinterpret_cast<const void*>(&mc_wg_kernel<256>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   MC_WG_LDS_MAX) == hipSuccess &&
+               hipFuncSetAttribute(reinterpret_cast<const void*>(&mc_wg_kernel<1024>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   MC_WG_LDS_MAX) == hipSuccess)
+                ok = 1;
+            (void)hipGetLastError();
+        }
+        return ok == 1;
     }
 
     // Greedy first-fit colouring in row order over the symmetrised pattern.
@@ -189,6 +381,73 @@ namespace
         while(l < 64 && 4 * l < mean_row)
             l <<= 1;
         return l;
+    }
+
+    // The one-workgroup sweep's plan (mc_wg_kernel) of a colour-ordered level
+    // when it qualifies: at most max_nnz entries (one workgroup streams the
+    // matrix from memory far slower than the per-colour launches on larger
+    // levels), no row longer than a chunk, and x, b, the product buffers and
+    // the plan within one workgroup's LDS.  NT is the smallest of 64, 256 and 1024 lanes
+    // whose chunk holds the level's mean colour.  Every chunk is checked
+    // against what the kernel assumes (a level whose plan failed that check
+    // would take the per-colour launches).
+    void mc_wg_plan(mc_sched& s, const std::vector<int>& prp, int max_nnz)
+    {
+        const int m  = (int)prp.size() - 1;
+        s.wg         = false;
+        s.nchunks    = 0;
+        s.plan_h.clear();
+        if(m <= 0 || s.nnz <= 0 || s.nnz > max_nnz || s.ncol <= 0)
+            return;
+        int longest = 0;
+        for(int t = 0; t < m; t++)
+            longest = std::max(longest, prp[t + 1] - prp[t]);
+        const double mean_colour = (double)s.nnz / s.ncol;
+        int          nt          = 64;
+        while(nt < 1024 && (mean_colour > nt * MC_WG_U || longest > nt * MC_WG_U))
+            nt *= 4;
+        if(longest > nt * MC_WG_U)
+            return;
+        int lg = 0; // lanes per row: the level's, at most 16 (group_sum16)
+        while((1 << lg) < s.lanes && lg < 4)
+            lg++;
+        const int ch = nt * MC_WG_U, rows_max = nt >> lg;
+        std::vector<int>& plan = s.plan_h;
+        for(int pass = 0; pass < 2; pass++)
+        {
+            const size_t first = plan.size();
+            for(int q = 0; q < s.ncol; q++)
+            {
+                const int k  = pass == 0 ? q : s.ncol - 1 - q;
+                const int t1 = s.off[k + 1];
+                for(int t = s.off[k]; t < t1;)
+                {
+                    const int ta = t, c0 = prp[ta];
+                    while(t < t1 && t - ta < rows_max && prp[t + 1] - c0 <= ch)
+                        t++;
+                    plan.insert(plan.end(), {c0, prp[t], ta, t, t == t1 ? 1 : 0, 0, 0, 0});
+                }
+            }
+            while((plan.size() - first) / 8 % MC_WG_PF != 0)
+                plan.insert(plan.end(), {s.nnz, s.nnz, 0, 0, 1, 0, 0, 0});
+            if(pass == 0)
+                s.nchunks = (int)(plan.size() / 8);
+        }
+        if((int)(plan.size() / 8) != 2 * s.nchunks ||
+           mc_wg_lds_bytes(nt, m, 2 * s.nchunks) > MC_WG_LDS_MAX)
+            return;
+        for(size_t j = 0; j < plan.size(); j += 8)
+        {
+            const int c0 = plan[j], c1 = plan[j + 1], ta = plan[j + 2], tb = plan[j + 3];
+            const bool ok = 0 <= c0 && c0 <= c1 && c1 <= s.nnz && c1 - c0 <= ch && 0 <= ta &&
+                            ta <= tb && tb <= m && tb - ta <= rows_max &&
+                            (ta == tb || (prp[ta] == c0 && prp[tb] == c1));
+            if(!ok)
+                return;
+        }
+        s.wg_nt = nt;
+        s.wg_lg = lg;
+        s.wg    = true;
     }
 }
 
@@ -248,17 +507,11 @@ namespace lal
         }
         s.ncol  = ncol;
         s.lanes = lanes_for(m ? (double)pci.size() / m : 0.0);
-        s.wg    = m <= MC_WG_ROWS;
+        s.nnz = (int)pci.size();
+        mc_wg_plan(s, prp, MC_WG_MAX_NNZ);
+        const std::vector<int>& plan = s.plan_h;
         if(s.wg)
-        {
-            // one pass of the workgroup's lane groups should cover the largest
-            // colour, so a colour costs one round of loads, not several
-            int widest = 1;
-            for(int k = 0; k < ncol; k++)
-                widest = std::max(widest, s.off[k + 1] - s.off[k]);
-            while(s.lanes > 1 && (1024 / s.lanes) < widest)
-                s.lanes >>= 1;
-        }
+            LAL_TRY(alloc_n(&s.plan, plan.size()));
         LAL_TRY(alloc_n(&s.perm, m));
         LAL_TRY(alloc_n(&s.off_d, s.off.size()));
         LAL_TRY(alloc_n(&s.prp, m + 1));
@@ -280,6 +533,9 @@ namespace lal
         }
         LAL_HIP(hipMemcpyAsync(s.pdinv, pdinv.data(), sizeof(double) * m, hipMemcpyHostToDevice,
                                c->stream));
+        if(s.wg)
+            LAL_HIP(hipMemcpyAsync(s.plan, plan.data(), sizeof(int) * plan.size(),
+                                   hipMemcpyHostToDevice, c->stream));
         LAL_HIP(hipStreamSynchronize(c->stream));
         s.built = true;
         return LAL_OK;
@@ -291,11 +547,22 @@ namespace lal
     {
         if(m == 0 || s.ncol == 0)
             return LAL_OK;
-        if(s.wg)
+        if(s.wg && mc_wg_lds_ok())
         {
-            mc_wg_kernel<<<1, 1024, sizeof(double) * m, c->stream>>>(
-                m, s.ncol, s.off_d, s.perm, s.prp, s.pci, s.pv, s.pdinv, b, x, s.lanes,
-                sym ? 1 : 0);
+            // forward: the plan's first half; symmetric: both halves
+            const int steps = s.nchunks * (sym ? 2 : 1);
+            const int lds   = (int)mc_wg_lds_bytes(s.wg_nt, m, steps);
+            const auto* pl  = reinterpret_cast<const int4v*>(s.plan);
+#define LAL_MC_WG(NT)                                                                      \
+    mc_wg_kernel<NT><<<1, NT, lds, c->stream>>>(m, s.nnz, steps, s.wg_lg, pl, s.perm, s.prp, \
+                                                s.pci, s.pv, s.pdinv, b, x)
+            switch(s.wg_nt)
+            {
+            case 64: LAL_MC_WG(64); break;
+            case 256: LAL_MC_WG(256); break;
+            default: LAL_MC_WG(1024); break;
+            }
+#undef LAL_MC_WG
             LAL_LAUNCHED();
             return LAL_OK;
         }
