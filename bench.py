@@ -58,41 +58,48 @@ def parse():
                          "amg: AMG as a solver (V/W/F cycles, Jacobi/GS/SSOR) on 128^3, "
                          "krylov: BiCGSTAB / GMRES(30) with Jacobi and SA-AMG "
                          "preconditioning on 128^3 Poisson and 1024^2 convection-diffusion")
+    ap.add_argument("--hbm-ceiling", action="store_true",
+                    help="only time a 2 GiB streaming reduction and a 1 GiB copy (torch's "
+                         "kernels): the HBM bandwidth the SpMV fraction is set against")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     return ap.parse_args()
 
 
-def spmv_bytes(m, n, nnz, vbytes, xbytes, beta=False):
-    """Algorithmic bytes of y = A x: values + int32 columns per nonzero,
-    int32 row pointers, x read once, y written (and read when beta != 0)."""
-    return nnz * (vbytes + 4) + 4 * (m + 1) + xbytes * n + xbytes * m * (2 if beta else 1)
+def spmv_bytes(m, n, nnz, vbytes, xbytes, beta=False, cbytes=4):
+    """Algorithmic bytes of y = A x: values + columns (cbytes each: int32, or
+    the 16-bit offsets a hierarchy's operators may carry) per nonzero, int32
+    row pointers, x read once, y written (and read when beta != 0)."""
+    return nnz * (vbytes + cbytes) + 4 * (m + 1) + xbytes * n + xbytes * m * (2 if beta else 1)
 
 
 def solve_bytes(levels, iters, vbytes):
     """Algorithmic bytes of a SA-AMG-PCG solve: per iteration one SpMV with A,
     one V-cycle, two dot products, one inf-norm and three axpy-class updates
-    (vectors of vbytes each); levels = [(m, nnz_A, nnz_P)] finest first, the
-    last entry being the coarsest operator."""
+    (vectors of vbytes each); levels = [(m, nnz_A, nnz_P, cA, cP, cR)] finest
+    first (cX: bytes per column index the kernels read for A, P and R =
+    P^T), the last entry being the coarsest operator."""
     v = vbytes
-    m0, nnz0, _ = levels[0]
+    m0, nnz0, _, c0, _, _ = levels[0]
     prec = 0
-    for (m, nnz, nnz_p), (mc, _, _) in zip(levels[:-1], levels[1:]):
+    for (m, nnz, nnz_p, ca, cp, cr), nxt in zip(levels[:-1], levels[1:]):
+        mc = nxt[0]
         jac0 = 3 * v * m                                   # x = b / a_ii
-        res = nnz * (v + 4) + 4 * (m + 1) + v * m + 2 * v * m
-        restr = spmv_bytes(mc, m, nnz_p, v, v)
-        prol = spmv_bytes(m, mc, nnz_p, v, v, beta=True)
-        post = nnz * (v + 4) + 4 * (m + 1) + 3 * v * m
+        res = nnz * (v + ca) + 4 * (m + 1) + v * m + 2 * v * m
+        restr = spmv_bytes(mc, m, nnz_p, v, v, cbytes=cr)
+        prol = spmv_bytes(m, mc, nnz_p, v, v, beta=True, cbytes=cp)
+        post = nnz * (v + ca) + 4 * (m + 1) + 3 * v * m
         prec += jac0 + res + restr + prol + post
-    per_iter = spmv_bytes(m0, m0, nnz0, v, v) + prec + 2 * 2 * v * m0 + v * m0 + 3 * 3 * v * m0
+    per_iter = (spmv_bytes(m0, m0, nnz0, v, v, cbytes=c0) + prec + 2 * 2 * v * m0 + v * m0 +
+                3 * 3 * v * m0)
     return iters * per_iter
 
 
-def reliable_update_bytes(m, nnz):
+def reliable_update_bytes(m, nnz, cbytes=4):
     """Algorithmic bytes of one FP64 update of the reliable-update scheme:
     x += ||r0|| y (y FP32 read, x FP64 read and written), y = 0, r = b - A x in
-    FP64, its inf-norm, and the scaled FP32 copy of r."""
-    return 20 * m + 4 * m + (nnz * 12 + 4 * (m + 1) + 24 * m) + 8 * m + 12 * m
+    FP64 (cbytes per column index), its inf-norm, and the scaled FP32 copy of r."""
+    return 20 * m + 4 * m + (nnz * (8 + cbytes) + 4 * (m + 1) + 24 * m) + 8 * m + 12 * m
 
 
 def measured_traffic():
@@ -511,6 +518,29 @@ def run_gradient(args):
     print(json.dumps(out), flush=True)
 
 
+def hbm_ceiling(torch):
+    """Best of 10: a 2 GiB FP64 sum (reads 2 GiB) and a 1 GiB copy (reads and
+    writes 1 GiB each) with torch's own kernels, as TB/s of bytes moved."""
+    dev = torch.device("cuda:0")
+    n = 1 << 28
+    x = torch.ones(n, dtype=torch.float64, device=dev)
+    y = torch.empty(n // 2, dtype=torch.float64, device=dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    out = {}
+    for name, fn, nbytes in (("read_sum", lambda: x.sum(), 8 * n),
+                             ("copy", lambda: y.copy_(x[: n // 2]), 8 * n)):
+        best = float("inf")
+        for _ in range(11):
+            e0.record()
+            fn()
+            e1.record()
+            e1.synchronize()
+            best = min(best, e0.elapsed_time(e1) / 1e3)
+        out[name + "_tbs"] = round(nbytes / best / 1e12, 3)
+    print(json.dumps({"metric": "HBM streaming bandwidth (best of 10)", **out,
+                      "peak_tbs": HBM_PEAK_GBS / 1e3}), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -520,6 +550,9 @@ def main():
         sys.exit(2)
     import torch
 
+    if args.hbm_ceiling:
+        hbm_ceiling(torch)
+        return
     if args.config != "c2":
         sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
         run_config(args)
@@ -604,7 +637,8 @@ def main():
     for lvl in range(H.levels + 1):
         lm, _, annz = H.dims(lvl, "A")
         pnnz = H.dims(lvl, "P")[2] if lvl < H.levels else 0
-        levels.append((lm, annz, pnnz))
+        cb = [H.column_bytes(lvl, k) if k == "A" or lvl < H.levels else 4 for k in "APR"]
+        levels.append((lm, annz, pnnz, *cb))
     print(f"# setup {setup_s:.3f} s, levels {[(l[0], l[1]) for l in levels]}", file=sys.stderr,
           flush=True)
 
@@ -653,7 +687,7 @@ def main():
     solve64_gbs = solve_bytes(levels, f64["iters"], 8) / f64["median_s"] / 1e9
     rel = results["reliable"]
     rel_bytes = (solve_bytes(levels, rel["iters"], 4)
-                 + rel["outer"] * reliable_update_bytes(levels[0][0], levels[0][1]))
+                 + rel["outer"] * reliable_update_bytes(levels[0][0], levels[0][1], levels[0][3]))
     solve_rel_gbs = rel_bytes / rel["median_s"] / 1e9
 
     cpu = None

@@ -278,6 +278,13 @@ int lal_hierarchy_levels(lal_hierarchy hier, int* total_levels);
 /* kind: 0 = A, 1 = P, 2 = R.  Device pointers stay owned by the hierarchy. */
 int lal_hierarchy_level(lal_hierarchy hier, int level, int kind, int* m, int* n, int* nnz,
                         const int** row_ptr, const int** col_ind, const double** val);
+/* Bytes per column index that the fast-path kernels read for A (kind 0), P
+ * (1) or R (2) at a level: 2 when the setup also stored that operator's
+ * columns as 16-bit offsets from each row block's smallest column (possible
+ * when no block spans more than 65535 columns), else 4.  col_ind itself is
+ * always int32.  No counterpart in LinAlgLib: it describes this backend's
+ * storage, for byte accounting (bench.py's roofline of a solve). */
+int lal_hierarchy_column_bytes(lal_hierarchy hier, int level, int kind, int* bytes);
 
 /* z = M^-1 r: z zeroed, one V-cycle with n1 pre and n2 post undamped Jacobi
  * sweeps -- the reference composition

@@ -7,6 +7,8 @@
 // lal_spmv.h.
 #include "lal_rowvec.h"
 
+#include <climits>
+
 #include <vector>
 
 namespace
@@ -111,6 +113,49 @@ namespace
             }
     }
 
+    // One workgroup per row block: the block's column range, its base (the
+    // smallest column) and every entry's offset from it as 16 bits; *fail
+    // is set when a block spans more than 65535 columns.
+    __global__ void __launch_bounds__(256)
+        c16_kernel(const int* __restrict__ blocks, int nb, const int* __restrict__ ci,
+                   int* __restrict__ cbase, unsigned short* __restrict__ c16, int* fail)
+    {
+        typedef int int4v __attribute__((ext_vector_type(4)));
+        __shared__ int sh_mn[4], sh_mx[4];
+        const int4v mt = reinterpret_cast<const int4v*>(blocks + lal::block_meta_offset(nb))[blockIdx.x];
+        const int   e0 = mt.z, e1 = mt.w;
+        int         mn = INT_MAX, mx = INT_MIN;
+        for(int k = e0 + (int)threadIdx.x; k < e1; k += 256)
+        {
+            const int col = ci[k];
+            mn            = min(mn, col);
+            mx            = max(mx, col);
+        }
+#pragma unroll
+        for(int o = 32; o > 0; o >>= 1)
+        {
+            mn = min(mn, __shfl_xor(mn, o, 64));
+            mx = max(mx, __shfl_xor(mx, o, 64));
+        }
+        if((threadIdx.x & 63) == 0)
+        {
+            sh_mn[threadIdx.x >> 6] = mn;
+            sh_mx[threadIdx.x >> 6] = mx;
+        }
+        __syncthreads();
+        mn = min(min(sh_mn[0], sh_mn[1]), min(sh_mn[2], sh_mn[3]));
+        mx = max(max(sh_mx[0], sh_mx[1]), max(sh_mx[2], sh_mx[3]));
+        const int base = e1 > e0 ? mn : 0;
+        if(threadIdx.x == 0)
+        {
+            cbase[blockIdx.x] = base;
+            if(e1 > e0 && (long long)mx - mn > 65535)
+                atomicOr(fail, 1);
+        }
+        for(int k = e0 + (int)threadIdx.x; k < e1; k += 256)
+            c16[k] = (unsigned short)(ci[k] - base);
+    }
+
     template <typename T, typename ACC, int MODE>
     int csrmv_launch(lal_context* c, const lal_csrmv_info* d, int m, int n, int nnz, ACC alpha,
                      const int* rp, const int* ci, const T* v, const T* x, ACC beta,
@@ -128,6 +173,7 @@ namespace
             lal::rv_args a{m,    d->nnz, d->by_nnz, d->d_blocks, d->nblocks,     rp,           ci, v, x, nullptr,
                            b,    y,           nullptr,        (double)alpha, (double)beta, d->win};
             a.keep = d->keep;
+            lal::use_c16(a, d);
             return lal::rowvec_launch<T>(c, MODE == 0 ? lal::RV_SPMV : lal::RV_RESIDUAL, a);
         }
         // an exact descriptor, or none: one lane per row in CSR order (the
@@ -247,6 +293,51 @@ namespace lal
         return LAL_OK;
     }
 
+    int compact_columns(lal_context* c, lal_csrmv_info* d, int nnz, const int* ci)
+    {
+        if(!d || d->exact || nnz <= 0 || d->nblocks <= 0 || d->d_c16)
+            return LAL_OK;
+        unsigned short* c16   = nullptr;
+        int*            cbase = nullptr;
+        int*            fail  = nullptr;
+        int             h_fail = 1;
+        auto release = [&]() {
+            (void)hipFree(c16);
+            (void)hipFree(cbase);
+            (void)hipFree(fail);
+        };
+        // rounded up to whole 16-byte lines: the kernels load four entries at a time
+        if(hipMalloc(&c16, ((size_t)nnz * sizeof(unsigned short) + 15) & ~(size_t)15) !=
+               hipSuccess ||
+           hipMalloc(&cbase, sizeof(int) * d->nblocks) != hipSuccess ||
+           hipMalloc(&fail, sizeof(int)) != hipSuccess)
+        {
+            release();
+            return LAL_ERR_OOM;
+        }
+        int rc = hipMemsetAsync(fail, 0, sizeof(int), c->stream) == hipSuccess ? LAL_OK : LAL_ERR_HIP;
+        if(rc == LAL_OK)
+        {
+            c16_kernel<<<d->nblocks, 256, 0, c->stream>>>(d->d_blocks, d->nblocks, ci, cbase, c16,
+                                                         fail);
+            if(hipGetLastError() != hipSuccess ||
+               hipMemcpyAsync(&h_fail, fail, sizeof(int), hipMemcpyDeviceToHost, c->stream) !=
+                   hipSuccess ||
+               hipStreamSynchronize(c->stream) != hipSuccess)
+                rc = LAL_ERR_HIP;
+        }
+        (void)hipFree(fail);
+        fail = nullptr;
+        if(rc != LAL_OK || h_fail)
+        {
+            release();
+            return rc;
+        }
+        d->d_c16   = c16;
+        d->d_cbase = cbase;
+        return LAL_OK;
+    }
+
     int csrmv_f64(lal_context* c, const lal_csrmv_info* d, int m, int n, int nnz, double alpha,
                   const int* rp, const int* ci, const double* v, const double* x, double beta,
                   double* y)
@@ -304,6 +395,8 @@ extern "C" int lal_csrmv_descr_destroy(lal_csrmv_descr d)
     if(!d)
         return LAL_OK;
     (void)hipFree(d->d_blocks);
+    (void)hipFree(d->d_c16);
+    (void)hipFree(d->d_cbase);
     delete d;
     return LAL_OK;
 }

@@ -104,6 +104,45 @@ namespace lal
         return x * per + (x < rem ? x : rem) + k;
     }
 
+    // v + the value of v in another lane of the same quad, half row or row
+    // (DPP control CTRL: quad_perm, row_half_mirror 0x141, row_mirror 0x140).
+    template <int CTRL>
+    __device__ __forceinline__ double dpp_add(double v)
+    {
+        const long long b  = __builtin_bit_cast(long long, v);
+        const int       lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+        const int       hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+        return v + __builtin_bit_cast(double, (long long)(unsigned)lo | ((long long)hi << 32));
+    }
+    template <int CTRL>
+    __device__ __forceinline__ float dpp_add(float v)
+    {
+        const int o = __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false);
+        return v + __builtin_bit_cast(float, o);
+    }
+
+    // Sum over aligned groups of LANES (a power of two <= 64) lanes, left in
+    // every lane of the group: lane ^ 1 and lane ^ 2 (quad permutations), the
+    // half-row and row mirrors (each adds the other half's partial sum), then
+    // exchanges with the lanes 16 and 32 away.
+    template <int LANES, typename T>
+    __device__ __forceinline__ T lane_sum(T v)
+    {
+        if(LANES >= 2)
+            v = dpp_add<0xB1>(v); // quad_perm [1, 0, 3, 2]
+        if(LANES >= 4)
+            v = dpp_add<0x4E>(v); // quad_perm [2, 3, 0, 1]
+        if(LANES >= 8)
+            v = dpp_add<0x141>(v);
+        if(LANES >= 16)
+            v = dpp_add<0x140>(v);
+        if(LANES >= 32)
+            v += __shfl_xor(v, 16, 64);
+        if(LANES >= 64)
+            v += __shfl_xor(v, 32, 64);
+        return v;
+    }
+
     template <typename T>
     __device__ __forceinline__ T wave_sum(T v)
     {

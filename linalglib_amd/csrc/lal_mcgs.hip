@@ -40,51 +40,19 @@ namespace
 
     typedef int int4v __attribute__((ext_vector_type(4)));
 
-    // v + the value of v in lane SRC of the same quad, half row or row (DPP
-    // control CTRL), for a double.
-    template <int CTRL>
-    __device__ __forceinline__ double dpp_add(double v)
-    {
-        const long long b  = __builtin_bit_cast(long long, v);
-        const int       lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
-        const int       hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
-        return v + __builtin_bit_cast(double, (long long)(unsigned)lo | ((long long)hi << 32));
-    }
-
     // Sum over aligned groups of 2^lg <= 16 lanes, left in every lane of the
     // group: quad permutations (lane ^ 1, lane ^ 2), then the half-row and
     // row mirrors, each adding the other half's partial sum.
     __device__ __forceinline__ double group_sum16(double v, int lg)
     {
         if(lg >= 1)
-            v = dpp_add<0xB1>(v); // quad_perm [1, 0, 3, 2]
+            v = lal::dpp_add<0xB1>(v); // quad_perm [1, 0, 3, 2]
         if(lg >= 2)
-            v = dpp_add<0x4E>(v); // quad_perm [2, 3, 0, 1]
+            v = lal::dpp_add<0x4E>(v); // quad_perm [2, 3, 0, 1]
         if(lg >= 3)
-            v = dpp_add<0x141>(v); // row_half_mirror
+            v = lal::dpp_add<0x141>(v); // row_half_mirror
         if(lg >= 4)
-            v = dpp_add<0x140>(v); // row_mirror
-        return v;
-    }
-
-    // Sum over aligned groups of LANES (a power of two <= 64) lanes, left in
-    // every lane of the group: group_sum16's steps, then exchanges with the
-    // lanes 16 and 32 away.
-    template <int LANES>
-    __device__ __forceinline__ double lane_sum(double v)
-    {
-        if(LANES >= 2)
-            v = dpp_add<0xB1>(v);
-        if(LANES >= 4)
-            v = dpp_add<0x4E>(v);
-        if(LANES >= 8)
-            v = dpp_add<0x141>(v);
-        if(LANES >= 16)
-            v = dpp_add<0x140>(v);
-        if(LANES >= 32)
-            v += __shfl_xor(v, 16, 64);
-        if(LANES >= 64)
-            v += __shfl_xor(v, 32, 64);
+            v = lal::dpp_add<0x140>(v); // row_mirror
         return v;
     }
 
@@ -130,7 +98,7 @@ namespace
             for(int u = 0; u < 4; u++)
                 s += k0 + u * LANES < re ? v[u] * xx[u] : 0.0;
         }
-        s = lane_sum<LANES>(s);
+        s = lal::lane_sum<LANES>(s);
         if(lig == 0)
             x[i] = (bi - s) * di;
     }

@@ -65,7 +65,23 @@ namespace lal
         double      beta;
         int         win; // LDS window in entries (lal_csrmv_info::win)
         int         keep = 0; // the matrix may stay in the cache (lal_csrmv_info::keep)
+        // 16-bit columns and their per-block bases (lal_csrmv_info::d_c16), or null
+        const unsigned short* c16   = nullptr;
+        const int*            cbase = nullptr;
     };
+
+    // Builds the 16-bit column copy of a fast-path descriptor's matrix
+    // (d->d_c16, d->d_cbase) when every row block's columns span at most
+    // 65535; otherwise leaves the descriptor as it is.  Results of the row
+    // kernels are the same either way: only the bytes read change.
+    int compact_columns(lal_context* c, lal_csrmv_info* d, int nnz, const int* ci);
+
+    // The descriptor's 16-bit columns into a's fields.
+    inline void use_c16(rv_args& a, const lal_csrmv_info* d)
+    {
+        a.c16   = d->d_c16;
+        a.cbase = d->d_cbase;
+    }
 
     template <typename T>
     int rowvec_launch(lal_context* c, int mode, const rv_args& a);

@@ -15,7 +15,8 @@ namespace
         return __builtin_nontemporal_load(p);
     }
 
-    typedef int    int4v __attribute__((ext_vector_type(4)));
+    typedef int            int4v __attribute__((ext_vector_type(4)));
+    typedef unsigned short us4v __attribute__((ext_vector_type(4)));
     typedef float  flt4v __attribute__((ext_vector_type(4)));
     typedef double dbl2v __attribute__((ext_vector_type(2)));
 
@@ -221,6 +222,23 @@ namespace
         return acc;
     }
 
+    // Four consecutive column indices at q (a multiple of 4): 16 bytes of
+    // int32, or 8 bytes of 16-bit offsets from the block's base cb.
+    template <bool KEEP, bool C16>
+    __device__ __forceinline__ int4v load_cols(const int* __restrict__ ci,
+                                               const unsigned short* __restrict__ c16, int cb,
+                                               int q)
+    {
+        if(C16)
+        {
+            const us4v* p = reinterpret_cast<const us4v*>(c16 + q);
+            const us4v  w = KEEP ? *p : __builtin_nontemporal_load(p);
+            return int4v{cb + (int)w.x, cb + (int)w.y, cb + (int)w.z, cb + (int)w.w};
+        }
+        const int4v* p = reinterpret_cast<const int4v*>(ci + q);
+        return KEEP ? *p : __builtin_nontemporal_load(p);
+    }
+
     // Quad-staged row-gather kernel (16-byte aligned ci and val).
     //
     // Phase 1 copies the block's column indices and values into LDS with
@@ -234,10 +252,11 @@ namespace
     // every offset of a handful of rows.  Fewer and wider memory instructions
     // and compact gathers are what lift this kernel over the product-staging
     // one on CDNA.
-    template <typename T, int MODE, bool KEEP>
+    template <typename T, int MODE, bool KEEP, bool C16>
     __global__ void __launch_bounds__(SPMV_THREADS)
         quad_kernel(int m, int nnz, const int* __restrict__ blocks, int nb, int win,
                     const int* __restrict__ rp, const int* __restrict__ ci,
+                    const unsigned short* __restrict__ c16, const int* __restrict__ cbase,
                     const T* __restrict__ val, const T* __restrict__ x,
                     const T* __restrict__ d, const T* __restrict__ b, T alpha, T beta,
                     T* __restrict__ y, T* __restrict__ r)
@@ -263,6 +282,7 @@ namespace
         const int  base   = mt.z;
         const int  end    = mt.w;
         const int  nnz4   = nnz & ~3;
+        const int  cb     = C16 ? cbase[tile] : 0; // this block's 16-bit column base
         // The row bounds travel with the first window's loads (through LDS)
         // instead of being awaited before them: rp[r0] and rp[r1] are the
         // descriptor's base and end, lane i loads rp[r0 + i] for 0 < i < nrows.
@@ -283,9 +303,8 @@ namespace
 #pragma unroll
                 for(int u = 0; u < 2; u++)
                 {
-                    qq[u]           = min(q0 + 4 * SPMV_THREADS * u, vend - 4);
-                    const int4v* cp = reinterpret_cast<const int4v*>(ci + qq[u]);
-                    cq[u]           = KEEP ? *cp : __builtin_nontemporal_load(cp);
+                    qq[u] = min(q0 + 4 * SPMV_THREADS * u, vend - 4);
+                    cq[u] = load_cols<KEEP, C16>(ci, c16, cb, qq[u]);
                     vq[u].template load<KEEP>(val + qq[u]);
                 }
                 // Every quad is stored unconditionally: a lane whose second
@@ -304,7 +323,7 @@ namespace
             // the arrays' last nnz % 4 entries
             for(int k = max(vend, ws) + tid; k < we; k += SPMV_THREADS)
             {
-                s_c[k - ws] = ci[k];
+                s_c[k - ws] = C16 ? cb + (int)c16[k] : ci[k];
                 s_v[k - ws] = val[k];
             }
             if(!bounds)
@@ -345,10 +364,11 @@ namespace
     // stores the products in LDS; the rows then sum their products.  Used for
     // the rectangular transfer operators, whose short and uneven rows would
     // leave lanes of the per-row gather idle.
-    template <typename T, int MODE>
+    template <typename T, int MODE, bool C16>
     __global__ void __launch_bounds__(SPMV_THREADS)
         quad_nnz_kernel(int m, int nnz, const int* __restrict__ blocks, int nb, int win,
                         const int* __restrict__ rp, const int* __restrict__ ci,
+                        const unsigned short* __restrict__ c16, const int* __restrict__ cbase,
                         const T* __restrict__ val, const T* __restrict__ x,
                         const T* __restrict__ d, const T* __restrict__ b, T alpha, T beta,
                         T* __restrict__ y, T* __restrict__ r)
@@ -372,6 +392,7 @@ namespace
         const int  base   = mt.z;
         const int  end    = mt.w;
         const int  nnz4   = nnz & ~3;
+        const int  cb     = C16 ? cbase[tile] : 0; // this block's 16-bit column base
         // row bounds through LDS, loaded with the first window (quad_kernel)
         const int rpv = (tid > 0 && tid < nrows) ? rp[r0 + tid] : 0;
         int       rs = 0, re = 0;
@@ -394,8 +415,17 @@ namespace
                 for(int u = 0; u < 2; u++)
                 {
                     qq[u] = min(q0 + 4 * SPMV_THREADS * u, vend - 4);
-                    cq[u] = __builtin_nontemporal_load(reinterpret_cast<const int4v*>(ci + qq[u]));
+                    cq[u] = load_cols<false, C16>(ci, c16, cb, qq[u]);
                     vq[u].load(val + qq[u]);
+                    // a 16-bit column of a neighbouring block's entry (a quad
+                    // straddling the block's ends) decodes against this
+                    // block's base: gather at the base instead (its product
+                    // belongs to no row of this block)
+                    if(C16)
+#pragma unroll
+                        for(int e = 0; e < 4; e++)
+                            if(qq[u] + e < base || qq[u] + e >= end)
+                                cq[u][e] = cb;
                 }
                 T v8[8], x8[8];
 #pragma unroll
@@ -413,7 +443,9 @@ namespace
                         s_p[qq[u] - ws + e] = v8[4 * u + e] * x8[4 * u + e];
             }
             for(int k = max(vend, ws) + tid; k < we; k += SPMV_THREADS)
-                s_p[k - ws] = val[k] * gather<T, MODE>(x, d, b, ci[k]);
+                s_p[k - ws] = val[k] * gather<T, MODE>(x, d, b,
+                                                       C16 ? (k < base ? cb : cb + (int)c16[k])
+                                                           : ci[k]);
             if(!bounds)
             {
                 if(tid > 0 && tid < nrows)
@@ -447,33 +479,46 @@ namespace
         row_epilogue<T, MODE>(row, acc, x, d, b, alpha, beta, y, r);
     }
 
+    template <typename T, int MODE, bool C16>
+    void launch_staged(lal_context* c, const lal::rv_args& a, int nb)
+    {
+        // LDS sized to the widest block window of this matrix, so that short
+        // blocks leave room for more resident workgroups
+        const int    win  = a.win;
+        const size_t cbyt = C16 ? sizeof(unsigned short) : sizeof(int);
+        if(a.by_nnz)
+            quad_nnz_kernel<T, MODE, C16><<<nb, SPMV_THREADS, RP_LDS_BYTES + sizeof(T) * win,
+                                           c->stream>>>(
+                a.m, a.nnz, a.blocks, nb, win, a.rp, a.ci, a.c16, a.cbase, (const T*)a.val,
+                (const T*)a.x, (const T*)a.d, (const T*)a.b, (T)a.alpha, (T)a.beta, (T*)a.y,
+                (T*)a.r);
+        else if(a.keep && (sizeof(T) + cbyt) * (size_t)a.nnz <= lal::CACHE_KEEP_BYTES)
+            quad_kernel<T, MODE, true, C16><<<nb, SPMV_THREADS,
+                                              RP_LDS_BYTES + (sizeof(int) + sizeof(T)) * win,
+                                              c->stream>>>(
+                a.m, a.nnz, a.blocks, nb, win, a.rp, a.ci, a.c16, a.cbase, (const T*)a.val,
+                (const T*)a.x, (const T*)a.d, (const T*)a.b, (T)a.alpha, (T)a.beta, (T*)a.y,
+                (T*)a.r);
+        else
+            quad_kernel<T, MODE, false, C16><<<nb, SPMV_THREADS,
+                                               RP_LDS_BYTES + (sizeof(int) + sizeof(T)) * win,
+                                               c->stream>>>(
+                a.m, a.nnz, a.blocks, nb, win, a.rp, a.ci, a.c16, a.cbase, (const T*)a.val,
+                (const T*)a.x, (const T*)a.d, (const T*)a.b, (T)a.alpha, (T)a.beta, (T*)a.y,
+                (T*)a.r);
+    }
+
     template <typename T, int MODE>
     int launch(lal_context* c, const lal::rv_args& a)
     {
         const int nb = a.blocks ? a.nb : (a.m + SPMV_THREADS - 1) / SPMV_THREADS;
         if(nb == 0)
             return LAL_OK;
-        const bool aligned = ((reinterpret_cast<uintptr_t>(a.ci) |
-                               reinterpret_cast<uintptr_t>(a.val)) & 15) == 0;
-        // LDS sized to the widest block window of this matrix, so that short
-        // blocks leave room for more resident workgroups
-        const int win = a.win;
-        if(aligned && a.blocks && a.by_nnz)
-            quad_nnz_kernel<T, MODE><<<nb, SPMV_THREADS, RP_LDS_BYTES + sizeof(T) * win,
-                                      c->stream>>>(
-                a.m, a.nnz, a.blocks, nb, win, a.rp, a.ci, (const T*)a.val, (const T*)a.x,
-                (const T*)a.d, (const T*)a.b, (T)a.alpha, (T)a.beta, (T*)a.y, (T*)a.r);
-        else if(aligned && a.blocks && a.keep &&
-                (sizeof(T) + sizeof(int)) * (size_t)a.nnz <= lal::CACHE_KEEP_BYTES)
-            quad_kernel<T, MODE, true><<<nb, SPMV_THREADS, RP_LDS_BYTES + (sizeof(int) + sizeof(T)) * win,
-                                         c->stream>>>(
-                a.m, a.nnz, a.blocks, nb, win, a.rp, a.ci, (const T*)a.val, (const T*)a.x,
-                (const T*)a.d, (const T*)a.b, (T)a.alpha, (T)a.beta, (T*)a.y, (T*)a.r);
-        else if(aligned && a.blocks)
-            quad_kernel<T, MODE, false><<<nb, SPMV_THREADS, RP_LDS_BYTES + (sizeof(int) + sizeof(T)) * win,
-                                          c->stream>>>(
-                a.m, a.nnz, a.blocks, nb, win, a.rp, a.ci, (const T*)a.val, (const T*)a.x,
-                (const T*)a.d, (const T*)a.b, (T)a.alpha, (T)a.beta, (T*)a.y, (T*)a.r);
+        const bool val16 = (reinterpret_cast<uintptr_t>(a.val) & 15) == 0;
+        if(a.blocks && val16 && a.c16 && a.cbase)
+            launch_staged<T, MODE, true>(c, a, nb);
+        else if(a.blocks && val16 && (reinterpret_cast<uintptr_t>(a.ci) & 15) == 0)
+            launch_staged<T, MODE, false>(c, a, nb);
         else
             staged_kernel<T, MODE><<<nb, SPMV_THREADS, 0, c->stream>>>(
                 a.m, a.blocks, nb, a.rp, a.ci, (const T*)a.val, (const T*)a.x, (const T*)a.d,

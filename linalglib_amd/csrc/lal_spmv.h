@@ -27,6 +27,11 @@ struct lal_csrmv_info
     int* d_blocks = nullptr; // nblocks + 1 row offsets, then per block
                              // {r0, r1, row_ptr[r0], row_ptr[r1]} at
                              // lal::block_meta_offset(nblocks)
+    // 16-bit columns (fast path of the operators a hierarchy owns, set by
+    // lal::compact_columns): entry k of block b stores col_ind[k] - d_cbase[b];
+    // null when not built or a block's columns span more than 65535
+    unsigned short* d_c16   = nullptr;
+    int*            d_cbase = nullptr;
 };
 
 namespace lal

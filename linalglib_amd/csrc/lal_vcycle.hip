@@ -470,6 +470,7 @@ namespace lal
         {
             rv_args a{m, d->nnz, d->by_nnz, d->d_blocks, d->nblocks, rp, ci, v, xold, dinv, b, xnew, nullptr, 0.0,
                       0.0, d->win};
+            lal::use_c16(a, d);
             a.keep = d->keep;
             return rowvec_launch<T>(c, RV_JACOBI, a);
         }
@@ -492,6 +493,7 @@ namespace lal
         if(d && !d->exact)
         {
             rv_args a{m, d->nnz, d->by_nnz, d->d_blocks, d->nblocks, rp, ci, v, nullptr, dinv, b, x, r, 0.0, 0.0, d->win};
+            lal::use_c16(a, d);
             a.keep = d->keep;
             return rowvec_launch<T>(c, RV_JACOBI0, a);
         }
@@ -603,6 +605,7 @@ namespace lal
         if(m == 0)
             return LAL_OK;
         rv_args a{m, d->nnz, d->by_nnz, d->d_blocks, d->nblocks, rp, ci, v, r, dinv_c, nullptr, b_c, x_c, 1.0, 0.0, d->win};
+        lal::use_c16(a, d);
         a.keep = d->keep;
         return rowvec_launch<T>(c, RV_RESTRICT, a);
     }

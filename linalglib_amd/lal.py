@@ -108,6 +108,7 @@ _SIGS = {
     "lal_hierarchy_levels": [vp, C.POINTER(i32)],
     "lal_hierarchy_level": [vp, i32, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
                             C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)],
+    "lal_hierarchy_column_bytes": [vp, i32, i32, C.POINTER(i32)],
     "lal_amg_precond_f64": [vp, vp, vp, vp, i32, i32],
     "lal_solve_default_options": [C.POINTER(SolveOptions)],
     "lal_pcg_solve_f64": [vp, vp, vp, vp, C.POINTER(SolveOptions), C.POINTER(SolveInfo)],
@@ -286,6 +287,14 @@ class Hierarchy:
                                          C.byref(nnz), C.byref(rp), C.byref(ci), C.byref(v)),
               "lal_hierarchy_level")
         return m.value, n.value, nnz.value
+
+    def column_bytes(self, level, kind):
+        """Bytes per column index the fast path reads for A/P/R at a level
+        (2: 16-bit offsets from each row block's base, 4: int32)."""
+        out = i32()
+        check(load().lal_hierarchy_column_bytes(self.hp, level, self.KIND[kind], C.byref(out)),
+              "lal_hierarchy_column_bytes")
+        return out.value
 
     def level(self, level, kind):
         """Host copy (row_ptr, col_ind, val, m, n) of A/P/R at a level."""

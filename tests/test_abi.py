@@ -78,13 +78,24 @@ def test_bench_byte_formulas():
     m, nnz = 2097152, 14581760
     assert bench.spmv_bytes(m, m, nnz, 8, 8) == 216_924_164  # 8(nnz+m+n)+4(m+1+nnz)
     assert bench.spmv_bytes(m, m, nnz, 4, 4) == 141_819_908
+    # 16-bit columns: 2 bytes per nonzero less
+    assert bench.spmv_bytes(m, m, nnz, 8, 8, cbytes=2) == 216_924_164 - 2 * nnz
     # one-level "hierarchy": only the PCG terms remain
-    assert bench.solve_bytes([(10, 30, 0)], 1, 8) == bench.spmv_bytes(10, 10, 30, 8, 8) + 14 * 8 * 10
+    assert bench.solve_bytes([(10, 30, 0, 4, 4, 4)], 1, 8) == \
+        bench.spmv_bytes(10, 10, 30, 8, 8) + 14 * 8 * 10
+    # two levels (A0 10 x 10 with 30 nonzeros, P 10 x 4 with 12): the Krylov
+    # SpMV, residual and post-sweep read A0's columns, the restriction R's,
+    # the prolongation P's; each column width counts once per nonzero read
+    lv = [(10, 30, 12, 2, 2, 4), (4, 10, 0, 4, 4, 4)]
+    wide = [(10, 30, 12, 4, 4, 4), (4, 10, 0, 4, 4, 4)]
+    assert bench.solve_bytes(wide, 1, 8) - bench.solve_bytes(lv, 1, 8) == \
+        2 * 30 * 3 + 2 * 12  # SpMV, residual, post-sweep with A0; prolongation with P
     # one reliable update: an FP64 residual (SpMV bytes plus b read and r
     # written), the FP32 correction read with x read and written, y cleared,
     # the norm of r and its scaled FP32 copy
     upd = bench.reliable_update_bytes(m, nnz)
     assert upd == bench.spmv_bytes(m, m, nnz, 8, 8) + 8 * m + (4 + 16) * m + 4 * m + 8 * m + 12 * m
+    assert bench.reliable_update_bytes(m, nnz, 2) == upd - 2 * nnz
 
 
 def test_poisson_generators_symmetric_and_sorted():

@@ -481,3 +481,34 @@ def test_sddmm(h):
              lal.ptr(U), lal.ptr(W), lal.ptr(out))
     rows = np.repeat(np.arange(200), np.diff(A.row_ptr))
     assert bits_equal(host(out)[: A.nnz], (-1.0 * u[rows]) * w[A.col_ind])
+
+
+def test_sixteen_bit_columns(h, oracle):
+    """The fast path reads a hierarchy operator's columns as 16-bit offsets
+    from each row block's smallest column when no block spans more than
+    65535 columns, as 3D Poisson's finest operator does.  One weak coupling
+    100,000 columns away (0 <-> 100,000, -1e-3: symmetric, and far smaller
+    than the operator's smallest eigenvalue, 0.0123, so A stays SPD) makes
+    two of the finest operator's blocks span too far: it keeps int32
+    columns.  Either way PCG takes the oracle's iterations (within one)."""
+    for far in (False, True):
+        rp, ci, v, m = pb.poisson3d(48)
+        if far:
+            A = CSR(rp, ci, v, m, m).to_scipy().tolil()
+            A[0, 100000] = A[100000, 0] = -1e-3
+            A = A.tocsr()
+            A.sort_indices()
+            rp, ci, v = (A.indptr.astype(np.int32), A.indices.astype(np.int32),
+                         A.data.astype(np.float64))
+        Ao = CSR(rp, ci, v, m, m)
+        H = _hier(h, Ao)
+        widths = [H.column_bytes(l, k) for l in range(H.levels + 1) for k in "APR"
+                  if k == "A" or l < H.levels]
+        log(f"16-bit columns, far coupling {far}: widths {widths}")
+        assert widths[0] == (4 if far else 2), widths
+        b = pb.rhs_from_solution(rp, ci, v, pb.uniform(m, seed=5))
+        x = empty(m, torch.float64)
+        info = H.pcg(dev(b), x, rel_tol=1e-10, abs_tol=1e-300, max_iter=200)
+        _, ito, _ = oracle.saamg_setup(Ao).pcg(b, rel_tol=1e-10)
+        assert info.converged and abs(info.iterations - ito) <= 1, (far, info.iterations, ito)
+        H.close()
