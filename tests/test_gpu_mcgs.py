@@ -158,9 +158,10 @@ def test_pcg_with_symmetric_multicolour_vcycle(h, oracle):
 def test_multicolour_vcycle_time_128(h):
     """One V(1,1) cycle with multicolour Gauss-Seidel at 3D Poisson 128^3 (the
     colourings built beforehand), as amg_solve runs it: the cycle (captured as
-    a HIP graph) plus the residual norm that the host reads after it.  The
-    forward cycle measures 2.07 ms on MI355X, the bar is 2.3 ms; the
-    symmetric one sweeps twice per level and has twice the bar."""
+    a HIP graph) plus the residual norm that the host reads after it, the best
+    of three runs of ten cycles.  At most 2 ms forward (1.91 ms measured on
+    MI355X); the symmetric cycle sweeps twice per level and has twice the
+    bar."""
     rp, ci, v, m = pb.poisson3d(128)
     H, _ = _hier(h, rp, ci, v, m)
     b = dev(pb.uniform(m, seed=3))
@@ -168,15 +169,17 @@ def test_multicolour_vcycle_time_128(h):
     for sm in ("mcgs", "mcsgs"):
         H.amg_solve(b, X, 1, 1, smoother=sm, rel_tol=1e-300, abs_tol=1e-300, max_iter=0)
         torch.cuda.synchronize()
-        reps = 10
-        t0 = time.perf_counter()
-        X.zero_()
-        H.amg_solve(b, X, 1, 1, smoother=sm, rel_tol=1e-300, abs_tol=1e-300,
-                    max_iter=reps - 1)
-        torch.cuda.synchronize()
-        per = (time.perf_counter() - t0) / reps
-        log(f"128^3 V-cycle with {sm}: {per * 1e3:.3f} ms (incl. the residual norm per cycle)")
-        assert per <= (2.3e-3 if sm == "mcgs" else 4.6e-3), per
+        reps, best = 10, float("inf")
+        for _ in range(3):
+            X.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            H.amg_solve(b, X, 1, 1, smoother=sm, rel_tol=1e-300, abs_tol=1e-300,
+                        max_iter=reps - 1)
+            torch.cuda.synchronize()
+            best = min(best, (time.perf_counter() - t0) / reps)
+        log(f"128^3 V-cycle with {sm}: {best * 1e3:.3f} ms (incl. the residual norm per cycle)")
+        assert best <= (2e-3 if sm == "mcgs" else 4e-3), best
 
 
 def test_multicolour_cycle_graph_replays_match_eager(h):
