@@ -536,62 +536,10 @@ namespace
     int run_cycle(lal_hierarchy_t* h, double* x, const double* b, int n1, int n2, int n3,
                   int cycle, int smoother)
     {
-        lal_context* c = h->c;
         if(smoother < 5)
             return cycle_on(h, 0, x, b, n1, n2, n3, cycle, smoother);
-        lal_hierarchy_t::cycle_graph* g = nullptr;
-        for(auto& e : h->cycle_graphs)
-            if(e.x == x && e.b == b && e.n1 == n1 && e.n2 == n2 && e.n3 == n3 &&
-               e.cycle == cycle && e.smoother == smoother)
-                g = &e;
-        if(g == nullptr)
-        {
-            LAL_TRY(cycle_on(h, 0, x, b, n1, n2, n3, cycle, smoother));
-            h->cycle_graphs.push_back({x, b, n1, n2, n3, cycle, smoother, false, nullptr});
-            return LAL_OK;
-        }
-        if(g->exec)
-        {
-            LAL_HIP(hipGraphLaunch(g->exec, c->stream));
-            return LAL_OK;
-        }
-        if(g->failed)
-            return cycle_on(h, 0, x, b, n1, n2, n3, cycle, smoother);
-        // Captured on the hierarchy's own stream (the caller's may be the
-        // null stream, which cannot be captured): the launches of cycle_on go
-        // to c->stream, so it points at the capture stream meanwhile.
-        if(h->capture_stream == nullptr &&
-           hipStreamCreateWithFlags(&h->capture_stream, hipStreamNonBlocking) != hipSuccess)
-        {
-            (void)hipGetLastError();
-            h->capture_stream = nullptr;
-            g->failed         = true;
-            return cycle_on(h, 0, x, b, n1, n2, n3, cycle, smoother);
-        }
-        hipGraph_t        graph = nullptr;
-        hipGraphExec_t    exec  = nullptr;
-        const hipStream_t user  = c->stream;
-        if(hipStreamBeginCapture(h->capture_stream, hipStreamCaptureModeThreadLocal) ==
-           hipSuccess)
-        {
-            c->stream          = h->capture_stream;
-            const int st       = cycle_on(h, 0, x, b, n1, n2, n3, cycle, smoother);
-            c->stream          = user;
-            const hipError_t e = hipStreamEndCapture(h->capture_stream, &graph);
-            if(st == LAL_OK && e == hipSuccess && graph != nullptr &&
-               hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess)
-                g->exec = exec;
-            if(graph)
-                (void)hipGraphDestroy(graph);
-        }
-        (void)hipGetLastError();
-        if(g->exec == nullptr)
-        {
-            g->failed = true;
-            return cycle_on(h, 0, x, b, n1, n2, n3, cycle, smoother);
-        }
-        LAL_HIP(hipGraphLaunch(g->exec, c->stream));
-        return LAL_OK;
+        return lal::captured(h, x, b, n1, n2, n3, cycle, smoother,
+                             [&] { return cycle_on(h, 0, x, b, n1, n2, n3, cycle, smoother); });
     }
 }
 

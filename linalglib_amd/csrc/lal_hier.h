@@ -162,12 +162,13 @@ struct lal_hierarchy_t
     int*           tail_img32 = nullptr;
     lal::tail_desc tail64{};
     lal::tail_desc tail32{};
-    // captured cycles of the multicolour smoothers (lal_cycles.hip), one per
-    // (x, b, sweeps, cycle, smoother) combination
+    // captured cycles (lal::captured): the multicolour smoothers' cycles
+    // (kind = smoother 5 or 6) and the PCG preconditioner's V-cycle (kind
+    // lal::KIND_PRECOND + 2 xinit + FP32), one per (x, b, sweeps, cycle, kind)
     struct cycle_graph
     {
-        const double*  x;
-        const double*  b;
+        const void*    x;
+        const void*    b;
         int            n1, n2, n3, cycle, smoother;
         bool           failed;
         hipGraphExec_t exec;
@@ -246,4 +247,70 @@ namespace lal
     int post_scalar(lal_hierarchy_t* h, int op, const double* part, int slot, unsigned* seq);
     // Waits for the note carrying `seq` (checks the stream now and then).
     int wait_scalar(lal_hierarchy_t* h, unsigned seq, int slot, double* v);
+
+    constexpr int KIND_PRECOND = 100;
+
+    // Runs body() -- launches on h->c->stream only: no allocation, memset or
+    // host synchronisation -- eagerly on the first use of its key, captures it
+    // as a HIP graph on the hierarchy's own non-blocking stream on the second
+    // (the caller's stream may be the null stream, which cannot be captured;
+    // the context's stream points at the capture stream meanwhile) and
+    // replays the graph on the caller's stream from then on.  A capture that
+    // fails leaves the key eager.
+    template <typename Body>
+    int captured(lal_hierarchy_t* h, const void* x, const void* b, int n1, int n2, int n3,
+                 int cycle, int kind, Body&& body)
+    {
+        lal_context*                  c = h->c;
+        lal_hierarchy_t::cycle_graph* g = nullptr;
+        for(auto& e : h->cycle_graphs)
+            if(e.x == x && e.b == b && e.n1 == n1 && e.n2 == n2 && e.n3 == n3 &&
+               e.cycle == cycle && e.smoother == kind)
+                g = &e;
+        if(g == nullptr)
+        {
+            LAL_TRY(body());
+            h->cycle_graphs.push_back({x, b, n1, n2, n3, cycle, kind, false, nullptr});
+            return LAL_OK;
+        }
+        if(g->exec)
+        {
+            LAL_HIP(hipGraphLaunch(g->exec, c->stream));
+            return LAL_OK;
+        }
+        if(g->failed)
+            return body();
+        if(h->capture_stream == nullptr &&
+           hipStreamCreateWithFlags(&h->capture_stream, hipStreamNonBlocking) != hipSuccess)
+        {
+            (void)hipGetLastError();
+            h->capture_stream = nullptr;
+            g->failed         = true;
+            return body();
+        }
+        hipGraph_t        graph = nullptr;
+        hipGraphExec_t    exec  = nullptr;
+        const hipStream_t user  = c->stream;
+        if(hipStreamBeginCapture(h->capture_stream, hipStreamCaptureModeThreadLocal) ==
+           hipSuccess)
+        {
+            c->stream          = h->capture_stream;
+            const int        st = body();
+            c->stream          = user;
+            const hipError_t e  = hipStreamEndCapture(h->capture_stream, &graph);
+            if(st == LAL_OK && e == hipSuccess && graph != nullptr &&
+               hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess)
+                g->exec = exec;
+            if(graph)
+                (void)hipGraphDestroy(graph);
+        }
+        (void)hipGetLastError();
+        if(g->exec == nullptr)
+        {
+            g->failed = true;
+            return body();
+        }
+        LAL_HIP(hipGraphLaunch(g->exec, c->stream));
+        return LAL_OK;
+    }
 }

@@ -184,3 +184,31 @@ def test_saamg_solve_reuses_hierarchy(h, oracle):
     assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) <= 1e-8
     assert np.linalg.norm(host(b.grad) - lo) / np.linalg.norm(lo) <= 1e-8
     assert np.linalg.norm(host(val.grad) - dA) / np.linalg.norm(dA) <= 1e-8
+
+
+def test_pcg_graphed_preconditioner_replays_match(h, oracle):
+    """On a small finest level (2D Poisson 128^2) the PCG's V-cycle is run
+    eagerly on its first application, captured as a HIP graph on its second
+    and replayed from then on, with the solver's own eager kernels and host
+    reads between replays.  A second solve, every application a replay,
+    reproduces the first bit for bit, in FP64 and in the reliable-update FP32
+    scheme, and the FP64 solve takes the oracle's iterations (within one)."""
+    rp, ci, v, m = pb.poisson2d(128)
+    RP, CI, V, _ = _dev_csr(h, rp, ci, v, m)
+    H = lal.Hierarchy(h, m, len(ci), RP, CI, V, build_f32=1)
+    b_h = pb.rhs_from_solution(rp, ci, v, pb.uniform(m, seed=11))
+    b = dev(b_h)
+    for name in ("pcg", "pcg_reliable"):
+        xs, its = [], []
+        for _ in range(2):
+            X = torch.zeros(m, dtype=torch.float64, device="cuda")
+            info = getattr(H, name)(b, X, rel_tol=1e-10, abs_tol=1e-300, max_iter=500)
+            torch.cuda.synchronize()
+            assert info.converged, name
+            xs.append(host(X))
+            its.append(info.iterations)
+        log(f"graphed preconditioner, {name}: {its[0]} / {its[1]} iterations")
+        assert its[0] == its[1] and bits_equal(xs[0], xs[1]), name
+        if name == "pcg":
+            _, ito, _ = oracle.saamg_setup(CSR(rp, ci, v, m, m)).pcg(b_h, rel_tol=1e-10)
+            assert abs(its[0] - ito) <= 1, (its[0], ito)
