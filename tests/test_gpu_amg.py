@@ -201,6 +201,40 @@ def test_spgemm_refuses_inconsistent_row_pointer(h, oracle):
         assert np.all(host(guard)[nnz.value:] == 7.0) and np.all(host(gci)[nnz.value:] == -7)
 
 
+def test_spgemm_refuses_repeated_column_in_b(h):
+    """A row of B that repeats a column index would make two lanes of the hash
+    pass fold into one slot at once; both passes refuse such a B
+    (LAL_ERR_UNSUPPORTED) before any product is formed, and the same operands
+    without the repeat multiply."""
+    rng = np.random.default_rng(23)
+    A = random_csr(rng, 40, 30, 5)
+    B = random_csr(rng, 30, 35, 6)
+    ci = B.col_ind.copy()
+    r = int(np.argmax(np.diff(B.row_ptr) >= 2))
+    ci[B.row_ptr[r] + 1] = ci[B.row_ptr[r]]  # row r names its first column twice
+    Bd = CSR(B.row_ptr, ci, B.val, B.m, B.n)
+    for Bx, refused in ((Bd, True), (B, False)):
+        da, db = DevCSR(A), DevCSR(Bx)
+        crp = empty(A.m + 1, torch.int32)
+        nnz = C.c_int()
+        args = (h.h, A.m, A.n, Bx.n, A.nnz, lal.ptr(da.row_ptr), lal.ptr(da.col_ind), Bx.nnz,
+                lal.ptr(db.row_ptr), lal.ptr(db.col_ind), lal.ptr(crp), C.byref(nnz))
+        if refused:
+            with pytest.raises(lal.LalError) as e:
+                lal.call("lal_csrgemm_nnz", *args)
+            assert e.value.status == 7, e.value.status  # LAL_ERR_UNSUPPORTED
+            cci = empty(64, torch.int32)
+            cv = empty(64, torch.float64)
+            with pytest.raises(lal.LalError):
+                lal.call("lal_csrgemm_solve_f64", h.h, A.m, A.n, Bx.n, A.nnz,
+                         lal.ptr(da.row_ptr), lal.ptr(da.col_ind), lal.ptr(da.val), Bx.nnz,
+                         lal.ptr(db.row_ptr), lal.ptr(db.col_ind), lal.ptr(db.val),
+                         lal.ptr(crp), lal.ptr(cci), lal.ptr(cv))
+        else:
+            lal.call("lal_csrgemm_nnz", *args)
+            assert nnz.value > 0
+
+
 # ------------------------------------------------------------------ hierarchy
 
 
