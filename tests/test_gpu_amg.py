@@ -546,3 +546,32 @@ def test_sixteen_bit_columns(h, oracle):
         _, ito, _ = oracle.saamg_setup(Ao).pcg(b, rel_tol=1e-10)
         assert info.converged and abs(info.iterations - ito) <= 1, (far, info.iterations, ito)
         H.close()
+
+
+@pytest.mark.parametrize("nx,far,width", [(256, 65535, 2), (257, 65536, 4)])
+def test_sixteen_bit_columns_span_limit(h, oracle, nx, far, width):
+    """The 16-bit limit itself: 2D Poisson on nx x 256 with one weak coupling
+    between rows 0 and `far`.  With 256 x 256 rows and far = 65535 (the last
+    row) the two blocks holding those rows span exactly 65535 columns, so the
+    finest operator stores offsets up to 65535, the largest 16 bits hold;
+    with far = 65536 they span one more and the operator keeps int32
+    columns.  The coupling, -1e-4, is below the operator's smallest
+    eigenvalue (about 3.0e-4), so A stays SPD.  Either way PCG matches the
+    oracle (iterations within one, solution to 1e-8)."""
+    rp, ci, v, m = pb.poisson2d(nx, 256)
+    A = CSR(rp, ci, v, m, m).to_scipy().tolil()
+    A[0, far] = A[far, 0] = -1e-4
+    A = A.tocsr()
+    A.sort_indices()
+    rp, ci, v = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64)
+    Ao = CSR(rp, ci, v, m, m)
+    H = _hier(h, Ao)
+    log(f"16-bit span limit: m={m} far={far} width {H.column_bytes(0, 'A')}")
+    assert H.column_bytes(0, "A") == width
+    b = pb.rhs_from_solution(rp, ci, v, pb.uniform(m, seed=6))
+    x = empty(m, torch.float64)
+    info = H.pcg(dev(b), x, rel_tol=1e-10, abs_tol=1e-300, max_iter=300)
+    xo, ito, _ = oracle.saamg_setup(Ao).pcg(b, rel_tol=1e-10)
+    assert info.converged and abs(info.iterations - ito) <= 1, (info.iterations, ito)
+    assert np.linalg.norm(host(x) - xo) <= 1e-8 * np.linalg.norm(xo)
+    H.close()
