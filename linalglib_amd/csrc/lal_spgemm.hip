@@ -4,10 +4,12 @@
 // Transpose: count, scan, scatter, then each row of the result is sorted by
 // column, which gives the stable order of backend/host/host_csr2csc.cpp:30-75.
 //
-// SpGEMM: one wavefront per row of C.  Rows are binned by their product count
-// into LDS hash tables of H = 64 ... 4096 slots (open addressing, linear
-// probing, slots claimed with atomicCAS); rows beyond that get a table in
-// global memory.  The numeric pass walks the A row sequentially and lets the
+// SpGEMM: rows of C are binned on the device by their product count into LDS
+// hash tables of H = 64 ... 4096 slots (open addressing, linear probing,
+// slots claimed with atomicCAS); rows beyond that get a table in global
+// memory.  Tables of up to 256 slots are built by H / 8 lanes per row (and
+// the symbolic pass's key-only tables of 512 and 1024 slots by 32), larger
+// ones by a wavefront per row.  The numeric pass walks the A row sequentially and lets the
 // lanes take the B row in parallel; the columns of one B row are distinct, so
 // every C entry receives at most one contribution per step and is folded in
 // A-row-then-B-row order from its first product -- the exact accumulation of
@@ -272,6 +274,142 @@ namespace
                             aci, av, brp, bci, bv, crp, cout_, cval, err);
     }
 
+    // Rows of C with small tables: G lanes per row (H / 8 for tables of up to
+    // 256 slots, 32 for the symbolic pass's 512 and 1024), 256 / G rows per
+    // workgroup.  The steps and the accumulation order are those of
+    // spgemm_row -- A-row entries one after another, the lanes of the group
+    // over the entries of that entry's B row -- but the group first loads up
+    // to G entries of the A row and their B-row bounds at once, one entry per
+    // lane, and each step takes its bounds from the owning lane, so a step
+    // waits on one load (the B row) instead of three dependent ones.  Rows
+    // of P and A P hold a few to a few tens of entries, which a full
+    // wavefront per row would leave mostly idle.
+    template <int H, int G, bool NUMERIC>
+    __global__ void __launch_bounds__(256)
+        spgemm_group_kernel(int nrows, const int* __restrict__ rows, const int* __restrict__ arp,
+                            const int* __restrict__ aci, const double* __restrict__ av,
+                            const int* __restrict__ brp, const int* __restrict__ bci,
+                            const double* __restrict__ bv, const int* __restrict__ crp,
+                            int* __restrict__ cout_, double* __restrict__ cval,
+                            int* __restrict__ err)
+    {
+        static_assert(G >= 2 && G <= 32 && (G & (G - 1)) == 0 && H % G == 0);
+        constexpr int RPB   = 256 / G;
+        constexpr int hmask = H - 1;
+        __shared__ int    s_keys[RPB][H];
+        __shared__ double s_vals[NUMERIC ? RPB : 1][NUMERIC ? H : 1];
+        const int         grp = (int)threadIdx.x / G;
+        const int         gl  = (int)threadIdx.x % G;       // lane in the group
+        const int         gb  = ((int)threadIdx.x & 63) - gl; // the group's first lane
+        const int         r   = blockIdx.x * RPB + grp;
+        if(r >= nrows)
+            return;
+        const int row  = rows[r];
+        int*      keys = s_keys[grp];
+        double*   vals = NUMERIC ? s_vals[grp] : nullptr;
+        for(int s = gl; s < H; s += G)
+            keys[s] = -1;
+        wave_sync_mem<false>();
+        int       local = 0;
+        const int rs = arp[row], re = arp[row + 1];
+        for(int j0 = rs; j0 < re; j0 += G)
+        {
+            const int j  = j0 + gl;
+            int       bs = 0, be = 0;
+            double    va = 0.0;
+            if(j < re)
+            {
+                const int ca = aci[j];
+                if(NUMERIC)
+                    va = 1.0 * av[j];
+                bs = brp[ca];
+                be = brp[ca + 1];
+            }
+            const int cnt = min(G, re - j0);
+            for(int t = 0; t < cnt; t++)
+            {
+                const int    tbs = __shfl(bs, gb + t, 64);
+                const int    tbe = __shfl(be, gb + t, 64);
+                const double tva = NUMERIC ? __shfl(va, gb + t, 64) : 0.0;
+                for(int p = tbs + gl; p < tbe; p += G)
+                {
+                    const int col = bci[p];
+                    int       h   = slot_of(col, hmask);
+                    for(int probe = 0;; probe++)
+                    {
+                        if(probe == H)
+                        {
+                            *err = 1; // table full: the row pointer undercounts this row
+                            break;
+                        }
+                        int old = atomicCAS(&keys[h], -1, col);
+                        if(old == -1)
+                        {
+                            local++;
+                            if(NUMERIC)
+                                vals[h] = tva * bv[p];
+                            break;
+                        }
+                        if(old == col)
+                        {
+                            if(NUMERIC)
+                                vals[h] += tva * bv[p];
+                            break;
+                        }
+                        h = (h + 1) & hmask;
+                    }
+                }
+                wave_sync_mem<false>();
+            }
+        }
+        if(!NUMERIC)
+        {
+#pragma unroll
+            for(int o = G / 2; o > 0; o >>= 1)
+                local += __shfl_xor(local, o, 64);
+            if(gl == 0)
+                cout_[row] = local;
+            return;
+        }
+        // compact the occupied slots into [0, nnz) in place
+        int                      base  = 0;
+        const unsigned long long below = (1ull << gl) - 1ull;
+        for(int s0 = 0; s0 < H; s0 += G)
+        {
+            const int                s     = s0 + gl;
+            const int                k     = keys[s];
+            const double             v     = vals[s];
+            const bool               valid = k != -1;
+            const unsigned long long mask  = (__ballot(valid) >> gb) & ((1ull << G) - 1ull);
+            wave_sync_mem<false>();
+            if(valid)
+            {
+                int pos   = base + __popcll(mask & below);
+                keys[pos] = k;
+                vals[pos] = v;
+            }
+            base += __popcll(mask);
+            wave_sync_mem<false>();
+        }
+        const int nnz = base;
+        const int out = crp[row];
+        const int len = crp[row + 1] - out;
+        if(nnz != len && gl == 0)
+            *err = 1;
+        for(int e = gl; e < nnz; e += G)
+        {
+            const int k  = keys[e];
+            int       rk = 0;
+            for(int t = 0; t < nnz; t++)
+                rk += keys[t] < k;
+            if(rk < len)
+            {
+                cout_[out + rk] = k;
+                cval[out + rk]  = vals[e];
+            }
+        }
+    }
+
     template <bool NUMERIC>
     __global__ void __launch_bounds__(64)
         spgemm_global_kernel(int nrows, const int* __restrict__ rows,
@@ -297,24 +435,27 @@ namespace
     constexpr int kLdsMaxH    = 4096;
     constexpr int kLdsMaxHSym = 32768;
 
-    int table_size(int prod, int ncols)
+    // Slots of a row's table: the smallest power of two >= 64 holding twice
+    // the row's bound on distinct columns (its product or nonzero count,
+    // capped by the column count of C).
+    __device__ __forceinline__ long long table_slots(int prod, int ncols)
     {
-        long long need = 2LL * std::min(prod, std::max(ncols, 1));
-        long long H    = 64;
+        const long long cap  = ncols > 1 ? ncols : 1;
+        const long long need = 2LL * (prod < cap ? prod : cap);
+        long long       H    = 64;
         while(H < need)
             H <<= 1;
-        return (int)H;
+        return H;
     }
 
     // Rows of C grouped by hash-table size.
     struct spgemm_plan
     {
-        std::vector<int>  bin_start; // per LDS bin (H = 64 << b), into rows
-        std::vector<int>  rows;      // LDS-bin rows followed by global rows
+        std::vector<int>  bin_start; // per LDS bin (H = 64 << b), into d_rows
         int               n_lds    = 0;
         int               n_global = 0;
         std::vector<long long> toff; // global table offsets (n_global + 1)
-        int*              d_rows   = nullptr;
+        int*              d_rows   = nullptr; // LDS-bin rows, then global rows
         long long*        d_toff   = nullptr;
         int*              d_keys   = nullptr;
         double*           d_vals   = nullptr;
@@ -339,74 +480,150 @@ namespace
 
     constexpr int kBins = 10; // 64 .. 32768 (above 4096 only for the symbolic pass)
 
-    // Rows of C binned by table size: the symbolic pass sizes a row's table by
-    // its product count (an upper bound of its distinct columns), the numeric
-    // pass by its exact nonzero count from the symbolic pass (crp), which keeps
-    // almost every row in LDS.
+    // Bin of row i of C, or -1 for a row the pass skips: bins 0 .. kBins-1
+    // hold LDS tables of 64 << b slots, bin kBins tables in global memory
+    // (*slots gets the row's table size).  The symbolic pass sizes a row by
+    // its product count (`prod`) and skips rows without products; the
+    // numeric pass sizes every row by its nonzero count from the symbolic
+    // pass (`crp`), empty rows included, so that a row pointer which
+    // undercounts them is detected.
+    __device__ __forceinline__ int row_bin(int i, const int* __restrict__ prod,
+                                           const int* __restrict__ crp, int ncols, int max_h,
+                                           long long* slots)
+    {
+        const int p = crp ? crp[i + 1] - crp[i] : prod[i];
+        if(!crp && p <= 0)
+            return -1;
+        const long long H = table_slots(p > 0 ? p : 0, ncols);
+        *slots            = H;
+        int b             = 0;
+        while((64LL << b) < H && b < kBins)
+            b++;
+        return H > max_h ? kBins : b;
+    }
+
+    // Rows per bin, gathered per workgroup in LDS.
+    __global__ void __launch_bounds__(256)
+        bin_count_kernel(int m, const int* __restrict__ prod, const int* __restrict__ crp,
+                         int ncols, int max_h, int* __restrict__ hist)
+    {
+        __shared__ int s_n[kBins + 1];
+        if(threadIdx.x <= kBins)
+            s_n[threadIdx.x] = 0;
+        __syncthreads();
+        const int i = blockIdx.x * 256 + threadIdx.x;
+        long long H = 0;
+        const int b = i < m ? row_bin(i, prod, crp, ncols, max_h, &H) : -1;
+        if(b >= 0)
+            atomicAdd(&s_n[b], 1);
+        __syncthreads();
+        if(threadIdx.x <= kBins && s_n[threadIdx.x])
+            atomicAdd(&hist[threadIdx.x], s_n[threadIdx.x]);
+    }
+
+    // Rows into their bins: bin b follows the rows of bins 0 .. b-1, and
+    // within a bin the workgroups take their places in order of arrival (each
+    // row of C is computed on its own, so the order changes no result).  A
+    // row of the global bin also records its table size in gslots.
+    __global__ void __launch_bounds__(256)
+        bin_scatter_kernel(int m, const int* __restrict__ prod, const int* __restrict__ crp,
+                           int ncols, int max_h, const int* __restrict__ hist,
+                           int* __restrict__ cursor, int* __restrict__ rows,
+                           long long* __restrict__ gslots)
+    {
+        __shared__ int s_n[kBins + 1], s_base[kBins + 1];
+        if(threadIdx.x <= kBins)
+            s_n[threadIdx.x] = 0;
+        __syncthreads();
+        const int i     = blockIdx.x * 256 + threadIdx.x;
+        long long H     = 0;
+        const int b     = i < m ? row_bin(i, prod, crp, ncols, max_h, &H) : -1;
+        const int local = b >= 0 ? atomicAdd(&s_n[b], 1) : 0;
+        __syncthreads();
+        if(threadIdx.x <= kBins && s_n[threadIdx.x])
+        {
+            int start = 0;
+            for(int q = 0; q < (int)threadIdx.x; q++)
+                start += hist[q];
+            s_base[threadIdx.x] = start + atomicAdd(&cursor[threadIdx.x], s_n[threadIdx.x]);
+        }
+        __syncthreads();
+        if(b >= 0)
+        {
+            const int pos = s_base[b] + local;
+            rows[pos]     = i;
+            if(b == kBins)
+            {
+                int n_lds = 0;
+                for(int q = 0; q < kBins; q++)
+                    n_lds += hist[q];
+                gslots[pos - n_lds] = H;
+            }
+        }
+    }
+
+    // Rows of C binned by table size on the device; the host reads back the
+    // eleven bin counts (and, when some rows need global tables, their table
+    // sizes) to size the launches and the global tables.
     int make_plan(lal_context* c, int m, int n, const int* arp, const int* aci, const int* brp,
                   const int* crp, bool numeric, spgemm_plan& P)
     {
-        std::vector<int> prod(m);
-        if(numeric)
+        constexpr int nb = kBins + 1;
+        static_assert(nb <= 16, "the bin counts are read back through the context's flags");
+        P.bin_start.assign(nb, 0);
+        P.n_lds    = 0;
+        P.n_global = 0;
+        if(m <= 0)
+            return LAL_OK;
+        const int  max_h  = numeric ? kLdsMaxH : kLdsMaxHSym;
+        const int  grid   = lal::grid_for(m, 256);
+        int*       d_prod = nullptr;
+        int*       d_bins = nullptr; // counts, then cursors
+        long long* d_gsl  = nullptr;
+        if(!numeric)
         {
-            std::vector<int> h(m + 1);
-            LAL_HIP(hipMemcpyAsync(h.data(), crp, sizeof(int) * (m + 1), hipMemcpyDeviceToHost,
-                                   c->stream));
-            LAL_HIP(hipStreamSynchronize(c->stream));
-            for(int i = 0; i < m; i++)
-                prod[i] = h[i + 1] - h[i];
-        }
-        else
-        {
-            int* d_prod = nullptr;
-            LAL_HIP(hipMallocAsync((void**)&d_prod, sizeof(int) * std::max(m, 1), c->stream));
-            products_kernel<<<lal::grid_for(m, 256), 256, 0, c->stream>>>(m, arp, aci, brp,
-                                                                          d_prod);
+            LAL_HIP(hipMallocAsync((void**)&d_prod, sizeof(int) * m, c->stream));
+            products_kernel<<<grid, 256, 0, c->stream>>>(m, arp, aci, brp, d_prod);
             LAL_LAUNCHED();
-            LAL_HIP(hipMemcpyAsync(prod.data(), d_prod, sizeof(int) * m,
-                                   hipMemcpyDeviceToHost, c->stream));
-            LAL_HIP(hipFreeAsync(d_prod, c->stream));
-            LAL_HIP(hipStreamSynchronize(c->stream));
         }
-
-        const int                     max_h = numeric ? kLdsMaxH : kLdsMaxHSym;
-        std::vector<std::vector<int>> bins(kBins + 1);
-        for(int i = 0; i < m; i++)
-        {
-            // the numeric pass also visits rows the row pointer says are empty,
-            // so that a row pointer which undercounts them is detected
-            if(prod[i] <= 0 && !numeric)
-                continue;
-            int H = table_size(std::max(prod[i], 0), n);
-            int b = 0;
-            while((64 << b) < H && b < kBins)
-                b++;
-            bins[H > max_h ? kBins : b].push_back(i);
-        }
-        P.bin_start.assign(kBins + 1, 0);
-        P.rows.clear();
+        LAL_HIP(hipMallocAsync((void**)&d_bins, sizeof(int) * 2 * nb, c->stream));
+        LAL_HIP(hipMemsetAsync(d_bins, 0, sizeof(int) * 2 * nb, c->stream));
+        bin_count_kernel<<<grid, 256, 0, c->stream>>>(m, d_prod, numeric ? crp : nullptr, n,
+                                                      max_h, d_bins);
+        LAL_LAUNCHED();
+        LAL_HIP(hipMemcpyAsync(c->h_flag, d_bins, sizeof(int) * nb, hipMemcpyDeviceToHost,
+                               c->stream));
+        LAL_HIP(hipStreamSynchronize(c->stream));
+        long long total = 0;
         for(int b = 0; b < kBins; b++)
         {
-            P.bin_start[b] = (int)P.rows.size();
-            P.rows.insert(P.rows.end(), bins[b].begin(), bins[b].end());
+            P.bin_start[b] = (int)total;
+            total += c->h_flag[b];
         }
-        P.bin_start[kBins] = (int)P.rows.size();
-        P.n_lds            = (int)P.rows.size();
-        P.rows.insert(P.rows.end(), bins[kBins].begin(), bins[kBins].end());
-        P.n_global = (int)bins[kBins].size();
-        P.toff.assign(P.n_global + 1, 0);
-        for(int r = 0; r < P.n_global; r++)
-            P.toff[r + 1] = P.toff[r] + table_size(prod[bins[kBins][r]], n);
-
-        if(!P.rows.empty())
+        P.bin_start[kBins] = (int)total;
+        P.n_lds            = (int)total;
+        P.n_global         = c->h_flag[kBins];
+        total += P.n_global;
+        if(total)
         {
-            LAL_HIP(hipMallocAsync((void**)&P.d_rows, sizeof(int) * P.rows.size(), c->stream));
-            LAL_HIP(hipMemcpyAsync(P.d_rows, P.rows.data(), sizeof(int) * P.rows.size(),
-                                   hipMemcpyHostToDevice, c->stream));
+            LAL_HIP(hipMallocAsync((void**)&P.d_rows, sizeof(int) * total, c->stream));
+            if(P.n_global)
+                LAL_HIP(hipMallocAsync((void**)&d_gsl, sizeof(long long) * P.n_global,
+                                       c->stream));
+            bin_scatter_kernel<<<grid, 256, 0, c->stream>>>(m, d_prod, numeric ? crp : nullptr, n,
+                                                            max_h, d_bins, d_bins + nb, P.d_rows,
+                                                            d_gsl);
+            LAL_LAUNCHED();
         }
         if(P.n_global)
         {
-            long long slots = P.toff[P.n_global];
+            P.toff.assign(P.n_global + 1, 0);
+            LAL_HIP(hipMemcpyAsync(P.toff.data() + 1, d_gsl, sizeof(long long) * P.n_global,
+                                   hipMemcpyDeviceToHost, c->stream));
+            LAL_HIP(hipStreamSynchronize(c->stream));
+            for(int r = 0; r < P.n_global; r++)
+                P.toff[r + 1] += P.toff[r];
+            const long long slots = P.toff[P.n_global];
             LAL_HIP(hipMallocAsync((void**)&P.d_toff, sizeof(long long) * P.toff.size(),
                                    c->stream));
             LAL_HIP(hipMemcpyAsync(P.d_toff, P.toff.data(), sizeof(long long) * P.toff.size(),
@@ -414,9 +631,14 @@ namespace
             LAL_HIP(hipMallocAsync((void**)&P.d_keys, sizeof(int) * slots, c->stream));
             if(numeric)
                 LAL_HIP(hipMallocAsync((void**)&P.d_vals, sizeof(double) * slots, c->stream));
+            // P.toff must outlive the copy to the device
+            LAL_HIP(hipStreamSynchronize(c->stream));
         }
-        // the host vectors above must outlive the async copies
-        LAL_HIP(hipStreamSynchronize(c->stream));
+        if(d_prod)
+            LAL_HIP(hipFreeAsync(d_prod, c->stream));
+        if(d_gsl)
+            LAL_HIP(hipFreeAsync(d_gsl, c->stream));
+        LAL_HIP(hipFreeAsync(d_bins, c->stream));
         return LAL_OK;
     }
 
@@ -434,13 +656,28 @@ namespace
 #define LAL_SPGEMM_BIN(HH, WPB, NUM)                                                         \
     spgemm_lds_kernel<HH, WPB, NUM><<<(nr + WPB - 1) / WPB, WPB * 64, 0, c->stream>>>(       \
         nr, rows, arp, aci, av, brp, bci, bv, crp, cout_, cval, c->d_flag)
+#define LAL_SPGEMM_GROUP(HH, G)                                                              \
+    spgemm_group_kernel<HH, G, NUMERIC><<<(nr + 256 / G - 1) / (256 / G), 256, 0, c->stream>>>( \
+        nr, rows, arp, aci, av, brp, bci, bv, crp, cout_, cval, c->d_flag)
             switch(b)
             {
-            case 0: LAL_SPGEMM_BIN(64, 4, NUMERIC); break;
-            case 1: LAL_SPGEMM_BIN(128, 4, NUMERIC); break;
-            case 2: LAL_SPGEMM_BIN(256, 4, NUMERIC); break;
-            case 3: LAL_SPGEMM_BIN(512, 4, NUMERIC); break;
-            case 4: LAL_SPGEMM_BIN(1024, 4, NUMERIC); break;
+            case 0: LAL_SPGEMM_GROUP(64, 8); break;
+            case 1: LAL_SPGEMM_GROUP(128, 16); break;
+            case 2: LAL_SPGEMM_GROUP(256, 32); break;
+            // key-only tables of 512 and 1024 slots (products, not distinct
+            // columns, size them) in 32-lane groups; 8 rows hold 16 or 32 KB
+            case 3:
+                if constexpr(NUMERIC)
+                    LAL_SPGEMM_BIN(512, 4, true);
+                else
+                    LAL_SPGEMM_GROUP(512, 32);
+                break;
+            case 4:
+                if constexpr(NUMERIC)
+                    LAL_SPGEMM_BIN(1024, 4, true);
+                else
+                    LAL_SPGEMM_GROUP(1024, 32);
+                break;
             case 5: LAL_SPGEMM_BIN(2048, 1, NUMERIC); break;
             case 6: LAL_SPGEMM_BIN(4096, 1, NUMERIC); break;
             // key-only tables of the symbolic pass (the numeric pass never bins here)
@@ -449,6 +686,7 @@ namespace
             case 9: LAL_SPGEMM_BIN(32768, 1, false); break;
             }
 #undef LAL_SPGEMM_BIN
+#undef LAL_SPGEMM_GROUP
             LAL_LAUNCHED();
         }
         if(P.n_global)
