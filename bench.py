@@ -61,6 +61,9 @@ def parse():
     ap.add_argument("--hbm-ceiling", action="store_true",
                     help="only time a 2 GiB streaming reduction and a 1 GiB copy (torch's "
                          "kernels): the HBM bandwidth the SpMV fraction is set against")
+    ap.add_argument("--setup-only", action="store_true",
+                    help="only build the SA-AMG hierarchy of the --nx Poisson problem "
+                         "--steps times (after --warmup builds) and print the setup times")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     return ap.parse_args()
@@ -593,6 +596,27 @@ def main():
     V = torch.from_numpy(v).to(dev)
     B = torch.from_numpy(b).to(dev)
     torch.cuda.synchronize()
+
+    if args.setup_only:
+        # hierarchy builds alone, each timed between two synchronisations
+        times = []
+        for k in range(args.warmup + args.steps):
+            ts = time.perf_counter()
+            Hk = lal.Hierarchy(h, m, nnz, RP, CI, V)
+            torch.cuda.synchronize()
+            if k >= args.warmup:
+                times.append(time.perf_counter() - ts)
+            levels = Hk.levels
+            Hk.close()
+            torch.cuda.synchronize()
+            print(f"# setup {k}: {(time.perf_counter() - ts) * 1e3:.2f} ms with destroy",
+                  file=sys.stderr, flush=True)
+        print(json.dumps({"metric": "saamg_setup_s", "workload": f"3D Poisson {nx}^3",
+                          "n": m, "nnz": nnz, "levels": levels,
+                          "setup_s": [round(t, 4) for t in times],
+                          "setup_s_median": round(sorted(times)[len(times) // 2], 4)}),
+              flush=True)
+        return
 
     # ---------------------------------------------------------------- SpMV
     def spmv_loop(dtype, calls):
