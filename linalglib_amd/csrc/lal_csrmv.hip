@@ -238,9 +238,10 @@ namespace lal
     }
 
     int csrmv_analysis_dev(lal_context* c, int m, int n, int nnz, const int* rp,
-                           const int* ci, int exact, lal_csrmv_info** out)
+                           const int* ci, int exact, lal_csrmv_info** out, bool trusted)
     {
-        LAL_TRY(check_columns(c, nnz, n, ci));
+        if(!trusted)
+            LAL_TRY(check_columns(c, nnz, n, ci));
         std::vector<int> h(m + 1);
         LAL_HIP(hipMemcpyAsync(h.data(), rp, sizeof(int) * (m + 1), hipMemcpyDeviceToHost,
                                c->stream));
@@ -299,18 +300,15 @@ namespace lal
             return LAL_OK;
         unsigned short* c16   = nullptr;
         int*            cbase = nullptr;
-        int*            fail  = nullptr;
-        int             h_fail = 1;
+        int*            fail  = c->d_flag; // the context's flag word, read back below
         auto release = [&]() {
             (void)hipFree(c16);
             (void)hipFree(cbase);
-            (void)hipFree(fail);
         };
         // rounded up to whole 16-byte lines: the kernels load four entries at a time
         if(hipMalloc(&c16, ((size_t)nnz * sizeof(unsigned short) + 15) & ~(size_t)15) !=
                hipSuccess ||
-           hipMalloc(&cbase, sizeof(int) * d->nblocks) != hipSuccess ||
-           hipMalloc(&fail, sizeof(int)) != hipSuccess)
+           hipMalloc(&cbase, sizeof(int) * d->nblocks) != hipSuccess)
         {
             release();
             return LAL_ERR_OOM;
@@ -321,13 +319,12 @@ namespace lal
             c16_kernel<<<d->nblocks, 256, 0, c->stream>>>(d->d_blocks, d->nblocks, ci, cbase, c16,
                                                          fail);
             if(hipGetLastError() != hipSuccess ||
-               hipMemcpyAsync(&h_fail, fail, sizeof(int), hipMemcpyDeviceToHost, c->stream) !=
+               hipMemcpyAsync(c->h_flag, fail, sizeof(int), hipMemcpyDeviceToHost, c->stream) !=
                    hipSuccess ||
                hipStreamSynchronize(c->stream) != hipSuccess)
                 rc = LAL_ERR_HIP;
         }
-        (void)hipFree(fail);
-        fail = nullptr;
+        const int h_fail = rc == LAL_OK ? c->h_flag[0] : 1;
         if(rc != LAL_OK || h_fail)
         {
             release();
@@ -387,7 +384,7 @@ extern "C" int lal_csrmv_analysis(lal_handle c, int m, int n, int nnz, const int
     LAL_REQUIRE(c != nullptr && descr != nullptr && m >= 0 && n >= 0 && nnz >= 0);
     LAL_REQUIRE(row_ptr != nullptr && (nnz == 0 || col_ind != nullptr));
     *descr = nullptr;
-    return lal::csrmv_analysis_dev(c, m, n, nnz, row_ptr, col_ind, exact, descr);
+    return lal::csrmv_analysis_dev(c, m, n, nnz, row_ptr, col_ind, exact, descr, false);
 }
 
 extern "C" int lal_csrmv_descr_destroy(lal_csrmv_descr d)

@@ -267,8 +267,12 @@ namespace
         for(int l = 0; l <= h->levels; l++)
         {
             level_t& L = h->L[l];
-            LAL_TRY(lal::csrmv_analysis_dev(c, L.A.m, L.A.n, L.A.nnz, L.A.rp, L.A.ci, exact,
-                                            &L.A.d));
+            // level 0 carries the descriptor of the input's validation (the
+            // same row pointer, copied); the rest are this setup's products
+            if(!L.A.d)
+                LAL_TRY(lal::csrmv_analysis_dev(c, L.A.m, L.A.n, L.A.nnz, L.A.rp, L.A.ci, exact,
+                                                &L.A.d, true));
+            L.A.d->exact = exact;
             // The finest operator is read three times per PCG iteration
             // (residual, post-sweep, the Krylov product), the last two with
             // only vector passes between them: it may stay in the Infinity
@@ -291,9 +295,9 @@ namespace
             if(l < h->levels)
             {
                 LAL_TRY(lal::csrmv_analysis_dev(c, L.P.m, L.P.n, L.P.nnz, L.P.rp, L.P.ci, exact,
-                                                &L.P.d));
+                                                &L.P.d, true));
                 LAL_TRY(lal::csrmv_analysis_dev(c, L.R.m, L.R.n, L.R.nnz, L.R.rp, L.R.ci, exact,
-                                                &L.R.d));
+                                                &L.R.d, true));
                 if(!exact)
                 {
                     LAL_TRY(lal::compact_columns(c, L.P.d, L.P.nnz, L.P.ci));
@@ -1065,13 +1069,13 @@ namespace
         if(opts)
             o = *opts;
         LAL_REQUIRE(o.max_levels >= 0 && o.eps >= 0.0);
-        // validate the input pattern before any setup kernel gathers through it
-        {
-            lal_csrmv_info* d = nullptr;
-            LAL_TRY(lal::csrmv_analysis_dev(c, m, n, nnz, rp, ci, 0, &d));
-            lal_csrmv_descr_destroy(d);
-        }
-        auto* h = new lal_hierarchy_t;
+        // validate the input pattern before any setup kernel gathers through
+        // it; the analysis describes the finest operator (a copy of the same
+        // pattern), so the hierarchy keeps it for level 0
+        lal_csrmv_info* d0 = nullptr;
+        LAL_TRY(lal::csrmv_analysis_dev(c, m, n, nnz, rp, ci, 0, &d0));
+        auto* h      = new lal_hierarchy_t;
+        h->L[0].A.d  = d0;
         int   s = setup(c, m, n, nnz, rp, ci, v, o, h, unsmoothed);
         if(s != LAL_OK)
         {

@@ -30,8 +30,10 @@ namespace lal
     int residual_f32(lal_context* c, const lal_csrmv_info* d, int m, int n, int nnz,
                      const int* rp, const int* ci, const float* v, const float* x,
                      const float* b, float* r);
+    // trusted: the pattern was built by this library (a hierarchy operator),
+    // so the column-range check is skipped
     int csrmv_analysis_dev(lal_context* c, int m, int n, int nnz, const int* rp, const int* ci,
-                           int exact, lal_csrmv_info** out);
+                           int exact, lal_csrmv_info** out, bool trusted = false);
 
     inline int csrmv(lal_context* c, const lal_csrmv_info* d, int m, int n, int nnz,
                      double alpha, const int* rp, const int* ci, const double* v,
