@@ -401,6 +401,237 @@ namespace
     constexpr long PROLONG_WAVE_MIN_ROW = 24;
 }
 
+namespace
+{
+    // Strength, the diagonal and the PMIS passes with G lanes per row (G = 16
+    // or 64, 256 / G rows per workgroup) for the long rows of the coarse
+    // Galerkin levels, where a lane walking a row of 40-190 entries alone is
+    // a chain of dependent loads.  Each lane takes the row's entries
+    // j = rs + gl, rs + gl + G, ... in order; every per-row result the
+    // thread-per-row kernels form in CSR order -- the first (or last) match,
+    // a lexicographic maximum that keeps its first occurrence -- is the
+    // group's reduction with the entry position as the tie-break, so the
+    // results are bit for bit the same.
+    template <int G>
+    __device__ __forceinline__ int group_row(int& gl)
+    {
+        gl = (int)threadIdx.x % G;
+        return blockIdx.x * (256 / G) + (int)threadIdx.x / G;
+    }
+
+    template <int G>
+    __device__ __forceinline__ int group_min(int v)
+    {
+#pragma unroll
+        for(int o = G / 2; o > 0; o >>= 1)
+            v = min(v, __shfl_xor(v, o, 64));
+        return v;
+    }
+
+    template <int G>
+    __device__ __forceinline__ int group_max(int v)
+    {
+#pragma unroll
+        for(int o = G / 2; o > 0; o >>= 1)
+            v = max(v, __shfl_xor(v, o, 64));
+        return v;
+    }
+
+    // d[i] = the row's first diagonal entry (0 when it has none)
+    template <int G>
+    __global__ void __launch_bounds__(256)
+        diagonal_group_kernel(int m, const int* __restrict__ rp, const int* __restrict__ ci,
+                              const double* __restrict__ val, double* __restrict__ d)
+    {
+        int       gl;
+        const int i = group_row<G>(gl);
+        if(i >= m)
+            return;
+        const int rs = rp[i], re = rp[i + 1];
+        int       first = INT_MAX;
+        for(int j = rs + gl; j < re; j += G)
+            if(ci[j] == i)
+            {
+                first = j;
+                break;
+            }
+        first = group_min<G>(first);
+        if(gl == 0 && first != INT_MAX)
+            d[i] = val[first];
+    }
+
+    template <int G>
+    __global__ void __launch_bounds__(256)
+        strength_group_kernel(int m, const int* __restrict__ rp, const int* __restrict__ ci,
+                              const double* __restrict__ val, const double* __restrict__ d,
+                              double eps, int* __restrict__ conn)
+    {
+        int       gl;
+        const int i = group_row<G>(gl);
+        if(i >= m)
+            return;
+        const double di = fabs(d[i]);
+        for(int j = rp[i] + gl; j < rp[i + 1]; j += G)
+        {
+            int    c = ci[j];
+            double v = val[j];
+            conn[j]  = (c != i) && (fabs(v) >= eps * sqrt(di * fabs(d[c])));
+        }
+    }
+
+    template <int G>
+    __global__ void __launch_bounds__(256)
+        pmis_init_group_kernel(int m, const int* __restrict__ rp, const int* __restrict__ conn,
+                               int* __restrict__ state, int* __restrict__ hash)
+    {
+        int       gl;
+        const int i = group_row<G>(gl);
+        if(i >= m)
+            return;
+        int strong = 0;
+        for(int j = rp[i] + gl; j < rp[i + 1] && !strong; j += G)
+            strong = conn[j] == 1;
+        strong = group_max<G>(strong);
+        if(gl == 0)
+        {
+            state[i] = strong ? 0 : -2;
+            hash[i]  = hash1((unsigned)i);
+        }
+    }
+
+    // A PMIS candidate: state, hash, node, and its position in the scan.
+    struct pmis_cand
+    {
+        int s, h, r, pos;
+    };
+
+    // b replaces a when (s, h) is lexicographically greater, or equal and
+    // earlier in the scan.
+    __device__ __forceinline__ void pmis_take(pmis_cand& a, const pmis_cand& b)
+    {
+        if(b.s > a.s || (b.s == a.s && (b.h > a.h || (b.h == a.h && b.pos < a.pos))))
+            a = b;
+    }
+
+    // The best strong neighbour of row `row` across the group: each lane keeps
+    // the first maximum of its entries, the group the earliest of the lanes'.
+    template <int G>
+    __device__ __forceinline__ pmis_cand pmis_scan(int row, int gl, const int* __restrict__ rp,
+                                                   const int* __restrict__ ci,
+                                                   const int* __restrict__ conn,
+                                                   const int* __restrict__ state,
+                                                   const int* __restrict__ hash)
+    {
+        pmis_cand best{INT_MIN, INT_MIN, -1, INT_MAX};
+        const int rs = rp[row], re = rp[row + 1];
+        for(int j = rs + gl; j < re; j += G)
+            if(conn[j] == 1)
+            {
+                const int c = ci[j];
+                const int sc = state[c], hc = hash[c];
+                if(sc > best.s || (sc == best.s && hc > best.h))
+                    best = pmis_cand{sc, hc, c, j - rs};
+            }
+#pragma unroll
+        for(int o = G / 2; o > 0; o >>= 1)
+        {
+            const pmis_cand other{__shfl_xor(best.s, o, 64), __shfl_xor(best.h, o, 64),
+                                  __shfl_xor(best.r, o, 64), __shfl_xor(best.pos, o, 64)};
+            pmis_take(best, other);
+        }
+        return best;
+    }
+
+    // pmis_round_kernel with G lanes per row: the running maximum starts at
+    // the row itself, takes the row's strong neighbours in order, then the
+    // strong neighbours of the row that maximum names.
+    template <int G>
+    __global__ void __launch_bounds__(256)
+        pmis_round_group_kernel(int m, const int* __restrict__ rp, const int* __restrict__ ci,
+                                const int* __restrict__ conn, const int* __restrict__ state,
+                                const int* __restrict__ hash, int64_t* __restrict__ agg,
+                                int* __restrict__ max_state, int* __restrict__ incomplete)
+    {
+        int       gl;
+        const int i = group_row<G>(gl);
+        if(i >= m)
+            return;
+        const int si = state[i];
+        if(si != 0)
+            return; // only undecided rows change anything
+        // the row itself precedes its neighbours, so it wins a tie
+        pmis_cand run{si, hash[i], i, -1};
+        pmis_cand c1 = pmis_scan<G>(i, gl, rp, ci, conn, state, hash);
+        if(c1.s > run.s || (c1.s == run.s && c1.h > run.h))
+            run = c1;
+        // the second row's entries all follow the first scan's
+        pmis_cand c2 = pmis_scan<G>(run.r, gl, rp, ci, conn, state, hash);
+        if(c2.s > run.s || (c2.s == run.s && c2.h > run.h))
+            run = c2;
+        if(gl != 0)
+            return;
+        if(run.r == i)
+        {
+            max_state[i] = 1;
+            agg[i]       = 1;
+        }
+        else if(run.s == 1)
+        {
+            max_state[i] = -1;
+            agg[i]       = 0;
+        }
+        else
+            *incomplete = 1;
+    }
+
+    template <int G>
+    __global__ void __launch_bounds__(256)
+        pmis_join_group_kernel(int m, const int* __restrict__ rp, const int* __restrict__ ci,
+                               const int* __restrict__ conn, const int* __restrict__ state,
+                               int64_t* __restrict__ agg, int* __restrict__ max_state)
+    {
+        int       gl;
+        const int i = group_row<G>(gl);
+        if(i >= m)
+            return;
+        const int si = state[i];
+        if(si == -1)
+        {
+            // the first strong neighbour (in CSR order) with state 1
+            int first = INT_MAX;
+            for(int j = rp[i] + gl; j < rp[i + 1]; j += G)
+                if(conn[j] == 1 && state[ci[j]] == 1)
+                {
+                    first = j;
+                    break;
+                }
+            first = group_min<G>(first);
+            if(gl == 0 && first != INT_MAX)
+            {
+                // c has state 1 in the snapshot, so its aggregate is not
+                // written during this pass
+                agg[i]       = agg[ci[first]];
+                max_state[i] = 1;
+            }
+        }
+        else if(si == -2 && gl == 0)
+            agg[i] = -2;
+    }
+
+    // Lanes per row for the row-walking setup passes: one for short rows (the
+    // finest level's stencil), 16 or 64 for the coarse Galerkin levels.
+    inline int setup_row_lanes(long nnz, int m)
+    {
+        const long avg = m > 0 ? nnz / m : 0;
+        return avg >= 48 ? 64 : avg >= 12 ? 16 : 1;
+    }
+
+    inline int group_grid(int m, int lanes)
+    {
+        return (int)(((long)m * lanes + 255) / 256);
+    }
+}
+
 namespace lal
 {
     int diagonal_f64(lal_context* c, int m, const int* rp, const int* ci, const double* v,
@@ -414,11 +645,28 @@ namespace lal
         double* d = nullptr;
         LAL_HIP(hipMallocAsync((void**)&d, sizeof(double) * m, c->stream));
         LAL_HIP(hipMemsetAsync(d, 0, sizeof(double) * m, c->stream));
-        LAL_TRY(diagonal_f64(c, m, rp, ci, v, d));
-        strength_kernel<<<grid_for(m, 256), 256, 0, c->stream>>>(m, rp, ci, v, d, eps, conn);
+        const int lanes = setup_row_lanes(nnz, m);
+        if(lanes == 64)
+        {
+            diagonal_group_kernel<64><<<group_grid(m, 64), 256, 0, c->stream>>>(m, rp, ci, v, d);
+            LAL_LAUNCHED();
+            strength_group_kernel<64>
+                <<<group_grid(m, 64), 256, 0, c->stream>>>(m, rp, ci, v, d, eps, conn);
+        }
+        else if(lanes == 16)
+        {
+            diagonal_group_kernel<16><<<group_grid(m, 16), 256, 0, c->stream>>>(m, rp, ci, v, d);
+            LAL_LAUNCHED();
+            strength_group_kernel<16>
+                <<<group_grid(m, 16), 256, 0, c->stream>>>(m, rp, ci, v, d, eps, conn);
+        }
+        else
+        {
+            LAL_TRY(diagonal_f64(c, m, rp, ci, v, d));
+            strength_kernel<<<grid_for(m, 256), 256, 0, c->stream>>>(m, rp, ci, v, d, eps, conn);
+        }
         LAL_LAUNCHED();
         LAL_HIP(hipFreeAsync(d, c->stream));
-        (void)nnz;
         return LAL_OK;
     }
 
@@ -432,16 +680,30 @@ namespace lal
         LAL_HIP(hipMallocAsync((void**)&max_state, sizeof(int) * m, c->stream));
         LAL_HIP(hipMallocAsync((void**)&hash, sizeof(int) * m, c->stream));
         LAL_HIP(hipMemsetAsync(agg, 0, sizeof(int64_t) * m, c->stream));
-        const int g = grid_for(m, 256);
-        pmis_init_kernel<<<g, 256, 0, c->stream>>>(m, rp, conn, max_state, hash);
+        const int g     = grid_for(m, 256);
+        const int lanes = setup_row_lanes(nnz, m);
+        const int gg    = group_grid(m, lanes);
+        if(lanes == 64)
+            pmis_init_group_kernel<64><<<gg, 256, 0, c->stream>>>(m, rp, conn, max_state, hash);
+        else if(lanes == 16)
+            pmis_init_group_kernel<16><<<gg, 256, 0, c->stream>>>(m, rp, conn, max_state, hash);
+        else
+            pmis_init_kernel<<<g, 256, 0, c->stream>>>(m, rp, conn, max_state, hash);
         LAL_LAUNCHED();
         for(int iter = 0; iter < 20; iter++)
         {
             LAL_HIP(hipMemcpyAsync(state, max_state, sizeof(int) * m, hipMemcpyDeviceToDevice,
                                    c->stream));
             LAL_HIP(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
-            pmis_round_kernel<<<g, 256, 0, c->stream>>>(m, rp, ci, conn, state, hash, agg,
-                                                        max_state, c->d_flag);
+            if(lanes == 64)
+                pmis_round_group_kernel<64><<<gg, 256, 0, c->stream>>>(
+                    m, rp, ci, conn, state, hash, agg, max_state, c->d_flag);
+            else if(lanes == 16)
+                pmis_round_group_kernel<16><<<gg, 256, 0, c->stream>>>(
+                    m, rp, ci, conn, state, hash, agg, max_state, c->d_flag);
+            else
+                pmis_round_kernel<<<g, 256, 0, c->stream>>>(m, rp, ci, conn, state, hash, agg,
+                                                            max_state, c->d_flag);
             LAL_LAUNCHED();
             LAL_HIP(hipMemcpyAsync(c->h_flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost,
                                    c->stream));
@@ -454,13 +716,20 @@ namespace lal
         {
             LAL_HIP(hipMemcpyAsync(state, max_state, sizeof(int) * m, hipMemcpyDeviceToDevice,
                                    c->stream));
-            pmis_join_kernel<<<g, 256, 0, c->stream>>>(m, rp, ci, conn, state, agg, max_state);
+            if(lanes == 64)
+                pmis_join_group_kernel<64>
+                    <<<gg, 256, 0, c->stream>>>(m, rp, ci, conn, state, agg, max_state);
+            else if(lanes == 16)
+                pmis_join_group_kernel<16>
+                    <<<gg, 256, 0, c->stream>>>(m, rp, ci, conn, state, agg, max_state);
+            else
+                pmis_join_kernel<<<g, 256, 0, c->stream>>>(m, rp, ci, conn, state, agg,
+                                                           max_state);
             LAL_LAUNCHED();
         }
         LAL_HIP(hipFreeAsync(state, c->stream));
         LAL_HIP(hipFreeAsync(max_state, c->stream));
         LAL_HIP(hipFreeAsync(hash, c->stream));
-        (void)nnz;
         return LAL_OK;
     }
 

@@ -291,7 +291,7 @@ namespace
             LAL_TRY(alloc_n(&L.w64.diag, L.A.m));
             LAL_TRY(alloc_n(&L.w64.dinv, L.A.m));
             LAL_TRY(lal::diag_inv<double>(c, L.A.m, L.A.rp, L.A.ci, L.A.v, L.w64.diag,
-                                          L.w64.dinv));
+                                          L.w64.dinv, L.A.nnz));
             if(l < h->levels)
             {
                 LAL_TRY(lal::csrmv_analysis_dev(c, L.P.m, L.P.n, L.P.nnz, L.P.rp, L.P.ci, exact,
@@ -319,7 +319,7 @@ namespace
                 LAL_TRY(alloc_n(&L.w32.diag, L.A.m));
                 LAL_TRY(alloc_n(&L.w32.dinv, L.A.m));
                 LAL_TRY(lal::diag_inv<float>(c, L.A.m, L.A.rp, L.A.ci, L.A.v32, L.w32.diag,
-                                             L.w32.dinv));
+                                             L.w32.dinv, L.A.nnz));
             }
         }
         // coarsest level

@@ -61,7 +61,9 @@ namespace lal
                          T* x, T* r);
     // d = the last diagonal entry of each row, dinv = 1/d.
     template <typename T>
-    int diag_inv(lal_context* c, int m, const int* rp, const int* ci, const T* v, T* d, T* dinv);
+    // nnz: the operator's nonzeros, which pick a lane or a wavefront per row
+    int diag_inv(lal_context* c, int m, const int* rp, const int* ci, const T* v, T* d, T* dinv,
+                 long nnz = 0);
     template <typename T>
     int coarse_scalar(lal_context* c, const T* v, const T* b, T* x);
     template <typename T>

@@ -129,6 +129,30 @@ namespace
         dinv[i] = (T)1 / a;
     }
 
+    // diag_kernel with 64 lanes per row, for long rows: the row's last
+    // diagonal entry is the group's largest matching position.
+    template <typename T>
+    __global__ void __launch_bounds__(256)
+        diag_wave_kernel(int m, const int* __restrict__ rp, const int* __restrict__ ci,
+                         const T* __restrict__ val, T* __restrict__ d, T* __restrict__ dinv)
+    {
+        const int lane = (int)threadIdx.x & 63;
+        const int i    = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+        if(i >= m)
+            return;
+        int last = -1;
+        for(int j = rp[i] + lane; j < rp[i + 1]; j += 64)
+            if(ci[j] == i)
+                last = j;
+        last = lal::wave_max(last);
+        if(lane == 0)
+        {
+            const T a = last >= 0 ? val[last] : (T)0;
+            d[i]      = a;
+            dinv[i]   = (T)1 / a;
+        }
+    }
+
     // Coarsest level with one unknown: x = b / a (amg.cpp:211-216).
     template <typename T>
     __global__ void scalar_solve_kernel(const T* __restrict__ val, const T* __restrict__ b,
@@ -506,11 +530,17 @@ namespace lal
     }
 
     template <typename T>
-    int diag_inv(lal_context* c, int m, const int* rp, const int* ci, const T* v, T* d, T* dinv)
+    int diag_inv(lal_context* c, int m, const int* rp, const int* ci, const T* v, T* d, T* dinv,
+                 long nnz)
     {
         if(m == 0)
             return LAL_OK;
-        diag_kernel<T><<<grid_for(m, 256), 256, 0, c->stream>>>(m, rp, ci, v, d, dinv);
+        // a lane per row walks the row alone: rows of 24 or more entries take
+        // a wavefront each
+        if(nnz >= 24L * m)
+            diag_wave_kernel<T><<<(m + 3) / 4, 256, 0, c->stream>>>(m, rp, ci, v, d, dinv);
+        else
+            diag_kernel<T><<<grid_for(m, 256), 256, 0, c->stream>>>(m, rp, ci, v, d, dinv);
         LAL_LAUNCHED();
         return LAL_OK;
     }
@@ -651,7 +681,8 @@ namespace lal
     template int jacobi0_residual<T>(lal_context*, const lal_csrmv_info*, int, const int*,   \
                                      const int*, const T*, const T*, const T*, const T*, T*, \
                                      T*);                                                    \
-    template int diag_inv<T>(lal_context*, int, const int*, const int*, const T*, T*, T*);   \
+    template int diag_inv<T>(lal_context*, int, const int*, const int*, const T*, T*, T*,   \
+                             long);                                                          \
     template int coarse_scalar<T>(lal_context*, const T*, const T*, T*);                     \
     template int coarse_gs<T>(lal_context*, int, const int*, const int*, const T*, const T*, \
                               T*);                                                           \
