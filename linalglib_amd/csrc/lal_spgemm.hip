@@ -326,15 +326,34 @@ namespace
                 be = brp[ca + 1];
             }
             const int cnt = min(G, re - j0);
+            // each lane's first entry of step t's B row is loaded during step
+            // t - 1, so the step's inserts do not wait on it
+            int    nbs = __shfl(bs, gb, 64), nbe = __shfl(be, gb, 64);
+            double nva  = NUMERIC ? __shfl(va, gb, 64) : 0.0;
+            int    ncol = nbs + gl < nbe ? bci[nbs + gl] : -1;
+            double nbv  = NUMERIC && nbs + gl < nbe ? bv[nbs + gl] : 0.0;
             for(int t = 0; t < cnt; t++)
             {
-                const int    tbs = __shfl(bs, gb + t, 64);
-                const int    tbe = __shfl(be, gb + t, 64);
-                const double tva = NUMERIC ? __shfl(va, gb + t, 64) : 0.0;
+                const int    tbs = nbs, tbe = nbe;
+                const double tva = nva;
+                const int    col0 = ncol;
+                const double bv0  = nbv;
+                if(t + 1 < cnt)
+                {
+                    nbs = __shfl(bs, gb + t + 1, 64);
+                    nbe = __shfl(be, gb + t + 1, 64);
+                    if(NUMERIC)
+                        nva = __shfl(va, gb + t + 1, 64);
+                    ncol = nbs + gl < nbe ? bci[nbs + gl] : -1;
+                    if(NUMERIC)
+                        nbv = nbs + gl < nbe ? bv[nbs + gl] : 0.0;
+                }
                 for(int p = tbs + gl; p < tbe; p += G)
                 {
-                    const int col = bci[p];
-                    int       h   = slot_of(col, hmask);
+                    const bool   first = p == tbs + gl;
+                    const int    col   = first ? col0 : bci[p];
+                    const double bp    = NUMERIC ? (first ? bv0 : bv[p]) : 0.0;
+                    int          h     = slot_of(col, hmask);
                     for(int probe = 0;; probe++)
                     {
                         if(probe == H)
@@ -347,13 +366,13 @@ namespace
                         {
                             local++;
                             if(NUMERIC)
-                                vals[h] = tva * bv[p];
+                                vals[h] = tva * bp;
                             break;
                         }
                         if(old == col)
                         {
                             if(NUMERIC)
-                                vals[h] += tva * bv[p];
+                                vals[h] += tva * bp;
                             break;
                         }
                         h = (h + 1) & hmask;
