@@ -1,9 +1,10 @@
 // SA-AMG setup kernels: strength of connection, PMIS aggregation and the
 // smoothed prolongator.
 //
-// Every kernel gives one row to one lane and walks that row in CSR order, so
-// each decision and each floating-point value is formed by the same
-// operations in the same order as the host path:
+// Each decision and each floating-point value is formed by the same
+// operations in the same order as the host path -- a lane walking its row in
+// CSR order, or a group of lanes whose reductions keep the CSR order (entry
+// positions as tie-breaks, folds taken lane by lane in entry order):
 //   strength     backend/host/host_amg_strength.cpp:33-70
 //   PMIS         backend/host/host_amg_aggregation.cpp:32-221 driven as
 //                iterative_solvers/amg/amg_aggregation.cpp:106-215
@@ -145,115 +146,24 @@ namespace
     {
         return c == i || cn == 1;
     }
-
-    // Distinct aggregates >= 0 among the diagonal and the strong neighbours.
-    __global__ void __launch_bounds__(256)
-        prolong_count_kernel(int m, const int* __restrict__ rp, const int* __restrict__ ci,
-                             const int* __restrict__ conn, const int64_t* __restrict__ agg,
-                             int* __restrict__ count)
-    {
-        int i = blockIdx.x * 256 + threadIdx.x;
-        if(i > m)
-            return;
-        if(i == m)
-        {
-            count[m] = 0;
-            return;
-        }
-        int       k  = 0;
-        const int rs = rp[i], re = rp[i + 1];
-        for(int j = rs; j < re; j++)
-        {
-            int c = ci[j];
-            if(!p_candidate(i, c, conn[j]))
-                continue;
-            int64_t a = agg[c];
-            if(a < 0)
-                continue;
-            bool seen = false;
-            for(int t = rs; t < j && !seen; t++)
-            {
-                int ct = ci[t];
-                seen   = p_candidate(i, ct, conn[t]) && agg[ct] == a;
-            }
-            k += !seen;
-        }
-        count[i] = k;
-    }
-
-    // Columns ascending; each column's value is the CSR-order fold of its
-    // contributions starting from the first one.
-    __global__ void __launch_bounds__(256)
-        prolong_fill_kernel(int m, const int* __restrict__ rp, const int* __restrict__ ci,
-                            const double* __restrict__ val, const int* __restrict__ conn,
-                            const int64_t* __restrict__ agg, double relax,
-                            const int* __restrict__ prp, int* __restrict__ pci,
-                            double* __restrict__ pv)
-    {
-        int i = blockIdx.x * 256 + threadIdx.x;
-        if(i >= m)
-            return;
-        const int rs = rp[i], re = rp[i + 1];
-        double    diagonal = 0.0;
-        for(int j = rs; j < re; j++)
-        {
-            int c = ci[j];
-            if(c == i)
-                diagonal += val[j];
-            else if(conn[j] == 0)
-                diagonal -= val[j];
-        }
-        const double invd = 1.0 / diagonal;
-        int          out  = prp[i];
-        const int    end  = prp[i + 1];
-        int64_t      last = -1;
-        for(; out < end; out++)
-        {
-            int64_t best = LLONG_MAX;
-            for(int j = rs; j < re; j++)
-            {
-                int c = ci[j];
-                if(!p_candidate(i, c, conn[j]))
-                    continue;
-                int64_t a = agg[c];
-                if(a > last && a < best)
-                    best = a;
-            }
-            bool   first = true;
-            double acc   = 0.0;
-            for(int j = rs; j < re; j++)
-            {
-                int c = ci[j];
-                if(!p_candidate(i, c, conn[j]) || agg[c] != best)
-                    continue;
-                double value = (c == i) ? 1.0 - relax : -relax * invd * val[j];
-                if(first)
-                {
-                    acc   = value;
-                    first = false;
-                }
-                else
-                    acc += value;
-            }
-            pci[out] = (int)best;
-            pv[out]  = acc;
-            last     = best;
-        }
-    }
 }
 
 namespace
 {
-    // The same two kernels with one wavefront per row, for long rows (the
-    // coarse Galerkin levels): lanes read the row in coalesced chunks of 64,
-    // the next aggregate in ascending order is a wave-wide minimum, and every
-    // CSR-order fold (the weak-lumped diagonal, each column's sum of
-    // contributions) runs over the lanes in entry order, so the results are
-    // bit for bit those of the thread-per-row kernels.
-    __device__ __forceinline__ int64_t wave_min_i64(int64_t v)
+    // The prolongator P = (I - relax D^-1 A_F) T (saamg.cpp:51-207) with G lanes
+    // per row: a wavefront (G = 64) for the long rows of the coarse Galerkin
+    // levels, 8 lanes for the finest level's stencil rows.  Row i's entries
+    // are its candidates' distinct aggregates >= 0 (the diagonal and the
+    // strong neighbours) in ascending order, each the CSR-order fold of its
+    // contributions from the first one, with the weak couplings lumped into
+    // the diagonal.  Lanes read the row in coalesced chunks of G, the next
+    // aggregate in ascending order is a group-wide minimum, and every fold
+    // runs over the lanes in entry order.
+    template <int G>
+    __device__ __forceinline__ int64_t group_min_i64(int64_t v)
     {
 #pragma unroll
-        for(int o = 32; o > 0; o >>= 1)
+        for(int o = G / 2; o > 0; o >>= 1)
         {
             const int64_t t = __shfl_xor(v, o, 64);
             v               = t < v ? t : v;
@@ -262,15 +172,16 @@ namespace
     }
 
     // Smallest aggregate id above `last` among the row's candidates (LLONG_MAX
-    // when there is none); wave-uniform.
-    __device__ __forceinline__ int64_t next_aggregate(int i, int rs, int re, int lane,
+    // when there is none); group-uniform.
+    template <int G>
+    __device__ __forceinline__ int64_t next_aggregate(int i, int rs, int re, int gl,
                                                       const int* __restrict__ ci,
                                                       const int* __restrict__ conn,
                                                       const int64_t* __restrict__ agg,
                                                       int64_t last)
     {
         int64_t best = LLONG_MAX;
-        for(int j = rs + lane; j < re; j += 64)
+        for(int j = rs + gl; j < re; j += G)
         {
             const int c = ci[j];
             if(p_candidate(i, c, conn[j]))
@@ -280,21 +191,22 @@ namespace
                     best = a;
             }
         }
-        return wave_min_i64(best);
+        return group_min_i64<G>(best);
     }
 
+    template <int G>
     __global__ void __launch_bounds__(256)
-        prolong_count_wave_kernel(int m, const int* __restrict__ rp, const int* __restrict__ ci,
-                                  const int* __restrict__ conn, const int64_t* __restrict__ agg,
-                                  int* __restrict__ count)
+        prolong_count_group_kernel(int m, const int* __restrict__ rp, const int* __restrict__ ci,
+                                   const int* __restrict__ conn, const int64_t* __restrict__ agg,
+                                   int* __restrict__ count)
     {
-        const int lane = threadIdx.x & 63;
-        const int i    = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+        const int gl = (int)threadIdx.x % G;
+        const int i  = blockIdx.x * (256 / G) + (int)threadIdx.x / G;
         if(i > m)
             return;
         if(i == m)
         {
-            if(lane == 0)
+            if(gl == 0)
                 count[m] = 0;
             return;
         }
@@ -303,33 +215,35 @@ namespace
         int64_t   last = -1; // aggregates >= 0 only
         for(;;)
         {
-            const int64_t best = next_aggregate(i, rs, re, lane, ci, conn, agg, last);
+            const int64_t best = next_aggregate<G>(i, rs, re, gl, ci, conn, agg, last);
             if(best == LLONG_MAX)
                 break;
             k++;
             last = best;
         }
-        if(lane == 0)
+        if(gl == 0)
             count[i] = k;
     }
 
+    template <int G>
     __global__ void __launch_bounds__(256)
-        prolong_fill_wave_kernel(int m, const int* __restrict__ rp, const int* __restrict__ ci,
-                                 const double* __restrict__ val, const int* __restrict__ conn,
-                                 const int64_t* __restrict__ agg, double relax,
-                                 const int* __restrict__ prp, int* __restrict__ pci,
-                                 double* __restrict__ pv)
+        prolong_fill_group_kernel(int m, const int* __restrict__ rp, const int* __restrict__ ci,
+                                  const double* __restrict__ val, const int* __restrict__ conn,
+                                  const int64_t* __restrict__ agg, double relax,
+                                  const int* __restrict__ prp, int* __restrict__ pci,
+                                  double* __restrict__ pv)
     {
-        const int lane = threadIdx.x & 63;
-        const int i    = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+        const int gl = (int)threadIdx.x % G;
+        const int gb = ((int)threadIdx.x & 63) - gl; // the group's first lane
+        const int i  = blockIdx.x * (256 / G) + (int)threadIdx.x / G;
         if(i >= m)
             return;
         const int rs = rp[i], re = rp[i + 1];
         // a_ii minus the weak couplings, summed in CSR order
         double diagonal = 0.0;
-        for(int j0 = rs; j0 < re; j0 += 64)
+        for(int j0 = rs; j0 < re; j0 += G)
         {
-            const int j    = j0 + lane;
+            const int j    = j0 + gl;
             int       kind = 0;
             double    v    = 0.0;
             if(j < re)
@@ -341,11 +255,11 @@ namespace
                     kind = 2;
                 v = val[j];
             }
-            const int n = min(64, re - j0);
+            const int n = min(G, re - j0);
             for(int t = 0; t < n; t++)
             {
-                const int    kt = __shfl(kind, t, 64);
-                const double vt = __shfl(v, t, 64);
+                const int    kt = __shfl(kind, gb + t, 64);
+                const double vt = __shfl(v, gb + t, 64);
                 if(kt == 1)
                     diagonal += vt;
                 else if(kt == 2)
@@ -356,12 +270,12 @@ namespace
         int64_t      last = -1;
         for(int out = prp[i]; out < prp[i + 1]; out++)
         {
-            const int64_t best  = next_aggregate(i, rs, re, lane, ci, conn, agg, last);
+            const int64_t best  = next_aggregate<G>(i, rs, re, gl, ci, conn, agg, last);
             bool          first = true;
             double        acc   = 0.0;
-            for(int j0 = rs; j0 < re; j0 += 64)
+            for(int j0 = rs; j0 < re; j0 += G)
             {
-                const int j     = j0 + lane;
+                const int j     = j0 + gl;
                 bool      match = false;
                 double    value = 0.0;
                 if(j < re)
@@ -373,11 +287,13 @@ namespace
                         value = (c == i) ? 1.0 - relax : -relax * invd * val[j];
                     }
                 }
-                unsigned long long mask = __ballot(match);
+                unsigned long long mask = __ballot(match) >> gb;
+                if(G < 64)
+                    mask &= (1ull << G) - 1ull;
                 while(mask)
                 {
                     const int    t  = __ffsll((long long)mask) - 1;
-                    const double vt = __shfl(value, t, 64);
+                    const double vt = __shfl(value, gb + t, 64);
                     mask &= mask - 1;
                     if(first)
                     {
@@ -388,7 +304,7 @@ namespace
                         acc += vt;
                 }
             }
-            if(lane == 0)
+            if(gl == 0)
             {
                 pci[out] = (int)best;
                 pv[out]  = acc;
@@ -397,7 +313,7 @@ namespace
         }
     }
 
-    // Rows this long on average take the wavefront-per-row kernels.
+    // Rows this long on average take a wavefront per row, shorter ones 8 lanes.
     constexpr long PROLONG_WAVE_MIN_ROW = 24;
 }
 
@@ -737,12 +653,13 @@ namespace lal
                            const int* conn, const int64_t* agg, int* prp, int* nnz_P,
                            int* n_c)
     {
+        // count[m] = 0 is written by the group of row m, so m + 1 groups
         if(m > 0 && nnz >= PROLONG_WAVE_MIN_ROW * (long)m)
-            prolong_count_wave_kernel<<<(m + 1 + 3) / 4, 256, 0, c->stream>>>(m, rp, ci, conn,
-                                                                              agg, prp);
+            prolong_count_group_kernel<64><<<(m + 1 + 3) / 4, 256, 0, c->stream>>>(m, rp, ci,
+                                                                                   conn, agg, prp);
         else
-            prolong_count_kernel<<<grid_for(m + 1, 256), 256, 0, c->stream>>>(m, rp, ci, conn,
-                                                                              agg, prp);
+            prolong_count_group_kernel<8><<<(m + 1 + 31) / 32, 256, 0, c->stream>>>(m, rp, ci,
+                                                                                    conn, agg, prp);
         LAL_LAUNCHED();
         LAL_TRY(scan_i32_dev(c, m + 1, prp));
         int64_t mx = LLONG_MIN;
@@ -763,11 +680,11 @@ namespace lal
         if(m == 0)
             return LAL_OK;
         if(nnz >= PROLONG_WAVE_MIN_ROW * (long)m)
-            prolong_fill_wave_kernel<<<(m + 3) / 4, 256, 0, c->stream>>>(m, rp, ci, v, conn, agg,
-                                                                         relax, prp, pci, pv);
+            prolong_fill_group_kernel<64><<<(m + 3) / 4, 256, 0, c->stream>>>(
+                m, rp, ci, v, conn, agg, relax, prp, pci, pv);
         else
-            prolong_fill_kernel<<<grid_for(m, 256), 256, 0, c->stream>>>(m, rp, ci, v, conn, agg,
-                                                                         relax, prp, pci, pv);
+            prolong_fill_group_kernel<8><<<(m + 31) / 32, 256, 0, c->stream>>>(
+                m, rp, ci, v, conn, agg, relax, prp, pci, pv);
         LAL_LAUNCHED();
         return LAL_OK;
     }
