@@ -9,8 +9,8 @@
 // slots claimed with atomicCAS); rows beyond that get a table in global
 // memory.  Tables of up to 256 slots are built by H / 8 lanes per row (and
 // the symbolic pass's key-only tables of 512 and 1024 slots by 32), larger
-// ones by a wavefront per row.  The numeric pass walks the A row sequentially and lets the
-// lanes take the B row in parallel; the columns of one B row are distinct, so
+// ones by a wavefront per row.  The numeric pass walks the A row
+// sequentially and lets the lanes take the B row in parallel; the columns of one B row are distinct, so
 // every C entry receives at most one contribution per step and is folded in
 // A-row-then-B-row order from its first product -- the exact accumulation of
 // backend/host/host_csrgemm.cpp:91-183.  The table is then compacted and each
@@ -254,38 +254,20 @@ namespace
         }
     }
 
-    template <int H, int WPB, bool NUMERIC>
-    __global__ void __launch_bounds__(WPB * 64)
-        spgemm_lds_kernel(int nrows, const int* __restrict__ rows, const int* __restrict__ arp,
-                          const int* __restrict__ aci, const double* __restrict__ av,
-                          const int* __restrict__ brp, const int* __restrict__ bci,
-                          const double* __restrict__ bv, const int* __restrict__ crp,
-                          int* __restrict__ cout_, double* __restrict__ cval,
-                          int* __restrict__ err)
-    {
-        __shared__ int    s_keys[WPB][H];
-        __shared__ double s_vals[NUMERIC ? WPB : 1][NUMERIC ? H : 1];
-        const int         wid  = threadIdx.x >> 6;
-        const int         lane = threadIdx.x & 63;
-        const int         r    = blockIdx.x * WPB + wid;
-        if(r >= nrows)
-            return;
-        spgemm_row<NUMERIC, false>(rows[r], s_keys[wid], NUMERIC ? s_vals[wid] : nullptr, H, lane, arp,
-                            aci, av, brp, bci, bv, crp, cout_, cval, err);
-    }
-
-    // Rows of C with small tables: G lanes per row (H / 8 for tables of up to
-    // 256 slots, 32 for the symbolic pass's 512 and 1024), 256 / G rows per
-    // workgroup.  The steps and the accumulation order are those of
-    // spgemm_row -- A-row entries one after another, the lanes of the group
-    // over the entries of that entry's B row -- but the group first loads up
-    // to G entries of the A row and their B-row bounds at once, one entry per
-    // lane, and each step takes its bounds from the owning lane, so a step
-    // waits on one load (the B row) instead of three dependent ones.  Rows
-    // of P and A P hold a few to a few tens of entries, which a full
-    // wavefront per row would leave mostly idle.
-    template <int H, int G, bool NUMERIC>
-    __global__ void __launch_bounds__(256)
+    // Rows of C in LDS tables of H slots: G lanes per row, NT / G rows per
+    // workgroup (G = H / 8 up to 256 slots, 32 for the symbolic pass's 512-
+    // and 1024-slot key-only tables, a wavefront otherwise).  The steps and
+    // the accumulation order are those of spgemm_row -- A-row entries one
+    // after another, the lanes of the group over the entries of that entry's
+    // B row -- but the group first loads up to G entries of the A row and
+    // their B-row bounds at once, one entry per lane, and each step takes its
+    // bounds from the owning lane, so a step waits on one load (the B row)
+    // instead of three dependent ones; each lane's first entry of the next
+    // step's B row is loaded during the current step.  Rows of P and A P hold
+    // a few to a few tens of entries, which a full wavefront per row would
+    // leave mostly idle.
+    template <int H, int G, int NT, bool NUMERIC>
+    __global__ void __launch_bounds__(NT)
         spgemm_group_kernel(int nrows, const int* __restrict__ rows, const int* __restrict__ arp,
                             const int* __restrict__ aci, const double* __restrict__ av,
                             const int* __restrict__ brp, const int* __restrict__ bci,
@@ -293,8 +275,9 @@ namespace
                             int* __restrict__ cout_, double* __restrict__ cval,
                             int* __restrict__ err)
     {
-        static_assert(G >= 2 && G <= 32 && (G & (G - 1)) == 0 && H % G == 0);
-        constexpr int RPB   = 256 / G;
+        static_assert(G >= 2 && G <= 64 && (G & (G - 1)) == 0 && H % G == 0 && NT % G == 0);
+        constexpr int RPB   = NT / G;
+        constexpr unsigned long long GMASK = G == 64 ? ~0ull : (1ull << (G % 64)) - 1ull;
         constexpr int hmask = H - 1;
         __shared__ int    s_keys[RPB][H];
         __shared__ double s_vals[NUMERIC ? RPB : 1][NUMERIC ? H : 1];
@@ -399,7 +382,7 @@ namespace
             const int                k     = keys[s];
             const double             v     = vals[s];
             const bool               valid = k != -1;
-            const unsigned long long mask  = (__ballot(valid) >> gb) & ((1ull << G) - 1ull);
+            const unsigned long long mask  = (__ballot(valid) >> gb) & GMASK;
             wave_sync_mem<false>();
             if(valid)
             {
@@ -672,39 +655,37 @@ namespace
             if(nr == 0)
                 continue;
             const int* rows = P.d_rows + n0;
-#define LAL_SPGEMM_BIN(HH, WPB, NUM)                                                         \
-    spgemm_lds_kernel<HH, WPB, NUM><<<(nr + WPB - 1) / WPB, WPB * 64, 0, c->stream>>>(       \
-        nr, rows, arp, aci, av, brp, bci, bv, crp, cout_, cval, c->d_flag)
-#define LAL_SPGEMM_GROUP(HH, G)                                                              \
-    spgemm_group_kernel<HH, G, NUMERIC><<<(nr + 256 / G - 1) / (256 / G), 256, 0, c->stream>>>( \
+#define LAL_SPGEMM_GROUP(HH, G, NT, NUM)                                                     \
+    spgemm_group_kernel<HH, G, NT, NUM><<<(nr + NT / G - 1) / (NT / G), NT, 0, c->stream>>>(  \
         nr, rows, arp, aci, av, brp, bci, bv, crp, cout_, cval, c->d_flag)
             switch(b)
             {
-            case 0: LAL_SPGEMM_GROUP(64, 8); break;
-            case 1: LAL_SPGEMM_GROUP(128, 16); break;
-            case 2: LAL_SPGEMM_GROUP(256, 32); break;
-            // key-only tables of 512 and 1024 slots (products, not distinct
-            // columns, size them) in 32-lane groups; 8 rows hold 16 or 32 KB
+            case 0: LAL_SPGEMM_GROUP(64, 8, 256, NUMERIC); break;
+            case 1: LAL_SPGEMM_GROUP(128, 16, 256, NUMERIC); break;
+            case 2: LAL_SPGEMM_GROUP(256, 32, 256, NUMERIC); break;
+            // 512 and 1024 slots: a wavefront per row for the numeric pass
+            // (24 / 48 KB for four rows), 32 lanes for the symbolic pass's
+            // key-only tables (sized by products; 8 rows hold 16 / 32 KB)
             case 3:
                 if constexpr(NUMERIC)
-                    LAL_SPGEMM_BIN(512, 4, true);
+                    LAL_SPGEMM_GROUP(512, 64, 256, true);
                 else
-                    LAL_SPGEMM_GROUP(512, 32);
+                    LAL_SPGEMM_GROUP(512, 32, 256, false);
                 break;
             case 4:
                 if constexpr(NUMERIC)
-                    LAL_SPGEMM_BIN(1024, 4, true);
+                    LAL_SPGEMM_GROUP(1024, 64, 256, true);
                 else
-                    LAL_SPGEMM_GROUP(1024, 32);
+                    LAL_SPGEMM_GROUP(1024, 32, 256, false);
                 break;
-            case 5: LAL_SPGEMM_BIN(2048, 1, NUMERIC); break;
-            case 6: LAL_SPGEMM_BIN(4096, 1, NUMERIC); break;
+            // one row (a wavefront) per workgroup from 2048 slots on
+            case 5: LAL_SPGEMM_GROUP(2048, 64, 64, NUMERIC); break;
+            case 6: LAL_SPGEMM_GROUP(4096, 64, 64, NUMERIC); break;
             // key-only tables of the symbolic pass (the numeric pass never bins here)
-            case 7: LAL_SPGEMM_BIN(8192, 1, false); break;
-            case 8: LAL_SPGEMM_BIN(16384, 1, false); break;
-            case 9: LAL_SPGEMM_BIN(32768, 1, false); break;
+            case 7: LAL_SPGEMM_GROUP(8192, 64, 64, false); break;
+            case 8: LAL_SPGEMM_GROUP(16384, 64, 64, false); break;
+            case 9: LAL_SPGEMM_GROUP(32768, 64, 64, false); break;
             }
-#undef LAL_SPGEMM_BIN
 #undef LAL_SPGEMM_GROUP
             LAL_LAUNCHED();
         }
