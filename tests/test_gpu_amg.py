@@ -164,6 +164,28 @@ def test_spgemm_bins_and_global_tables(h, oracle):
     assert csr_equal(_gpu_spgemm(h, A2, B2), oracle.csrgemm(A2, B2))
 
 
+def test_spgemm_lane_groups_with_long_rows(h, oracle):
+    """Rows of C whose tables are small but whose A and B rows are long: the
+    lane groups (8 lanes for 64-slot tables, 16 for 128) then take several
+    chunks of the A row and several passes over each B row.  C has 12 or 48
+    columns, so the tables stay at 64 or 128 slots in both passes, while A
+    rows hold 20-40 entries and B rows 10 or 20 (more than the group's 8 or
+    16 lanes); bit for bit against the oracle."""
+    rng = np.random.default_rng(23)
+    for n, blen in ((12, 10), (48, 20)):
+        m, k = 500, 300
+        lens = rng.integers(20, 41, m)
+        rows = [np.sort(rng.choice(k, size=x, replace=False)) for x in lens]
+        rp = np.zeros(m + 1, np.int32)
+        rp[1:] = np.cumsum(lens)
+        A = CSR(rp, np.concatenate(rows).astype(np.int32), rng.uniform(-1, 1, rp[-1]), m, k)
+        brows = [np.sort(rng.choice(n, size=blen, replace=False)) for _ in range(k)]
+        brp = np.arange(k + 1, dtype=np.int32) * blen
+        B = CSR(brp, np.concatenate(brows).astype(np.int32), rng.uniform(-1, 1, k * blen), k, n)
+        log(f"spgemm lane groups: {n} columns, B rows of {blen}")
+        assert csr_equal(_gpu_spgemm(h, A, B), oracle.csrgemm(A, B))
+
+
 def test_spgemm_refuses_inconsistent_row_pointer(h, oracle):
     """The numeric pass sizes its tables by the row pointer of C; one that
     does not match the product's pattern (an entry moved between two rows, or
