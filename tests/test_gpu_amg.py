@@ -37,6 +37,14 @@ def csr_equal(got, ref):
 def test_strength_aggregates_prolongator(h, oracle):
     cases = [("diff12", grid("diff12")), ("poisson3d10", pb.poisson3d(10)),
              ("poisson2d33", pb.poisson2d(33))]
+    # the oracle's own Galerkin operators of 3D Poisson 24^3: 36 and 90
+    # entries per row, which the setup passes walk with 16 and 64 lanes per
+    # row instead of one
+    rp0, ci0, v0, m0 = pb.poisson3d(24)
+    H = oracle.saamg_setup(CSR(rp0, ci0, v0, m0, m0))
+    for lvl in (1, 2):
+        Ag = H.get(lvl, "A")
+        cases.append((f"galerkin{lvl}", (Ag.row_ptr, Ag.col_ind, Ag.val, Ag.m)))
     for name, (rp, ci, v, m) in cases:
         A = CSR(rp, ci, v, m, m)
         d = DevCSR(A)
