@@ -73,6 +73,11 @@ namespace
         }
     }
 
+    // PASS: decided rows also copy their state into max_state, so that the
+    // setup's own loop can alternate two buffers instead of copying state
+    // from max_state before every round (the C ABI round leaves them alone,
+    // as the reference's kernel does).
+    template <bool PASS>
     __global__ void __launch_bounds__(256)
         pmis_round_kernel(int m, const int* __restrict__ rp, const int* __restrict__ ci,
                           const int* __restrict__ conn, const int* __restrict__ state,
@@ -84,7 +89,11 @@ namespace
             return;
         const int si = state[i];
         if(si != 0)
+        {
+            if(PASS)
+                max_state[i] = si;
             return; // only undecided rows change anything
+        }
         int ms = si, mh = hash[i], mr = i;
         for(int j = rp[i]; j < rp[i + 1]; j++)
             if(conn[j] == 1)
@@ -461,7 +470,7 @@ namespace
     // pmis_round_kernel with G lanes per row: the running maximum starts at
     // the row itself, takes the row's strong neighbours in order, then the
     // strong neighbours of the row that maximum names.
-    template <int G>
+    template <int G, bool PASS>
     __global__ void __launch_bounds__(256)
         pmis_round_group_kernel(int m, const int* __restrict__ rp, const int* __restrict__ ci,
                                 const int* __restrict__ conn, const int* __restrict__ state,
@@ -474,7 +483,11 @@ namespace
             return;
         const int si = state[i];
         if(si != 0)
+        {
+            if(PASS && gl == 0)
+                max_state[i] = si;
             return; // only undecided rows change anything
+        }
         // the row itself precedes its neighbours, so it wins a tie
         pmis_cand run{si, hash[i], i, -1};
         pmis_cand c1 = pmis_scan<G>(i, gl, rp, ci, conn, state, hash);
@@ -606,27 +619,49 @@ namespace lal
         else
             pmis_init_kernel<<<g, 256, 0, c->stream>>>(m, rp, conn, max_state, hash);
         LAL_LAUNCHED();
-        for(int iter = 0; iter < 20; iter++)
-        {
-            LAL_HIP(hipMemcpyAsync(state, max_state, sizeof(int) * m, hipMemcpyDeviceToDevice,
-                                   c->stream));
-            LAL_HIP(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
+        // The reference's loop (amg_aggregation.cpp:106-215): up to 20 rounds,
+        // state = max_state before each, stop after the first complete one.
+        // Here the rounds alternate between two buffers (the round kernel
+        // writes every decided row through, and a row still undecided was
+        // undecided in the older buffer too, so it reads 0 there as in the
+        // copy), and the host checks after every second round: a round after
+        // a complete one finds no undecided row and changes nothing, so the
+        // aggregates are those of the reference's loop.
+        LAL_HIP(hipMemcpyAsync(state, max_state, sizeof(int) * m, hipMemcpyDeviceToDevice,
+                               c->stream));
+        int* buf[2] = {max_state, state}; // buf[cur]: the latest state
+        int  cur    = 0;
+        auto round  = [&](int* flag) -> int {
+            const int* in  = buf[cur];
+            int*       out = buf[cur ^ 1];
             if(lanes == 64)
-                pmis_round_group_kernel<64><<<gg, 256, 0, c->stream>>>(
-                    m, rp, ci, conn, state, hash, agg, max_state, c->d_flag);
+                pmis_round_group_kernel<64, true><<<gg, 256, 0, c->stream>>>(
+                    m, rp, ci, conn, in, hash, agg, out, flag);
             else if(lanes == 16)
-                pmis_round_group_kernel<16><<<gg, 256, 0, c->stream>>>(
-                    m, rp, ci, conn, state, hash, agg, max_state, c->d_flag);
+                pmis_round_group_kernel<16, true><<<gg, 256, 0, c->stream>>>(
+                    m, rp, ci, conn, in, hash, agg, out, flag);
             else
-                pmis_round_kernel<<<g, 256, 0, c->stream>>>(m, rp, ci, conn, state, hash, agg,
-                                                            max_state, c->d_flag);
+                pmis_round_kernel<true><<<g, 256, 0, c->stream>>>(m, rp, ci, conn, in, hash, agg,
+                                                                  out, flag);
             LAL_LAUNCHED();
-            LAL_HIP(hipMemcpyAsync(c->h_flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost,
+            cur ^= 1;
+            return LAL_OK;
+        };
+        for(int iter = 0; iter < 20; iter += 2)
+        {
+            LAL_HIP(hipMemsetAsync(c->d_flag, 0, 2 * sizeof(int), c->stream));
+            LAL_TRY(round(c->d_flag));
+            LAL_TRY(round(c->d_flag + 1));
+            LAL_HIP(hipMemcpyAsync(c->h_flag, c->d_flag + 1, sizeof(int), hipMemcpyDeviceToHost,
                                    c->stream));
             LAL_HIP(hipStreamSynchronize(c->stream));
             if(c->h_flag[0] == 0)
                 break;
         }
+        // the joins below read max_state: the latest state must be there
+        if(buf[cur] != max_state)
+            LAL_HIP(hipMemcpyAsync(max_state, buf[cur], sizeof(int) * m, hipMemcpyDeviceToDevice,
+                                   c->stream));
         LAL_TRY(scan_i64_dev(c, m, agg));
         for(int k = 0; k < 2; k++)
         {
@@ -783,8 +818,8 @@ extern "C" int lal_pmis_round(lal_handle c, int m, int nnz, const int* rp, const
     if(m == 0)
         return LAL_OK;
     LAL_HIP(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
-    pmis_round_kernel<<<lal::grid_for(m, 256), 256, 0, c->stream>>>(m, rp, ci, conn, state, hash,
-                                                                    agg, max_state, c->d_flag);
+    pmis_round_kernel<false><<<lal::grid_for(m, 256), 256, 0, c->stream>>>(
+        m, rp, ci, conn, state, hash, agg, max_state, c->d_flag);
     LAL_LAUNCHED();
     LAL_HIP(hipMemcpyAsync(c->h_flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     LAL_HIP(hipStreamSynchronize(c->stream));
