@@ -36,8 +36,9 @@ namespace lal
                             double* pv);
     int transpose_f64(lal_context* c, int m, int n, int nnz, const int* rp, const int* ci,
                       const double* v, int* trp, int* tci, double* tv);
+    // nnz_a: A's nonzeros, which pick the lanes per row of the product count
     int csrgemm_nnz(lal_context* c, int m, int k, int n, const int* arp, const int* aci,
-                    const int* brp, const int* bci, int* crp, int* nnz_c);
+                    const int* brp, const int* bci, int* crp, int* nnz_c, long nnz_a = 0);
     int csrgemm_solve_f64(lal_context* c, int m, int k, int n, const int* arp, const int* aci,
                           const double* av, const int* brp, const int* bci, const double* bv,
                           const int* crp, int* cci, double* cv);
@@ -493,7 +494,7 @@ namespace
             AP.n = nc;
             LAL_TRY(alloc_n(&AP.rp, Af.m + 1));
             LAL_TRY(lal::csrgemm_nnz(c, Af.m, Af.n, nc, Af.rp, Af.ci, P.rp, P.ci, AP.rp,
-                                     &AP.nnz));
+                                     &AP.nnz, Af.nnz));
             LAL_TRY(alloc_n(&AP.ci, AP.nnz));
             LAL_TRY(alloc_n(&AP.v, AP.nnz));
             LAL_TRY(lal::csrgemm_solve_f64(c, Af.m, Af.n, nc, Af.rp, Af.ci, Af.v, P.rp, P.ci,
@@ -502,7 +503,8 @@ namespace
             Ac.m = nc;
             Ac.n = nc;
             LAL_TRY(alloc_n(&Ac.rp, nc + 1));
-            LAL_TRY(lal::csrgemm_nnz(c, nc, Af.m, nc, R.rp, R.ci, AP.rp, AP.ci, Ac.rp, &Ac.nnz));
+            LAL_TRY(lal::csrgemm_nnz(c, nc, Af.m, nc, R.rp, R.ci, AP.rp, AP.ci, Ac.rp, &Ac.nnz,
+                                     R.nnz));
             LAL_TRY(alloc_n(&Ac.ci, Ac.nnz));
             LAL_TRY(alloc_n(&Ac.v, Ac.nnz));
             LAL_TRY(lal::csrgemm_solve_f64(c, nc, Af.m, nc, R.rp, R.ci, R.v, AP.rp, AP.ci, AP.v,

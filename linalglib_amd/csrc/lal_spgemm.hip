@@ -131,6 +131,31 @@ namespace
         prod[i] = (int)min(p, (long long)INT_MAX);
     }
 
+    // products_kernel with G lanes per row, for long rows of A: the lanes
+    // sum strided parts of the row, the group adds them up (integers: the
+    // same count).
+    template <int G>
+    __global__ void __launch_bounds__(256)
+        products_group_kernel(int m, const int* __restrict__ arp, const int* __restrict__ aci,
+                              const int* __restrict__ brp, int* __restrict__ prod)
+    {
+        const int gl = (int)threadIdx.x % G;
+        const int i  = blockIdx.x * (256 / G) + (int)threadIdx.x / G;
+        if(i >= m)
+            return;
+        long long p = 0;
+        for(int j = arp[i] + gl; j < arp[i + 1]; j += G)
+        {
+            int c = aci[j];
+            p += brp[c + 1] - brp[c];
+        }
+#pragma unroll
+        for(int o = G / 2; o > 0; o >>= 1)
+            p += __shfl_xor(p, o, 64);
+        if(gl == 0)
+            prod[i] = (int)min(p, (long long)INT_MAX);
+    }
+
     __device__ __forceinline__ int slot_of(int col, int hmask)
     {
         return (int)(((unsigned)col * 0x9E3779B1u) >> 7) & hmask;
@@ -568,7 +593,7 @@ namespace
     // eleven bin counts (and, when some rows need global tables, their table
     // sizes) to size the launches and the global tables.
     int make_plan(lal_context* c, int m, int n, const int* arp, const int* aci, const int* brp,
-                  const int* crp, bool numeric, spgemm_plan& P)
+                  const int* crp, bool numeric, spgemm_plan& P, long nnz_a = 0)
     {
         constexpr int nb = kBins + 1;
         static_assert(nb <= 16, "the bin counts are read back through the context's flags");
@@ -585,7 +610,12 @@ namespace
         if(!numeric)
         {
             LAL_HIP(hipMallocAsync((void**)&d_prod, sizeof(int) * m, c->stream));
-            products_kernel<<<grid, 256, 0, c->stream>>>(m, arp, aci, brp, d_prod);
+            // rows of A averaging 16 or more entries: 16 lanes per row
+            if(nnz_a >= 16L * m)
+                products_group_kernel<16><<<(int)(((long)m * 16 + 255) / 256), 256, 0,
+                                            c->stream>>>(m, arp, aci, brp, d_prod);
+            else
+                products_kernel<<<grid, 256, 0, c->stream>>>(m, arp, aci, brp, d_prod);
             LAL_LAUNCHED();
         }
         LAL_HIP(hipMallocAsync((void**)&d_bins, sizeof(int) * 2 * nb, c->stream));
@@ -734,12 +764,12 @@ namespace lal
     }
 
     int csrgemm_nnz(lal_context* c, int m, int k, int n, const int* arp, const int* aci,
-                    const int* brp, const int* bci, int* crp, int* nnz_c)
+                    const int* brp, const int* bci, int* crp, int* nnz_c, long nnz_a)
     {
         (void)k;
         LAL_HIP(hipMemsetAsync(crp, 0, sizeof(int) * (m + 1), c->stream));
         spgemm_plan P;
-        LAL_TRY(make_plan(c, m, n, arp, aci, brp, nullptr, false, P));
+        LAL_TRY(make_plan(c, m, n, arp, aci, brp, nullptr, false, P, nnz_a));
         int st = run_plan<false>(c, P, arp, aci, nullptr, brp, bci, nullptr, nullptr, crp,
                                  nullptr);
         P.release(c->stream);
@@ -853,7 +883,7 @@ extern "C" int lal_csrgemm_nnz(lal_handle c, int m, int k, int n, int nnz_A, con
     LAL_REQUIRE(crp != nullptr && nnz_C != nullptr && arp != nullptr && brp != nullptr);
     LAL_REQUIRE((nnz_A == 0 || aci) && (nnz_B == 0 || bci));
     LAL_TRY(check_product_operands(c, k, n, nnz_A, aci, nnz_B, brp, bci));
-    return lal::csrgemm_nnz(c, m, k, n, arp, aci, brp, bci, crp, nnz_C);
+    return lal::csrgemm_nnz(c, m, k, n, arp, aci, brp, bci, crp, nnz_C, nnz_A);
 }
 
 extern "C" int lal_csrgemm_solve_f64(lal_handle c, int m, int k, int n, int nnz_A,
