@@ -7,6 +7,7 @@
 // lal_spmv.h.
 #include "lal_rowvec.h"
 
+#include <algorithm>
 #include <climits>
 
 #include <vector>
@@ -205,10 +206,13 @@ namespace lal
         int r = 0;
         while(r < m)
         {
-            int start = r;
-            int nz0   = h_rp[r];
-            while(r < m && r - start < SPMV_THREADS && h_rp[r + 1] - nz0 <= cap)
-                r++;
+            // the block ends before the first row r (at most SPMV_THREADS rows
+            // on) with row_ptr[r + 1] - row_ptr[start] > cap; row_ptr is
+            // non-decreasing, so that row is a binary search away
+            const int       start = r;
+            const int       lim   = std::min(m, start + SPMV_THREADS);
+            const long long top   = (long long)h_rp[start] + cap;
+            r = (int)(std::upper_bound(h_rp + start + 1, h_rp + lim + 1, top) - h_rp) - 1;
             if(r == start)
                 r++;
             blocks.push_back(r);
@@ -251,7 +255,8 @@ namespace lal
             set_last_error("row_ptr does not start at 0 or end at nnz");
             return LAL_ERR_INVALID;
         }
-        for(int i = 0; i < m; i++)
+        // (a pattern this library built is monotone by construction)
+        for(int i = 0; i < m && !trusted; i++)
             if(h[i + 1] < h[i])
             {
                 set_last_error("row_ptr is not monotone");
